@@ -1,0 +1,158 @@
+// cb_device.hpp -- device helpers shared by the kernel files: block
+// reductions, compensated sums, lattice neighbours and the per-site energy
+// terms of measureEnergies.
+#pragma once
+
+#include "cb_common.hpp"
+
+namespace cb {
+
+// ---------------------------------------------------------------------------
+// Reductions: 64-lane wavefront shuffles, then the waves of the block in
+// order through LDS.  Deterministic for a fixed launch shape.
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ inline void block_partials(double (&acc)[NV], double *out)
+{
+    __shared__ double sm[NV][16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double v = acc[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) sm[q][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+        for (int i = 0; i < nw; ++i) s += sm[threadIdx.x][i];
+        out[threadIdx.x] = s;
+    }
+}
+
+// Neumaier-compensated running sum.
+struct Neu {
+    double s, c;
+    __device__ void add(double x)
+    {
+        double t = s + x;
+        if (fabs(s) >= fabs(x)) c += (s - t) + x;
+        else c += (x - t) + s;
+        s = t;
+    }
+    __device__ void merge(Neu o)
+    {
+        add(o.s);
+        c += o.c;
+    }
+    __device__ double value() const { return s + c; }
+};
+
+// Sum nblocks x NQ partials (row-major) into tot[NQ]; one block of 256.
+__device__ inline void sum_partials_block(const double *partials, int nblocks, double *tot)
+{
+    __shared__ double ss[NQ][256];
+    __shared__ double sc[NQ][256];
+    const int t = threadIdx.x;
+    Neu n[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) n[q] = {0.0, 0.0};
+    for (int b = t; b < nblocks; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) n[q].add(partials[(long long)b * NQ + q]);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) { ss[q][t] = n[q].s; sc[q][t] = n[q].c; }
+    __syncthreads();
+    for (int half = blockDim.x >> 1; half > 0; half >>= 1) {
+        if (t < half) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                Neu a = {ss[q][t], sc[q][t]};
+                a.merge(Neu{ss[q][t + half], sc[q][t + half]});
+                ss[q][t] = a.s;
+                sc[q][t] = a.c;
+            }
+        }
+        __syncthreads();
+    }
+    if (t < NQ) tot[t] = ss[t][0] + sc[t][0];
+}
+
+// ---------------------------------------------------------------------------
+// Lattice neighbours, src/gridfuncs.cpp:31-51: nb = {P0, M0, P1, M1, P2, M2},
+// axis 0 slowest.  In 1D nb[0] is the next point and nb[1] the previous one.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ inline void neighbours(long long s, int N, long long (&nb)[6])
+{
+    if constexpr (D == 1) {
+        int i = (int)s;
+        nb[0] = (i + 1 == N) ? 0 : i + 1;
+        nb[1] = (i == 0) ? N - 1 : i - 1;
+    } else if constexpr (D == 2) {
+        long long i = s / N;
+        int j = (int)(s - i * N);
+        long long ip = (i + 1 == N) ? 0 : i + 1, im = (i == 0) ? N - 1 : i - 1;
+        int jp = (j + 1 == N) ? 0 : j + 1, jm = (j == 0) ? N - 1 : j - 1;
+        nb[0] = ip * N + j;
+        nb[1] = im * N + j;
+        nb[2] = i * N + jp;
+        nb[3] = i * N + jm;
+    } else {
+        long long NN = (long long)N * N;
+        long long i = s / NN;
+        long long r = s - i * NN;
+        int j = (int)(r / N);
+        int kk = (int)(r - (long long)j * N);
+        long long ip = (i + 1 == N) ? 0 : i + 1, im = (i == 0) ? N - 1 : i - 1;
+        int jp = (j + 1 == N) ? 0 : j + 1, jm = (j == 0) ? N - 1 : j - 1;
+        int kp = (kk + 1 == N) ? 0 : kk + 1, km = (kk == 0) ? N - 1 : kk - 1;
+        long long base = (long long)j * N + kk;
+        nb[0] = ip * NN + base;
+        nb[1] = im * NN + base;
+        nb[2] = i * NN + (long long)jp * N + kk;
+        nb[3] = i * NN + (long long)jm * N + kk;
+        nb[4] = i * NN + (long long)j * N + kp;
+        nb[5] = i * NN + (long long)j * N + km;
+    }
+}
+
+// Energy terms of one site (measureEnergies, src/bubble.cpp:790-827), in
+// double.  gP = {forward neighbour along axis 1, 0, 2} per the reference's
+// gx, gy, gz naming (1D: gx only).
+template <int D>
+__device__ inline void energy_terms(double ph, const double (&fwd)[3], double pn, double po,
+                                    const SimConst &k, const DevScalars *sc, double (&acc)[NQ])
+{
+    const double adx = sc->adx, pk2 = sc->pk2, pk1 = sc->pk1, dxLD = k.dxLD;
+    double gx = 0., gy = 0., gz = 0.;
+    if constexpr (D == 1) {
+        gx = (fwd[0] - ph) / adx;
+    } else {
+        gx = (fwd[1] - ph) / adx;
+        gy = (fwd[0] - ph) / adx;
+        if constexpr (D == 3) gz = (fwd[2] - ph) / adx;
+    }
+    double gx2 = gx * gx, gy2 = gy * gy, gz2 = gz * gz;
+    double Eg = 0.5 * (gx2 + gy2 + gz2);
+    double Ep = vpot(ph, k);
+    double Ek = 0.5 * pk2 * pn * pn;
+    double Et = Ek + Eg + Ep;
+    double pp = pn * pk1;
+    acc[Q_KIN] += Ek * dxLD;
+    acc[Q_POT] += Ep * dxLD;
+    acc[Q_GRAD] += Eg * dxLD;
+    acc[Q_PHI] += ph * dxLD;
+    acc[Q_PHIP] += pp * dxLD;
+    acc[Q_VKIN] += Ek * dxLD * Ek * dxLD;
+    acc[Q_VPOT] += Ep * dxLD * Ep * dxLD;
+    acc[Q_VGRAD] += Eg * dxLD * Eg * dxLD;
+    acc[Q_VTOT] += Et * dxLD * Et * dxLD;
+    acc[Q_VPHI] += ph * dxLD * ph * dxLD;
+    acc[Q_VPHIP] += pn * pk1 * dxLD * pn * pk1 * dxLD;
+    acc[Q_KIN_OLD] += 0.5 * pk2 * po * po * dxLD;
+}
+
+} // namespace cb

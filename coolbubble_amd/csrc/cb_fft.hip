@@ -1,0 +1,637 @@
+// cb_fft.hip -- power spectrum and thermal initial conditions on gfx950:
+// a hand-written Stockham FFT in LDS (no FFT library), radial k-binning, and
+// the seeded Gaussian random field of mode -3.
+//
+// Reference: FourierTransform + ComputePowerSpectrum (src/bubble.cpp:642-709)
+// use an FFTW r2c of phi (FFTW_FORWARD, unnormalised) into the half-complex
+// layout (rows = N^(D-1), N/2+1 columns), then bin |phi_k|^2 in shells of
+// width 2 pi / L.  GenerateThermalFluc (src/bubble.cpp:459-523) fills that
+// layout with mt19937_64 + normal_distribution draws, enforces Hermitian
+// symmetry on the planes j = 0 and j = N/2 and runs an FFTW c2r.
+//
+// Transforms here:
+//   rows   : length-N transform of each contiguous row, r2c or c2r
+//   columns: length-N transform along a strided axis of the half-complex
+//            array, CB adjacent columns per block so every row segment is a
+//            coalesced load
+// each as a radix-4/2/3 (+ any prime) Stockham sweep between two LDS
+// buffers with twiddles from a table computed in long double on the host.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cb_device.hpp"
+#include "cb_sim.hpp"
+
+namespace cb {
+
+template <typename R>
+struct cpx {
+    R x, y;
+};
+
+static constexpr int kMaxStages = 40;
+static constexpr size_t kLdsBudget = 65536;
+
+struct Stages {
+    int n;
+    int ns;
+    int radix[kMaxStages];
+};
+
+// ---------------------------------------------------------------------------
+// Device: batched Stockham FFT in LDS.  Sequence b, element e at
+// buf[b * bs + e * es].  sign = -1 forward, +1 backward; tw[e] =
+// exp(-2 pi i e / n).  Returns the buffer holding the result.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ inline cpx<R> cmul(cpx<R> a, cpx<R> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+template <typename R>
+__device__ inline cpx<R> twid(const cpx<R> *tw, int e, int n, int sign)
+{
+    e %= n;
+    cpx<R> w = tw[e];
+    if (sign > 0) w.y = -w.y;
+    return w;
+}
+
+template <typename R>
+__device__ cpx<R> *lds_fft(cpx<R> *A, cpx<R> *B, int batch, int bs, int es, const Stages &st,
+                           const cpx<R> *__restrict__ tw, int sign)
+{
+    const int n = st.n;
+    int Ns = 1;
+    for (int s = 0; s < st.ns; ++s) {
+        const int Rd = st.radix[s];
+        const int m = n / Rd;
+        const int total = batch * m;
+        const int tstep = n / (Ns * Rd);      // twiddle index step for angle 1/(Ns*Rd)
+        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+            const int b = idx / m;
+            const int j = idx - b * m;
+            const int k = j % Ns;
+            const cpx<R> *in = A + b * bs;
+            cpx<R> *out = B + b * bs;
+            const int dst = (j / Ns) * Ns * Rd + k;
+            if (Rd == 2 || Rd == 4 || Rd == 3) {
+                cpx<R> v[4];
+                for (int r = 0; r < Rd; ++r) {
+                    cpx<R> a = in[(j + r * m) * es];
+                    if (r > 0 && k > 0) a = cmul(a, twid(tw, r * k * tstep, n, sign));
+                    v[r] = a;
+                }
+                if (Rd == 2) {
+                    out[dst * es] = {v[0].x + v[1].x, v[0].y + v[1].y};
+                    out[(dst + Ns) * es] = {v[0].x - v[1].x, v[0].y - v[1].y};
+                } else if (Rd == 4) {
+                    cpx<R> a0 = {v[0].x + v[2].x, v[0].y + v[2].y};
+                    cpx<R> a1 = {v[0].x - v[2].x, v[0].y - v[2].y};
+                    cpx<R> b0 = {v[1].x + v[3].x, v[1].y + v[3].y};
+                    cpx<R> b1 = {v[1].x - v[3].x, v[1].y - v[3].y};
+                    // (-sign i) * b1 : forward multiplies by -i
+                    cpx<R> rb1 = (sign < 0) ? cpx<R>{b1.y, -b1.x} : cpx<R>{-b1.y, b1.x};
+                    out[dst * es] = {a0.x + b0.x, a0.y + b0.y};
+                    out[(dst + Ns) * es] = {a1.x + rb1.x, a1.y + rb1.y};
+                    out[(dst + 2 * Ns) * es] = {a0.x - b0.x, a0.y - b0.y};
+                    out[(dst + 3 * Ns) * es] = {a1.x - rb1.x, a1.y - rb1.y};
+                } else {
+                    const R h = (R)0.86602540378443864676;   // sqrt(3)/2
+                    cpx<R> t1 = {v[1].x + v[2].x, v[1].y + v[2].y};
+                    cpx<R> t2 = {v[0].x - (R)0.5 * t1.x, v[0].y - (R)0.5 * t1.y};
+                    cpx<R> d = {v[1].x - v[2].x, v[1].y - v[2].y};
+                    // (-sign i) * h * d
+                    cpx<R> t3 = (sign < 0) ? cpx<R>{h * d.y, -h * d.x} : cpx<R>{-h * d.y, h * d.x};
+                    out[dst * es] = {v[0].x + t1.x, v[0].y + t1.y};
+                    out[(dst + Ns) * es] = {t2.x + t3.x, t2.y + t3.y};
+                    out[(dst + 2 * Ns) * es] = {t2.x - t3.x, t2.y - t3.y};
+                }
+            } else {
+                // any radix: direct length-Rd DFT of the twiddled inputs
+                for (int q = 0; q < Rd; ++q) {
+                    R sx = 0, sy = 0;
+                    for (int r = 0; r < Rd; ++r) {
+                        cpx<R> a = in[(j + r * m) * es];
+                        if (r > 0 && k > 0) a = cmul(a, twid(tw, r * k * tstep, n, sign));
+                        cpx<R> w = twid(tw, ((r * q) % Rd) * m, n, sign);
+                        cpx<R> p = cmul(a, w);
+                        sx += p.x;
+                        sy += p.y;
+                    }
+                    out[(dst + q * Ns) * es] = {sx, sy};
+                }
+            }
+        }
+        __syncthreads();
+        cpx<R> *t = A;
+        A = B;
+        B = t;
+        Ns *= Rd;
+    }
+    return A;
+}
+
+// r2c of RB contiguous rows of length N: real field -> half-complex rows.
+template <typename T, typename R>
+__global__ void __launch_bounds__(256)
+k_rows_r2c(const T *__restrict__ field, cpx<R> *__restrict__ H, long long rows, int RB, Stages st,
+           const cpx<R> *__restrict__ tw)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int N = st.n, nc = N / 2 + 1;
+    cpx<R> *A = reinterpret_cast<cpx<R> *>(smem_raw);
+    cpx<R> *B = A + (size_t)RB * N;
+    const long long r0 = (long long)blockIdx.x * RB;
+    const int nr = (int)((rows - r0) < RB ? (rows - r0) : RB);
+    const T *src = field + r0 * N;
+    for (int idx = threadIdx.x; idx < RB * N; idx += blockDim.x)
+        A[idx] = (idx < nr * N) ? cpx<R>{(R)src[idx], (R)0} : cpx<R>{(R)0, (R)0};
+    __syncthreads();
+    cpx<R> *res = lds_fft<R>(A, B, RB, N, 1, st, tw, -1);
+    cpx<R> *dst = H + r0 * nc;
+    for (int idx = threadIdx.x; idx < nr * nc; idx += blockDim.x) {
+        const int rr = idx / nc, j = idx - rr * nc;
+        dst[idx] = res[rr * N + j];
+    }
+}
+
+// c2r of RB rows: Hermitian-extended row, backward transform, real part,
+// scaled (src/bubble.cpp:515-519): field = x * norm + shift.
+template <typename T, typename R>
+__global__ void __launch_bounds__(256)
+k_rows_c2r(const cpx<R> *__restrict__ H, T *__restrict__ field, long long rows, int RB, Stages st,
+           const cpx<R> *__restrict__ tw, double norm, double shift)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int N = st.n, nc = N / 2 + 1;
+    cpx<R> *A = reinterpret_cast<cpx<R> *>(smem_raw);
+    cpx<R> *B = A + (size_t)RB * N;
+    const long long r0 = (long long)blockIdx.x * RB;
+    const int nr = (int)((rows - r0) < RB ? (rows - r0) : RB);
+    for (int idx = threadIdx.x; idx < RB * N; idx += blockDim.x) {
+        const int rr = idx / N, e = idx - rr * N;
+        cpx<R> v = {(R)0, (R)0};
+        if (rr < nr) {
+            const cpx<R> *row = H + (r0 + rr) * nc;
+            if (e == 0) v = {row[0].x, (R)0};
+            else if (2 * e < N) v = row[e];
+            else if (2 * e == N) v = {row[e].x, (R)0};
+            else v = {row[N - e].x, -row[N - e].y};
+        }
+        A[idx] = v;
+    }
+    __syncthreads();
+    cpx<R> *res = lds_fft<R>(A, B, RB, N, 1, st, tw, +1);
+    T *dst = field + r0 * N;
+    for (int idx = threadIdx.x; idx < nr * N; idx += blockDim.x)
+        dst[idx] = (T)((double)res[idx].x * norm + shift);
+}
+
+// Transform along a strided axis: array [outer][len][inner], CB adjacent
+// columns of one outer index per block, in place.
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_cols(cpx<R> *H, long long inner, int CB, Stages st, const cpx<R> *__restrict__ tw, int sign)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int len = st.n;
+    cpx<R> *A = reinterpret_cast<cpx<R> *>(smem_raw);
+    cpx<R> *B = A + (size_t)CB * len;
+    const long long tiles = (inner + CB - 1) / CB;
+    const long long o = blockIdx.x / tiles;
+    const long long c0 = (blockIdx.x - o * tiles) * CB;
+    const int nc = (int)((inner - c0) < CB ? (inner - c0) : CB);
+    cpx<R> *base = H + o * (long long)len * inner + c0;
+    for (int idx = threadIdx.x; idx < len * CB; idx += blockDim.x) {
+        const int e = idx / CB, cc = idx - e * CB;
+        A[cc * len + e] = (cc < nc) ? base[(long long)e * inner + cc] : cpx<R>{(R)0, (R)0};
+    }
+    __syncthreads();
+    cpx<R> *res = lds_fft<R>(A, B, CB, len, 1, st, tw, sign);
+    for (int idx = threadIdx.x; idx < len * CB; idx += blockDim.x) {
+        const int e = idx / CB, cc = idx - e * CB;
+        if (cc < nc) base[(long long)e * inner + cc] = res[cc * len + e];
+    }
+}
+
+// Radial binning of the half-complex spectrum, ComputePowerSpectrum
+// (src/bubble.cpp:668-709): per block an LDS histogram of (power, weight,
+// weighted k) over a run of rows, written out as one partial per block.
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_bin(const cpx<R> *__restrict__ H, long long rows, int N, int D, int nBins, double dk, double pref,
+      double Pnorm, double *__restrict__ binpart)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *hp = reinterpret_cast<double *>(smem_raw);
+    double *hw = hp + nBins;
+    double *hk = hw + nBins;
+    for (int b = threadIdx.x; b < 3 * nBins; b += blockDim.x) hp[b] = 0.0;
+    __syncthreads();
+    const int nc = N / 2 + 1;
+    const long long total = rows * nc;
+    const long long per = (total + gridDim.x - 1) / gridDim.x;
+    const long long m0 = per * blockIdx.x;
+    const long long m1 = (m0 + per < total) ? m0 + per : total;
+    for (long long m = m0 + threadIdx.x; m < m1; m += blockDim.x) {
+        const long long r = m / nc;
+        const int j = (int)(m - r * nc);
+        double n2rows = 0.;
+        if (D == 2) {
+            const int s = (r <= N / 2) ? (int)r : (int)r - N;
+            n2rows = (double)s * (double)s;
+        } else if (D == 3) {
+            const int a = (int)(r / N), c = (int)(r % N);
+            const int s1 = (a <= N / 2) ? a : a - N, s2 = (c <= N / 2) ? c : c - N;
+            n2rows = (double)s1 * (double)s1 + (double)s2 * (double)s2;
+        }
+        const double n = sqrt(n2rows + (double)j * j);
+        const int bin = (int)n;
+        if (bin >= nBins) continue;
+        const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
+        const cpx<R> f = H[m];
+        const double kc = n * dk;
+        const double fx = (double)f.x, fy = (double)f.y;
+        atomicAdd(&hp[bin], w * pref * ipow_rn(kc, D) * Pnorm * (fx * fx + fy * fy));
+        atomicAdd(&hw[bin], w);
+        atomicAdd(&hk[bin], w * kc);
+    }
+    __syncthreads();
+    double *dst = binpart + (long long)blockIdx.x * 3 * nBins;
+    for (int b = threadIdx.x; b < 3 * nBins; b += blockDim.x) dst[b] = hp[b];
+}
+
+// Fixed-order sum of the per-block bin partials.
+__global__ void __launch_bounds__(256)
+k_bin_reduce(const double *__restrict__ binpart, int nblk, int nq, double *__restrict__ out)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    Neu s = {0.0, 0.0};
+    for (int b = 0; b < nblk; ++b) s.add(binpart[(long long)b * nq + q]);
+    out[q] = s.value();
+}
+
+// ---------------------------------------------------------------------------
+// Thermal modes (src/bubble.cpp:469-488): one thread per row r, an
+// mt19937_64 seeded with seed + r (state words interleaved across rows, so
+// the 312-word twists are coalesced), libstdc++'s generate_canonical and
+// Marsaglia polar method, sigma = sqrt(P(T, k^2, m) / 2), re drawn before im.
+// ---------------------------------------------------------------------------
+struct MtRow {
+    unsigned long long *st;
+    long long rows, r;
+    int mti;
+    __device__ unsigned long long &w(int i) { return st[(long long)i * rows + r]; }
+    __device__ void seed(unsigned long long s)
+    {
+        unsigned long long x = s;
+        w(0) = x;
+        for (int i = 1; i < 312; ++i) {
+            x = 6364136223846793005ULL * (x ^ (x >> 62)) + (unsigned long long)i;
+            w(i) = x;
+        }
+        mti = 312;
+    }
+    __device__ unsigned long long next()
+    {
+        const unsigned long long UM = 0xFFFFFFFF80000000ULL, LM = 0x7FFFFFFFULL, A = 0xB5026F5AA96619E9ULL;
+        if (mti >= 312) {
+            int i;
+            for (i = 0; i < 156; ++i) {
+                unsigned long long x = (w(i) & UM) | (w(i + 1) & LM);
+                w(i) = w(i + 156) ^ (x >> 1) ^ ((x & 1ULL) ? A : 0ULL);
+            }
+            for (; i < 311; ++i) {
+                unsigned long long x = (w(i) & UM) | (w(i + 1) & LM);
+                w(i) = w(i - 156) ^ (x >> 1) ^ ((x & 1ULL) ? A : 0ULL);
+            }
+            unsigned long long x = (w(311) & UM) | (w(0) & LM);
+            w(311) = w(155) ^ (x >> 1) ^ ((x & 1ULL) ? A : 0ULL);
+            mti = 0;
+        }
+        unsigned long long y = w(mti++);
+        y ^= (y >> 29) & 0x5555555555555555ULL;
+        y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+        y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+        y ^= (y >> 43);
+        return y;
+    }
+    __device__ double canonical()
+    {
+        double r = (double)next() / 18446744073709551616.0;
+        if (r >= 1.0) r = 0x1.fffffffffffffp-1;
+        return r;
+    }
+};
+
+__global__ void __launch_bounds__(256)
+k_thermal_modes(cpx<double> *H, unsigned long long *state, long long rows, int N, int D, double L, double T,
+                double m, int velocity, unsigned long long seed)
+{
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const double PI = 3.14159265358979323846;
+    const int nc = N / 2 + 1;
+    MtRow g{state, rows, r, 0};
+    g.seed(seed + (unsigned long long)r);
+    auto kphys = [&](long long i) -> double {
+        const long long s = (i <= N / 2) ? i : i - N;
+        return 2.0 * PI * (double)s / L;
+    };
+    double k2rows;
+    if (D == 2) { const double q = kphys(r); k2rows = q * q; }
+    else { const double q1 = kphys(r / N), q2 = kphys(r % N); k2rows = q1 * q1 + q2 * q2; }
+    bool have = false;
+    double saved = 0.0;
+    auto normal = [&]() -> double {
+        if (have) { have = false; return saved; }
+        double x, y, r2;
+        do {
+            x = 2.0 * g.canonical() - 1.0;
+            y = 2.0 * g.canonical() - 1.0;
+            r2 = x * x + y * y;
+        } while (r2 > 1.0 || r2 == 0.0);
+        const double mult = sqrt(-2 * log(r2) / r2);
+        saved = x * mult;
+        have = true;
+        return y * mult;
+    };
+    for (int j = 0; j < nc; ++j) {
+        const double kj = 2.0 * PI * j / L;
+        const double k2 = k2rows + kj * kj;
+        const double omega = sqrt(k2 + m * m);
+        const double n_k = 1.0 / (exp(omega / T) - 1.0);
+        const double P = velocity ? n_k * omega : n_k / omega;
+        const double sigma = sqrt(P / 2.);
+        const double re = sigma * (normal() * 1.0 + 0.0);
+        const double im = sigma * (normal() * 1.0 + 0.0);
+        H[r * nc + j] = {re, im};
+    }
+}
+
+// Hermitian symmetry on the planes j = 0 and j = N/2 (src/bubble.cpp:493-511).
+// Each row only reads a partner with a smaller index, which it never writes,
+// so the rows are independent.
+__global__ void __launch_bounds__(256) k_hermitian(cpx<double> *H, long long rows, int N, int D)
+{
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int nc = N / 2 + 1;
+    const long long partner = (D == 2) ? (N - r) % N : ((N - r / N) % N) * N + (N - r % N) % N;
+    const int planes[2] = {0, N / 2};
+    for (int q = 0; q < 2; ++q) {
+        const int j = planes[q];
+        cpx<double> *mode = H + r * nc + j;
+        if (partner == r) {
+            mode->x *= 1.4142135623730951;   // sqrt(2.)
+            mode->y = 0.;
+        } else if (partner < r) {
+            const cpx<double> p = H[partner * nc + j];
+            mode->x = p.x;
+            mode->y = -p.y;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct FftPlan {
+    int N = 0, D = 0;
+    bool fp32 = false;
+    Stages st{};
+    void *tw = nullptr;        // cpx<R> table, n entries
+    void *H = nullptr;         // half-complex work array
+    long long rows = 0;
+    int nBins = 0, binblocks = 0;
+    double *binpart = nullptr, *bins = nullptr;
+};
+
+static bool factorise(int n, Stages &st)
+{
+    st.n = n;
+    st.ns = 0;
+    int m = n;
+    while (m % 4 == 0) { st.radix[st.ns++] = 4; m /= 4; }
+    while (m % 2 == 0) { st.radix[st.ns++] = 2; m /= 2; }
+    while (m % 3 == 0) { st.radix[st.ns++] = 3; m /= 3; }
+    for (int p = 5; m > 1 && p * p <= m; p += 2)
+        while (m % p == 0) { st.radix[st.ns++] = p; m /= p; if (st.ns >= kMaxStages) return false; }
+    if (m > 1) st.radix[st.ns++] = m;
+    return st.ns <= kMaxStages;
+}
+
+template <typename R>
+static hipError_t make_twiddles(int n, void **out, hipStream_t s)
+{
+    std::vector<cpx<R>> h((size_t)n);
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    for (int e = 0; e < n; ++e) {
+        long double a = twopi * (long double)e / (long double)n;
+        h[(size_t)e] = {(R)cosl(a), (R)-sinl(a)};
+    }
+    hipError_t err = hipMalloc(out, sizeof(cpx<R>) * (size_t)n);
+    if (err != hipSuccess) return err;
+    err = hipMemcpyAsync(*out, h.data(), sizeof(cpx<R>) * (size_t)n, hipMemcpyHostToDevice, s);
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    return err;
+}
+
+// Rows / columns per block for an LDS budget.
+static int batch_for(int n, size_t csz)
+{
+    size_t per = 2 * (size_t)n * csz;
+    int b = (int)(kLdsBudget / per);
+    return b < 1 ? 0 : (b > 16 ? 16 : b);
+}
+
+// All axis transforms of one D-dim half-complex array, forward from a real
+// field (r2c) or backward into one (c2r).
+template <typename R, typename T>
+static hipError_t forward_r2c(const Stages &st, const cpx<R> *tw, const T *field, cpx<R> *H, int D,
+                              hipStream_t s)
+{
+    const int N = st.n, nc = N / 2 + 1;
+    const long long rows = (D == 3) ? (long long)N * N : (D == 2 ? N : 1);
+    const int RB = batch_for(N, sizeof(cpx<R>));
+    if (RB < 1) return hipErrorInvalidValue;
+    size_t shm = 2 * (size_t)RB * N * sizeof(cpx<R>);
+    k_rows_r2c<T, R><<<(unsigned)((rows + RB - 1) / RB), 256, shm, s>>>(field, H, rows, RB, st, tw);
+    const int CB = batch_for(N, sizeof(cpx<R>));
+    shm = 2 * (size_t)CB * N * sizeof(cpx<R>);
+    if (D == 3) {   // axis 1: [N][N][nc]
+        const long long tiles = ((long long)nc + CB - 1) / CB;
+        k_cols<R><<<(unsigned)(N * tiles), 256, shm, s>>>(H, nc, CB, st, tw, -1);
+    }
+    if (D >= 2) {   // axis 0: [1][N][rows/N * nc]
+        const long long inner = rows / N * nc;
+        const long long tiles = (inner + CB - 1) / CB;
+        k_cols<R><<<(unsigned)tiles, 256, shm, s>>>(H, inner, CB, st, tw, -1);
+    }
+    return hipGetLastError();
+}
+
+void fft_destroy(FftPlan *p)
+{
+    if (!p) return;
+    if (p->tw) hipFree(p->tw);
+    if (p->H) hipFree(p->H);
+    if (p->binpart) hipFree(p->binpart);
+    if (p->bins) hipFree(p->bins);
+    delete p;
+}
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                \
+            return CB_HIP_ERROR;                                                         \
+        }                                                                                \
+    } while (0)
+
+static cb_status plan_for(cb_sim *s)
+{
+    if (s->fft) return CB_OK;
+    const int N = s->p.nGrids, D = s->p.D;
+    FftPlan *p = new FftPlan();
+    p->N = N;
+    p->D = D;
+    p->fp32 = s->fp32;
+    if (!factorise(N, p->st) || batch_for(N, s->fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>)) < 1) {
+        delete p;
+        set_error("FFT length not supported by the in-LDS transform");
+        return CB_UNSUPPORTED;
+    }
+    p->rows = (D == 3) ? (long long)N * N : (D == 2 ? N : 1);
+    const int nc = N / 2 + 1;
+    const size_t csz = s->fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+    hipError_t e = s->fp32 ? make_twiddles<float>(N, &p->tw, s->st) : make_twiddles<double>(N, &p->tw, s->st);
+    if (e == hipSuccess) e = hipMalloc(&p->H, csz * (size_t)p->rows * nc);
+    p->nBins = s->nkmax + 1;
+    long long modes = p->rows * nc;
+    p->binblocks = (int)((modes + 4095) / 4096);
+    if (p->binblocks > 1024) p->binblocks = 1024;
+    if (p->binblocks < 1) p->binblocks = 1;
+    if (e == hipSuccess) e = hipMalloc((void **)&p->binpart, sizeof(double) * 3 * (size_t)p->nBins * p->binblocks);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->bins, sizeof(double) * 3 * (size_t)p->nBins);
+    if (e != hipSuccess) {
+        fft_destroy(p);
+        set_error(std::string("FFT plan: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? CB_OUT_OF_MEMORY : CB_HIP_ERROR;
+    }
+    s->fft = p;
+    return CB_OK;
+}
+
+cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int cap, int *nbins)
+{
+    cb_status r = plan_for(s);
+    if (r != CB_OK) return r;
+    FftPlan *p = s->fft;
+    const int D = s->p.D, N = s->p.nGrids, nB = p->nBins;
+    const double L = s->p.L;
+    const void *field = s->phi[s->iphi];
+    hipError_t e;
+    if (s->fp32) e = forward_r2c<float, float>(p->st, (const cpx<float> *)p->tw, (const float *)field, (cpx<float> *)p->H, D, s->st);
+    else e = forward_r2c<double, double>(p->st, (const cpx<double> *)p->tw, (const double *)field, (cpx<double> *)p->H, D, s->st);
+    HIP_TRY(e);
+    const double PI = 3.14159265358979323846;
+    const double dk = 2.0 * PI / L;
+    const double pref = (D == 1) ? 1.0 / PI : (D == 2) ? 1.0 / (2.0 * PI) : 1.0 / (2.0 * PI * PI);
+    const double Pnorm = s->dxD * s->dxD / std::pow(L, D);
+    const size_t shm = sizeof(double) * 3 * (size_t)nB;
+    if (shm > kLdsBudget) { set_error("too many spectrum bins for LDS"); return CB_UNSUPPORTED; }
+    if (s->fp32) k_bin<float><<<p->binblocks, 256, shm, s->st>>>((const cpx<float> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm, p->binpart);
+    else k_bin<double><<<p->binblocks, 256, shm, s->st>>>((const cpx<double> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm, p->binpart);
+    HIP_TRY(hipGetLastError());
+    k_bin_reduce<<<(3 * nB + 255) / 256, 256, 0, s->st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h(3 * (size_t)nB);
+    HIP_TRY(hipMemcpyAsync(h.data(), p->bins, sizeof(double) * 3 * nB, hipMemcpyDeviceToHost, s->st));
+    cb::DevScalars sc;
+    HIP_TRY(hipMemcpyAsync(&sc, s->dsc, sizeof sc, hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    int nout = 0;
+    for (int b = 0; b < nB; ++b) {
+        const double ws = h[(size_t)nB + b];
+        if (ws > 0) {
+            const double km = h[2 * (size_t)nB + b] / ws;
+            if (nout < cap) {
+                if (kmean) kmean[nout] = km;
+                if (kphys) kphys[nout] = km / sc.a;
+                if (delta) delta[nout] = h[(size_t)b] / ws;
+            }
+            ++nout;
+        }
+    }
+    *nbins = nout;
+    return CB_OK;
+}
+
+// Mode -3 field (GenerateThermalFluc, src/bubble.cpp:459-523), computed in
+// double whatever the storage precision.
+cb_status thermal_field(cb_sim *s, void *field, double T, double m, int velocity)
+{
+    const int N = s->p.nGrids, D = s->p.D;
+    if (D < 2) return CB_OK;   // the reference never seeds a 1D thermal field
+    Stages st;
+    if (!factorise(N, st) || batch_for(N, sizeof(cpx<double>)) < 1) {
+        set_error("FFT length not supported by the in-LDS transform");
+        return CB_UNSUPPORTED;
+    }
+    const long long rows = (D == 3) ? (long long)N * N : N;
+    const int nc = N / 2 + 1;
+    void *tw = nullptr, *H = nullptr, *state = nullptr;
+    auto cleanup = [&]() {
+        if (tw) hipFree(tw);
+        if (H) hipFree(H);
+        if (state) hipFree(state);
+    };
+    hipError_t e = make_twiddles<double>(N, &tw, s->st);
+    if (e == hipSuccess) e = hipMalloc(&H, sizeof(cpx<double>) * (size_t)rows * nc);
+    if (e == hipSuccess) e = hipMalloc(&state, sizeof(unsigned long long) * 312 * (size_t)rows);
+    if (e != hipSuccess) {
+        cleanup();
+        set_error(std::string("thermal IC: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? CB_OUT_OF_MEMORY : CB_HIP_ERROR;
+    }
+    // thermalSeed, src/bubble.cpp:134-142
+    const char *spec = velocity ? "velocitythermalspec" : "thermalspec";
+    unsigned long long off = 0;
+    for (const char *c = spec; *c; ++c) off = off * 131 + (unsigned char)*c;
+    const unsigned long long seed = (unsigned long long)s->p.thermal_base + off;
+    const unsigned nb = (unsigned)((rows + 255) / 256);
+    k_thermal_modes<<<nb, 256, 0, s->st>>>((cpx<double> *)H, (unsigned long long *)state, rows, N, D, s->p.L, T, m,
+                                           velocity, seed);
+    k_hermitian<<<nb, 256, 0, s->st>>>((cpx<double> *)H, rows, N, D);
+    const int CB = batch_for(N, sizeof(cpx<double>));
+    const size_t shm = 2 * (size_t)CB * N * sizeof(cpx<double>);
+    {   // axis 0 then (3D) axis 1, backward
+        const long long inner = rows / N * nc;
+        const long long tiles = (inner + CB - 1) / CB;
+        k_cols<double><<<(unsigned)tiles, 256, shm, s->st>>>((cpx<double> *)H, inner, CB, st, (const cpx<double> *)tw, +1);
+        if (D == 3) {
+            const long long t2 = ((long long)nc + CB - 1) / CB;
+            k_cols<double><<<(unsigned)(N * t2), 256, shm, s->st>>>((cpx<double> *)H, nc, CB, st, (const cpx<double> *)tw, +1);
+        }
+    }
+    const int RB = CB;
+    const double norm = 1.0 / std::sqrt(std::pow(s->p.L, D));
+    if (s->fp32)
+        k_rows_c2r<float, double><<<(unsigned)((rows + RB - 1) / RB), 256, shm, s->st>>>((const cpx<double> *)H, (float *)field, rows, RB, st, (const cpx<double> *)tw, norm, 0.0);
+    else
+        k_rows_c2r<double, double><<<(unsigned)((rows + RB - 1) / RB), 256, shm, s->st>>>((const cpx<double> *)H, (double *)field, rows, RB, st, (const cpx<double> *)tw, norm, 0.0);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    cleanup();
+    if (e != hipSuccess) { set_error(std::string("thermal IC: ") + hipGetErrorString(e)); return CB_HIP_ERROR; }
+    return CB_OK;
+}
+
+} // namespace cb

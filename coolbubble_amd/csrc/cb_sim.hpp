@@ -1,0 +1,51 @@
+// cb_sim.hpp -- the simulation handle behind the C ABI (internal).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/coolbubble.h"
+#include "cb_kernels.hpp"
+
+namespace cb {
+
+void set_error(const std::string &msg);
+
+struct FftPlan;   // cb_fft.hip
+
+} // namespace cb
+
+struct cb_sim {
+    cb_params p{};
+    cb::SimConst k{};
+    bool fp32 = false;
+    bool lagged = false;        // expanding background: lagged-drift step
+    size_t esz = 8;             // bytes per stored value
+    long long nsites = 0;
+    int nkmax = 0;              // src/bubble.hpp:49
+    double dxD = 0, phiC = 0, Rn = 0, lw = 0, mT = 0, phi0 = 1.0;
+    hipStream_t st = nullptr;
+    void *phi[2] = {nullptr, nullptr};
+    void *pi[2] = {nullptr, nullptr};
+    int iphi = 0, ipi = 0;
+    double *partials = nullptr;
+    double *dtot = nullptr;
+    cb::DevScalars *dsc = nullptr;
+    cb::SimpleCfg cfg{256, 1};
+    int tiled = -1;             // index of the tiled kernel config, -1 = general kernel
+    double t = 0.0;
+    long long step = 0;
+    std::vector<double> xns;    // centres (the __xns.dat rows)
+    cb::FftPlan *fft = nullptr;
+    // ctor-time energy row (spreads are zero there, src/bubble.cpp:44)
+    bool row_valid = false;
+};
+
+namespace cb {
+// spectrum of the current phi (cb_fft.hip); bins filled with the reference's
+// non-empty-shell convention
+cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int cap, int *nbins);
+// thermal initial condition of one field (cb_fft.hip)
+cb_status thermal_field(cb_sim *s, void *field, double T, double m, int velocity);
+void fft_destroy(FftPlan *p);
+} // namespace cb

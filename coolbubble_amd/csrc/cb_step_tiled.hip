@@ -1,0 +1,335 @@
+// cb_step_tiled.hip -- LDS-tiled, plane-marching fused step for 2D and 3D
+// lattices on gfx950.
+//
+// Layout (src/gridfuncs.cpp:25-26): site (i, j, k) at (i N + j) N + k, k
+// fastest.  A block owns a tile of TR rows x all N columns of the (j, k)
+// plane (2D: one row) and walks the slow axis i through a chunk of NI planes.
+// Per plane it loads the next plane's tile once from HBM (16-byte vector
+// loads, whole rows, so the k taps never leave the tile), keeps planes
+// i-1, i, i+1 in registers for the i taps, and stages plane i plus one halo
+// row above and below in LDS for the j taps and the k taps across vector
+// boundaries.  Each site is read once per step apart from the halo rows
+// (2/TR) and the chunk-end planes (2/NI), which adjacent blocks scheduled on
+// the same XCD (block ids are remapped so an XCD owns a contiguous run of
+// tiles) mostly find in that XCD's L2.
+//
+// The arithmetic per site is the reference's, in its order
+// (src/bubble.cpp:716-743 force, :854-856 kick, :868-870 drift, :776-836
+// energies), identical to the general kernel in cb_kernels.hip.
+#include <mutex>
+#include <vector>
+
+#include "cb_device.hpp"
+#include "cb_tiled.hpp"
+
+namespace cb {
+
+template <typename T, int VW>
+struct alignas(sizeof(T) * VW) Vec {
+    T v[VW];
+};
+
+template <typename T, int D, int VW, int ITEMS, int NTHR, bool LAGGED, bool ENERGY>
+__global__ void __launch_bounds__(NTHR)
+k_step_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restrict__ phi_out,
+             const T *pi_in, T *pi_out, const DevScalars *__restrict__ sc, double *__restrict__ partials)
+{
+    using V = Vec<T, VW>;
+    using C = T;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+
+    const int N = k.N;
+    const int NV = g.NV;
+    const long long NN = (D == 3) ? (long long)N * N : (long long)N;
+    const int G = gridDim.x, b = blockIdx.x;
+    const int lt = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;   // XCD-contiguous tiles
+    const int TRv = (D == 3) ? g.TR : 1;
+    const int rt = (D == 3) ? lt % g.ntr : 0;
+    const int ic = (D == 3) ? lt / g.ntr : lt;
+    const int r0 = rt * TRv;
+    const int i0 = ic * g.NI;
+    const int i1 = min(i0 + g.NI, N);
+    const int nvec = TRv * NV;
+
+    const C dt = (C)k.dt;
+    const C s_prev = LAGGED ? (C)sc->s_pending : (C)0;
+    const C fac_lap = (C)sc->fac_lap, fac_pot = (C)sc->fac_pot;
+    const C inv_dx2 = (C)(1.0 / k.dx2);
+    const C fc2 = (C)k.c2, fc3 = (C)k.c3, fc4 = (C)k.c4;
+
+    int rowq[ITEMS], col[ITEMS];
+    bool ok[ITEMS];
+    long long off[ITEMS];
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+        int q = threadIdx.x + u * NTHR;
+        ok[u] = q < nvec;
+        int qq = ok[u] ? q : 0;
+        rowq[u] = qq / NV;
+        col[u] = (qq - rowq[u] * NV) * VW;
+        off[u] = (D == 3) ? (long long)(r0 + rowq[u]) * N + col[u] : (long long)col[u];
+    }
+
+    V prev[ITEMS], cur[ITEMS], nxt[ITEMS];
+    V pcur[ITEMS], pnxt[ITEMS];   // incoming pi of the current / next plane (lagged)
+
+    auto load_plane = [&](int i, V (&out)[ITEMS], V (&pout)[ITEMS]) {
+        const long long base = (long long)i * NN;
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u) {
+            if (!ok[u]) continue;
+            V f = *reinterpret_cast<const V *>(phi_in + base + off[u]);
+            if constexpr (LAGGED) {
+                V p = *reinterpret_cast<const V *>(pi_in + base + off[u]);
+#pragma unroll
+                for (int e = 0; e < VW; ++e) f.v[e] = f.v[e] + dt * p.v[e] * s_prev;
+                pout[u] = p;
+            }
+            out[u] = f;
+        }
+    };
+
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+
+    load_plane(i0 == 0 ? N - 1 : i0 - 1, prev, pnxt);
+    load_plane(i0, cur, pcur);
+
+    for (int i = i0; i < i1; ++i) {
+        const int ip = (i + 1 == N) ? 0 : i + 1;
+        load_plane(ip, nxt, pnxt);
+        const long long base = (long long)i * NN;
+        if constexpr (!LAGGED) {
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u)
+                if (ok[u]) pcur[u] = *reinterpret_cast<const V *>(pi_in + base + off[u]);
+        }
+        // stage plane i (tile rows at 1..TR, halo rows at 0 and TR+1)
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u)
+            if (ok[u]) *reinterpret_cast<V *>(lds + (D == 3 ? (rowq[u] + 1) * N : 0) + col[u]) = cur[u];
+        if constexpr (D == 3) {
+            for (int h = threadIdx.x; h < 2 * NV; h += NTHR) {
+                const int side = h >= NV;
+                const int c0 = (h - side * NV) * VW;
+                const int grow = side ? ((r0 + TRv == N) ? 0 : r0 + TRv) : (r0 == 0 ? N - 1 : r0 - 1);
+                const long long idx = base + (long long)grow * N + c0;
+                V f = *reinterpret_cast<const V *>(phi_in + idx);
+                if constexpr (LAGGED) {
+                    V p = *reinterpret_cast<const V *>(pi_in + idx);
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) f.v[e] = f.v[e] + dt * p.v[e] * s_prev;
+                }
+                *reinterpret_cast<V *>(lds + (side ? TRv + 1 : 0) * N + c0) = f;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u) {
+            if (!ok[u]) continue;
+            const T *lrow = lds + (D == 3 ? (rowq[u] + 1) * N : 0);
+            V rm, rp;
+            if constexpr (D == 3) {
+                rm = *reinterpret_cast<const V *>(lrow - N + col[u]);
+                rp = *reinterpret_cast<const V *>(lrow + N + col[u]);
+            }
+            const C fl = lrow[col[u] == 0 ? N - 1 : col[u] - 1];
+            const C fr = lrow[(col[u] + VW == N) ? 0 : col[u] + VW];
+            V po = pcur[u], pn, phn;
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                const C c = cur[u].v[e];
+                const C fM = (e == 0) ? fl : cur[u].v[e - 1];
+                const C fP = (e == VW - 1) ? fr : cur[u].v[e + 1];
+                const C mM = prev[u].v[e], mP = nxt[u].v[e];
+                C dd = (C)0.0;
+                dd += mP - (C)2.0 * c + mM;
+                if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * c + rm.v[e];
+                dd += fP - (C)2.0 * c + fM;
+                C F;
+                if constexpr (sizeof(T) == 8) F = fac_lap * dd / k.dx2 - fac_pot * vprime(c, k);
+                else F = fac_lap * dd * inv_dx2 - fac_pot * vprime_f(c, fc2, fc3, fc4);
+                const C pnew = po.v[e] + dt * F;
+                pn.v[e] = pnew;
+                if constexpr (LAGGED) phn.v[e] = c;
+                else phn.v[e] = c + dt * pnew;
+                if constexpr (ENERGY) {
+                    double fwd[3];
+                    fwd[0] = (double)mP;
+                    if constexpr (D == 3) { fwd[1] = (double)rp.v[e]; fwd[2] = (double)fP; }
+                    else { fwd[1] = (double)fP; fwd[2] = 0.0; }
+                    energy_terms<D>((double)c, fwd, (double)pnew, (double)po.v[e], k, sc, acc);
+                }
+            }
+            *reinterpret_cast<V *>(pi_out + base + off[u]) = pn;
+            *reinterpret_cast<V *>(phi_out + base + off[u]) = phn;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u) {
+            prev[u] = cur[u];
+            cur[u] = nxt[u];
+            if constexpr (LAGGED) pcur[u] = pnxt[u];
+        }
+    }
+    if constexpr (ENERGY) block_partials<NQ>(acc, partials + (long long)lt * NQ);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static TiledGeom geom(const SimConst &k, const TiledCfg &c)
+{
+    TiledGeom g;
+    g.TR = (k.D == 3) ? c.TR : 1;
+    g.NI = c.NI;
+    g.NV = k.N / c.vw;
+    g.ntr = (k.D == 3) ? k.N / c.TR : 1;
+    g.nic = (k.N + c.NI - 1) / c.NI;
+    return g;
+}
+
+static size_t lds_bytes(const SimConst &k, int fp32, const TiledCfg &c)
+{
+    const size_t esz = fp32 ? 4 : 8;
+    const int rows = (k.D == 3) ? c.TR + 2 : 1;
+    return (size_t)rows * k.N * esz;
+}
+
+bool tiled_valid(const SimConst &k, int fp32, const TiledCfg &c)
+{
+    if (k.D < 2) return false;
+    const int want_vw = fp32 ? 4 : 2;
+    if (c.vw != want_vw) return false;
+    if (k.N % c.vw != 0 || k.N < 4) return false;
+    if (c.items != 1 && c.items != 2 && c.items != 4) return false;
+    if (c.nthr != 128 && c.nthr != 256) return false;
+    if (c.NI < 1) return false;
+    if (k.D == 3 && (c.TR < 1 || k.N % c.TR != 0)) return false;
+    const int rows = (k.D == 3) ? c.TR : 1;
+    if ((long long)rows * (k.N / c.vw) > (long long)c.items * c.nthr) return false;
+    if (lds_bytes(k, fp32, c) > 65536) return false;
+    return true;
+}
+
+int tiled_blocks(const SimConst &k, const TiledCfg &c)
+{
+    TiledGeom g = geom(k, c);
+    return g.ntr * g.nic;
+}
+
+int tiled_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
+{
+    int n = 0;
+    const int vw = fp32 ? 4 : 2;
+    auto push = [&](int items, int nthr, int TR, int NI) {
+        TiledCfg c{vw, items, nthr, TR, NI};
+        if (n < cap && tiled_valid(k, fp32, c)) out[n++] = c;
+    };
+    if (k.D == 3) {
+        const int trs[] = {8, 4, 2, 16, 1};
+        const int nis[] = {32, 16, 64, 8};
+        for (int TR : trs)
+            for (int nthr : {256, 128}) {
+                const int nv = TR * (k.N / vw);
+                int items = (nv + nthr - 1) / nthr;
+                if (items == 3) items = 4;
+                for (int NI : nis) push(items, nthr, TR, NI);
+            }
+    } else {
+        const int nis[] = {16, 8, 32, 4};
+        for (int nthr : {128, 256}) {
+            const int nv = k.N / vw;
+            int items = (nv + nthr - 1) / nthr;
+            if (items == 3) items = 4;
+            for (int NI : nis) push(items, nthr, 1, NI);
+        }
+    }
+    return n;
+}
+
+template <typename T, int D, int ITEMS, int NTHR>
+static void launch_t(const SimConst &k, int lagged, int energy, const TiledCfg &c, const StepPtrs &p,
+                     const DevScalars *sc, double *partials, hipStream_t st, size_t shm)
+{
+    constexpr int VW = sizeof(T) == 4 ? 4 : 2;
+    TiledGeom g = geom(k, c);
+    dim3 grid(g.ntr * g.nic), block(NTHR);
+    const T *pin = (const T *)p.phi_in;
+    T *pout = (T *)p.phi_out;
+    const T *qin = (const T *)p.pi_in;
+    T *qout = (T *)p.pi_out;
+    if constexpr (D == 3) {
+        if (lagged) {
+            if (energy) k_step_tiled<T, D, VW, ITEMS, NTHR, true, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            else k_step_tiled<T, D, VW, ITEMS, NTHR, true, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            return;
+        }
+    }
+    if (energy) k_step_tiled<T, D, VW, ITEMS, NTHR, false, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+    else k_step_tiled<T, D, VW, ITEMS, NTHR, false, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+}
+
+template <typename T, int D>
+static void launch_d(const SimConst &k, int lagged, int energy, const TiledCfg &c, const StepPtrs &p,
+                     const DevScalars *sc, double *partials, hipStream_t st, size_t shm)
+{
+    if (c.nthr == 256) {
+        if (c.items == 1) launch_t<T, D, 1, 256>(k, lagged, energy, c, p, sc, partials, st, shm);
+        else if (c.items == 2) launch_t<T, D, 2, 256>(k, lagged, energy, c, p, sc, partials, st, shm);
+        else launch_t<T, D, 4, 256>(k, lagged, energy, c, p, sc, partials, st, shm);
+    } else {
+        if (c.items == 1) launch_t<T, D, 1, 128>(k, lagged, energy, c, p, sc, partials, st, shm);
+        else if (c.items == 2) launch_t<T, D, 2, 128>(k, lagged, energy, c, p, sc, partials, st, shm);
+        else launch_t<T, D, 4, 128>(k, lagged, energy, c, p, sc, partials, st, shm);
+    }
+}
+
+hipError_t launch_step_tiled_cfg(const SimConst &k, int fp32, int lagged, int energy, const TiledCfg &c,
+                                 const StepPtrs &p, const DevScalars *sc, double *partials, hipStream_t st)
+{
+    if (!tiled_valid(k, fp32, c)) return hipErrorInvalidValue;
+    if (lagged && k.D != 3) return hipErrorInvalidValue;
+    const size_t shm = lds_bytes(k, fp32, c);
+    if (fp32) {
+        if (k.D == 3) launch_d<float, 3>(k, lagged, energy, c, p, sc, partials, st, shm);
+        else launch_d<float, 2>(k, lagged, energy, c, p, sc, partials, st, shm);
+    } else {
+        if (k.D == 3) launch_d<double, 3>(k, lagged, energy, c, p, sc, partials, st, shm);
+        else launch_d<double, 2>(k, lagged, energy, c, p, sc, partials, st, shm);
+    }
+    return hipGetLastError();
+}
+
+static std::mutex g_reg_mu;
+static std::vector<TiledCfg> g_reg;
+
+int tiled_register(const TiledCfg &c)
+{
+    std::lock_guard<std::mutex> lk(g_reg_mu);
+    for (size_t i = 0; i < g_reg.size(); ++i) {
+        const TiledCfg &o = g_reg[i];
+        if (o.vw == c.vw && o.items == c.items && o.nthr == c.nthr && o.TR == c.TR && o.NI == c.NI) return (int)i;
+    }
+    g_reg.push_back(c);
+    return (int)g_reg.size() - 1;
+}
+
+TiledCfg tiled_get(int which)
+{
+    std::lock_guard<std::mutex> lk(g_reg_mu);
+    return g_reg.at((size_t)which);
+}
+
+hipError_t launch_step_tiled(const SimConst &k, int fp32, int lagged, int energy, int which,
+                             const StepPtrs &p, const DevScalars *sc, double *partials, hipStream_t st,
+                             int *nblocks)
+{
+    TiledCfg c = tiled_get(which);
+    if (nblocks) *nblocks = tiled_blocks(k, c);
+    return launch_step_tiled_cfg(k, fp32, lagged, energy, c, p, sc, partials, st);
+}
+
+} // namespace cb

@@ -1,0 +1,49 @@
+// cb_tiled.hpp -- LDS-tiled marching step kernel: configurations and launcher.
+#pragma once
+
+#include "cb_kernels.hpp"
+
+namespace cb {
+
+// One launch shape of the tiled kernel.  A block owns TR rows (3D; 1 in 2D)
+// of every plane of an i-chunk of NI planes and marches through the chunk:
+// the planes i-1, i, i+1 of its own columns sit in registers, the plane i
+// tile plus one halo row on each side sits in LDS for the j and k taps.
+struct TiledCfg {
+    int vw;        // elements per vector load (16 B: 4 float / 2 double); 1 = scalar
+    int items;     // vectors per thread per plane (template: 1, 2, 4)
+    int nthr;      // threads per block (template: 128, 256)
+    int TR;        // rows per tile (3D)
+    int NI;        // planes per block
+};
+
+// Geometry of one launch, derived from (N, cfg).
+struct TiledGeom {
+    int TR, NI, ntr, nic, NV;
+};
+
+// Is cfg usable for this lattice?  (vector width divides N, tile fits LDS,
+// every vector of a tile plane has an owner thread)
+bool tiled_valid(const SimConst &k, int fp32, const TiledCfg &c);
+
+// Blocks a launch of cfg uses (energy partials are written per block).
+int tiled_blocks(const SimConst &k, const TiledCfg &c);
+
+// Candidate configurations for (D, N, precision), best-guess first.
+int tiled_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap);
+
+hipError_t launch_step_tiled_cfg(const SimConst &k, int fp32, int lagged, int energy, const TiledCfg &c,
+                                 const StepPtrs &p, const DevScalars *sc, double *partials,
+                                 hipStream_t st);
+
+// Launch with the configuration registered under index `which` for this sim
+// (see tiled_register); *nblocks receives the block count.
+hipError_t launch_step_tiled(const SimConst &k, int fp32, int lagged, int energy, int which,
+                             const StepPtrs &p, const DevScalars *sc, double *partials, hipStream_t st,
+                             int *nblocks);
+
+// Registry of chosen configurations (the autotuner writes, the step reads).
+int tiled_register(const TiledCfg &c);
+TiledCfg tiled_get(int which);
+
+} // namespace cb

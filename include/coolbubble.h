@@ -1,0 +1,144 @@
+/*
+ * coolbubble.h -- C ABI of the MI355X lattice time-stepper (libcoolbubble.so).
+ *
+ * Drop-in for CoolBubble's `Bubble` object and lattice helpers
+ * (src/bubble.hpp:34-135, src/gridfuncs.h:27-41).  The reference has no FFI:
+ * its boundary is C++ linkage plus process globals (src/gridfuncs.h:27-32,
+ * defined at src/main.cpp:45-50) read by member initialisers, and
+ * compile-time constants (src/def.h).  Here the same state lives behind an
+ * opaque handle, every parameter is a run-time field of cb_params, and no
+ * call exits, throws or prints: each returns a cb_status.
+ *
+ * Threading: one handle owns one HIP stream; calls on one handle must be
+ * serialised by the caller; distinct handles may be driven from different
+ * threads.  All device memory is owned by the library and freed by
+ * cb_destroy.  Host buffers are owned by the caller.
+ */
+#ifndef COOLBUBBLE_H
+#define COOLBUBBLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cb_status {
+    CB_OK = 0,
+    CB_INVALID_ARG = 1,     /* reference: exit(1) in parseArguments, src/main.cpp:105-138 */
+    CB_CFL_VIOLATION = 2,   /* reference: CheckStability returns false, src/main.cpp:171-183 */
+    CB_OUT_OF_MEMORY = 3,
+    CB_HIP_ERROR = 4,
+    CB_NAN_DETECTED = 5,
+    CB_NO_DEVICE = 6,
+    CB_ADMISSION_REJECTED = 7,
+    CB_NOT_READY = 8,
+    CB_UNSUPPORTED = 9
+} cb_status;
+
+typedef enum cb_precision {
+    CB_FP64 = 0,            /* phi, pi stored and stepped in double (reference arithmetic) */
+    CB_FP32 = 1             /* phi, pi stored and stepped in float; reductions, scale factor
+                               and spectrum bins in double */
+} cb_precision;
+
+/* Run parameters.  Field comments give the reference global / #define. */
+typedef struct cb_params {
+    int D;                       /* D, src/gridfuncs.h:27 (1, 2 or 3) */
+    int nGrids;                  /* nGrids, src/gridfuncs.h:28 (>= 2) */
+    double L;                    /* L, src/gridfuncs.h:29 */
+    double c;                    /* c, src/gridfuncs.h:32 */
+    int readxns;                 /* readxns (IC mode 1, 0, -1, -2, -3), src/gridfuncs.h:31 */
+    double dt;                   /* TIME_STEP, src/def.h:34 */
+    int nBubbles;                /* nBubbles, src/def.h:46 */
+    double Trad;                 /* Trad, src/def.h:48 */
+    double DeltaR;               /* DeltaR, src/def.h:50 */
+    int test_wave;               /* TEST, src/def.h:51 */
+    int expansion;               /* expansion, src/def.h:56 (3D only, bubble.hpp:77) */
+    int alpha;                   /* ALPHAVAL, src/def.h:57 (0 cosmic, 1 conformal time) */
+    double Mpl;                  /* Mpl, src/def.h:58 */
+    unsigned int rand_seed;      /* srand seed of mode 1 (42: src/bubble.cpp:401) */
+    unsigned long long thermal_base; /* thermal seed base (12345: src/bubble.cpp:137) */
+    int precision;               /* cb_precision (no counterpart: the reference is FP64) */
+    int device;                  /* HIP device ordinal */
+} cb_params;
+
+/* The 16 columns of one EnergyDensities.dat row (outEP, src/bubble.cpp:977-989):
+ * t, a, Etot, Ekin, Egrad, Epot, sqrt(EtotVar), sqrt(EkinVar), sqrt(EgradVar),
+ * sqrt(EpotVar), <phi>, sqrt(<phi>Var), <phi'>, sqrt(<phi'>Var), H, C. */
+typedef struct cb_energies {
+    double col[16];
+} cb_energies;
+
+/* Scalar state beside the fields (src/bubble.hpp:75-84). */
+typedef struct cb_state {
+    double a, ap, H, C, t, EkinHalfPrev;
+    long long step;
+} cb_state;
+
+typedef struct cb_sim cb_sim;
+
+/* Reference defaults from src/def.h:29-58 (D=2, nGrids=512, L=32, c=4, mode 1,
+ * TIME_STEP 0.004, 5 bubbles, Trad 0.2, static, Mpl 200), FP64, device 0. */
+cb_status cb_default_params(cb_params *p);
+
+/* CheckStability, src/main.cpp:171-183: CB_OK iff dt < dx/sqrt(D). */
+cb_status cb_check_stability(const cb_params *p);
+
+/* Bubble::Bubble, src/bubble.cpp:156-316: allocate the lattice on the
+ * device, seed the initial condition (mode 1 draws its centres like
+ * GenerateNucleationPoints, :397-417; mode 0 takes the caller's centres,
+ * which replace <run>/__xns.dat, :420-453), measure the initial energies and
+ * step pi back by half a step.  xns: n_xns rows of (x, y, z); ignored unless
+ * readxns == 0. */
+cb_status cb_create(const cb_params *p, const double *xns, int n_xns, cb_sim **out);
+
+/* Bubble::oneStep, src/bubble.cpp:849-876, nsteps times.  Energies are
+ * measured on the last step, so cb_energies afterwards gives the row the
+ * reference would print after that step (src/main.cpp:280). */
+cb_status cb_step(cb_sim *s, int nsteps);
+
+/* outEP, src/bubble.cpp:977-989: the current energy row. */
+cb_status cb_energies_row(cb_sim *s, cb_energies *out);
+
+/* FourierTransform + ComputePowerSpectrum, src/bubble.cpp:642-709: for each
+ * non-empty |k| shell (width 2 pi / L) the mean comoving k, k / a and
+ * Delta(k).  cap = capacity of each output array; *nbins = bins written. */
+cb_status cb_power_spectrum(cb_sim *s, double *kmean, double *kphys, double *delta,
+                            int cap, int *nbins);
+
+/* The fields a snapshot writes (OutField, src/bubble.cpp:887-917): phi at
+ * the current whole step and pi half a step behind, as double (FP32 runs are
+ * widened).  Either pointer may be NULL.  Each holds nGrids^D values in
+ * lattice order (src/gridfuncs.cpp:25-26). */
+cb_status cb_get_fields(cb_sim *s, double *phi, double *pi);
+
+/* Scalar state, and state import (phi, pi given as double; FP32 runs round
+ * them).  cb_set_state replaces the fields and scalars; EkinHalfPrev must be
+ * the kinetic energy of pi (the previous half step), as the ctor and
+ * oneStep leave it.  No reference counterpart (the reference cannot
+ * restart, its snapshots being float32). */
+cb_status cb_get_state(cb_sim *s, cb_state *st);
+cb_status cb_set_state(cb_sim *s, const cb_state *st, const double *phi, const double *pi);
+
+/* Centres used by mode 1 / 0 (the contents of __xns.dat, src/bubble.cpp:408-416):
+ * *n = rows available; min(cap, *n) rows of (x, y, z) written to xyz. */
+cb_status cb_get_xns(cb_sim *s, double *xyz, int cap, int *n);
+
+/* The HIP stream (hipStream_t) every kernel of this handle runs on. */
+cb_status cb_stream(cb_sim *s, void **stream);
+
+/* Number of lattice sites N = nGrids^D (src/bubble.hpp:43). */
+long long cb_nsites(const cb_sim *s);
+
+cb_status cb_destroy(cb_sim *s);
+
+/* Text of a status code, and the message of the last failed call on this
+ * host thread. */
+const char *cb_status_string(cb_status st);
+const char *cb_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COOLBUBBLE_H */

@@ -1,0 +1,87 @@
+/*
+ * cb_oracle.h -- CPU FP64 restatement of CoolBubble's lattice time-stepper.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
+ * nowhere else.  The product (coolbubble_amd, libcoolbubble.so) never links,
+ * loads or calls it.
+ *
+ * Every function restates the reference algorithm in plain C, in the
+ * reference's operation order (left-to-right C++ evaluation, no FMA), citing
+ * the reference file:line it follows.  It is built with gcc -O2
+ * -ffp-contract=off and calls glibc pow/tanh/exp/log/sqrt where the
+ * reference does, so its per-site arithmetic is the reference's.
+ *
+ * The FFTs (reference: FFTW3 r2c / c2r, src/bubble.cpp:465-467,515,648-651)
+ * are an exact mathematical DFT evaluated in long double (mixed radix,
+ * recursive), rounded to double; FFTW itself is absent from the image.
+ */
+#ifndef CB_ORACLE_H
+#define CB_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Run parameters: the reference's process globals (src/gridfuncs.h:27-32)
+ * and compile-time constants (src/def.h:29-76) as one struct. */
+typedef struct cbo_params {
+    int D;                 /* src/def.h:30   */
+    int nGrids;            /* src/def.h:31   */
+    double L;              /* src/def.h:32   */
+    double c;              /* src/def.h:39   */
+    int readxns;           /* src/def.h:40   initial-condition mode */
+    double dt;             /* src/def.h:34   TIME_STEP */
+    int nBubbles;          /* src/def.h:46   */
+    double Trad;           /* src/def.h:48   */
+    double DeltaR;         /* src/def.h:50   */
+    int test_wave;         /* src/def.h:51   TEST */
+    int expansion;         /* src/def.h:56   */
+    int alpha;             /* src/def.h:57   ALPHAVAL */
+    double Mpl;            /* src/def.h:58   */
+    unsigned int rand_seed;      /* srand(42), src/bubble.cpp:401 */
+    uint64_t thermal_base;       /* 12345, src/bubble.cpp:137 */
+} cbo_params;
+
+typedef struct cbo_sim cbo_sim;
+
+/* ---- whole-run API (Bubble ctor / oneStep / outEP / spectrum) ---- */
+cbo_sim *cbo_create(const cbo_params *p, const double *xns, int n_xns, int nthreads);
+void cbo_destroy(cbo_sim *s);
+void cbo_step(cbo_sim *s, int nsteps);
+void cbo_energy_row(const cbo_sim *s, double row[16]);
+int cbo_spectrum(cbo_sim *s, double *kmean, double *kphys, double *delta, int cap);
+long long cbo_nsites(const cbo_sim *s);
+double *cbo_phi(cbo_sim *s);
+double *cbo_pi(cbo_sim *s);
+int cbo_xns(const cbo_sim *s, double *xyz, int cap);
+/* scalars: a, ap, H, C, t, EkinHalfPrev */
+void cbo_get_scalars(const cbo_sim *s, double out[6]);
+void cbo_set_scalars(cbo_sim *s, const double in[6]);
+void cbo_set_threads(cbo_sim *s, int nthreads);
+
+/* ---- pieces, exported for the pinning tests ---- */
+double cbo_V(double phi, double c, double DeltaV);
+double cbo_Vpr(double phi, double c);
+int cbo_iGridM(int D, int nGrids, int iGrid, int iShift);
+int cbo_iGridP(int D, int nGrids, int iGrid, int iShift);
+void cbo_bubble_fit(int D, double c, double out[3]); /* phiC, Rn, lw */
+int cbo_generate_xns(int D, double L, double c, int nBubbles, unsigned seed,
+                     double *xyz, int cap);
+void cbo_mt64(uint64_t seed, long n, uint64_t *out);
+void cbo_normals(uint64_t seed, long n, double *out);
+void cbo_dft_ld(int n, int sign, const double *in_re, const double *in_im,
+                double *out_re, double *out_im);
+void cbo_r2c(int D, int N, const double *field, double *out_re, double *out_im);
+void cbo_thermal_modes(int D, int N, double L, double T, double m, int velocity,
+                       uint64_t base, double *re, double *im);
+void cbo_thermal_field(int D, int N, double L, double T, double m, int velocity,
+                       uint64_t base, double *field);
+void cbo_forces(cbo_sim *s, double *dpi_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
