@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Headline benchmark: site-updates/s of the lattice time-stepper on one MI355X.
+
+Workload: d=3, N=512^3, L=64, c=4, mode 1 with 32 seeded bubbles (radius 1.3x
+the fitted critical radius), dt = 0.2 dx, static background -- the north
+star's FP32 vs FP64 configuration.  A "step" is one leapfrog step
+(Bubble::oneStep, src/bubble.cpp:849-876) of the whole lattice; the steps are
+issued as the reference's driver issues them, in runs of 40 with the
+box-averaged energies measured at the end of each run (OutEnergyEvery = 40,
+src/def.h:65, src/main.cpp:280).
+
+Timing: HIP events recorded on the library's own stream around exactly K
+steps, inputs resident in HBM.  value = FP32 site-updates/s.  The FP64 GPU
+path runs the same workload in the same process; the CPU baseline times the
+oracle (the C/OpenMP FP64 restatement of the reference) on the host cores
+over a bounded sample of the same lattice.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK = 8.0e12  # MI355X datasheet, bytes/s
+
+CONFIG = dict(D=3, nGrids=512, L=64.0, c=4.0, readxns=1, nBubbles=32, expansion=0)
+CONFIG["dt"] = 0.2 * CONFIG["L"] / CONFIG["nGrids"]
+ENERGY_EVERY = 40
+
+
+class HipEvents:
+    """hipEvent timing on an explicit stream (libamdhip64 through ctypes)."""
+
+    def __init__(self):
+        self.h = ctypes.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                               ctypes.c_void_p]
+        self.h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        self.h.hipDeviceSynchronize.argtypes = []
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.h.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        ms = ctypes.c_float()
+        assert self.h.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
+        return ms.value
+
+    def sync(self):
+        assert self.h.hipDeviceSynchronize() == 0
+
+
+def run_precision(cb, hip, precision, steps, warmup, log):
+    t0 = time.time()
+    sim = cb.Bubble(precision=precision, **CONFIG)
+    log("%s: seeded %d^3 in %.1f s" % (precision, CONFIG["nGrids"], time.time() - t0))
+    best, ms = sim.autotune(steps=4)
+    nvar, sel = sim.kernel_variants()
+    log("%s: autotune picked variant %d of %d (%.3f ms/step; general kernel %.3f ms/step)"
+        % (precision, sel, nvar, ms[sel] if sel >= 0 else ms[-1], ms[-1]))
+    done = 0
+    while done < warmup:
+        n = min(ENERGY_EVERY, warmup - done)
+        sim.step(n)
+        done += n
+    stream = sim.stream()
+    a, b = hip.event(), hip.event()
+    hip.sync()
+    hip.record(a, stream)
+    done = 0
+    while done < steps:
+        n = min(ENERGY_EVERY, steps - done)
+        sim.step(n)
+        done += n
+    hip.record(b, stream)
+    ms_total = hip.elapsed_ms(a, b)
+    row = sim.energy_row()
+    log("%s: %d steps in %.3f ms (%.4f ms/step)" % (precision, steps, ms_total, ms_total / steps))
+    return sim, ms_total, row
+
+
+def spectrum_time(sim, hip, reps=3):
+    stream = sim.stream()
+    sim.spectrum()   # plan + warm
+    a, b = hip.event(), hip.event()
+    hip.record(a, stream)
+    for _ in range(reps):
+        sim.spectrum()
+    hip.record(b, stream)
+    return hip.elapsed_ms(a, b) / reps
+
+
+def cpu_baseline(sim64, nsteps, log):
+    """Oracle (FP64 C/OpenMP restatement, 4 passes per step with energies
+    every step, like the reference) stepping the same lattice from the GPU's
+    seeded state, on the host cores."""
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    phi, pi = sim64.fields()
+    st = sim64.state()
+    p = oracle.default_params(**CONFIG)
+    o = oracle.Sim(params=p, blank=True, nthreads=threads)
+    o.phi()[...] = phi
+    o.pi()[...] = pi
+    o.set_scalars([st.a, st.ap, st.H, st.C, st.t, st.EkinHalfPrev])
+    del phi, pi
+    o.step(1)   # first touch
+    t0 = time.time()
+    o.step(nsteps)
+    dt = time.time() - t0
+    o.close()
+    log("cpu baseline: %d steps in %.2f s on %d threads" % (nsteps, dt, threads))
+    return dt, threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the lattice time-stepper runs on a single MI355X "
+              "(multi-GPU slab decomposition is a design note, not built); use --gpus 1",
+              file=sys.stderr)
+        return 2
+
+    def log(msg):
+        print("[bench] " + msg, file=sys.stderr, flush=True)
+
+    import coolbubble_amd as cb
+    hip = HipEvents()
+    N = CONFIG["nGrids"]
+    sites = N ** 3
+    K, W = args.steps, args.warmup
+
+    sim32, ms32, row32 = run_precision(cb, hip, "fp32", K, W, log)
+    spec32 = spectrum_time(sim32, hip)
+    log("fp32 spectrum: %.3f ms" % spec32)
+    sim32.close()
+    sim64, ms64, row64 = run_precision(cb, hip, "fp64", K, W, log)
+    spec64 = spectrum_time(sim64, hip)
+    log("fp64 spectrum: %.3f ms" % spec64)
+
+    su32 = sites * K / (ms32 * 1e-3)
+    su64 = sites * K / (ms64 * 1e-3)
+    t_step32 = ms32 * 1e-3 / K
+    bytes32 = 16.0 * sites          # read phi, read pi, write pi, write phi (FP32)
+    bytes64 = 32.0 * sites
+    ach32 = bytes32 / t_step32
+    ach64 = bytes64 / (ms64 * 1e-3 / K)
+    # energies after the same W+K steps from the same seeded lattice
+    err = {name: abs(row32[i] - row64[i]) / abs(row64[i])
+           for i, name in ((2, "Etot"), (3, "Ekin"), (4, "Egrad"), (5, "Epot"))}
+    fft_bytes = lambda w: (2 * 3 - 1) * w * sites   # (2d-1) w N^d per spectrum
+    cpu = None
+    if not args.no_cpu:
+        cdt, threads = cpu_baseline(sim64, args.cpu_steps, log)
+        cpu = {"value": sites * args.cpu_steps / cdt, "unit": "site-updates/s", "cores": threads,
+               "kind": "port",
+               "sample": "%d oracle steps (FP64, energies every step) of the same 512^3 lattice "
+                         "from the GPU-seeded state" % args.cpu_steps}
+    sim64.close()
+    out = {
+        "metric": "site-updates/s (FP32 stepper, d=3 N=512 static, mode 1 32 bubbles)",
+        "value": su32,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms32 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded bubbles, reference mode 1 generator)",
+        "config": {"workload": "lattice time-stepper d=3 N=512^3 L=64 c=4 mode 1 nBubbles=32 "
+                               "dt=0.025 static, energies every 40 steps",
+                   "sites": sites, "energy_every": ENERGY_EVERY},
+        "roofline": {"bound": "hbm", "achieved": ach32 / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": ach32 / HBM_PEAK, "traffic": None},
+        "cpu_baseline": cpu,
+        "fp64": {"value": su64, "ms_per_step": ms64 / K, "achieved_GBps": ach64 / 1e9,
+                 "frac": ach64 / HBM_PEAK},
+        "fp32_over_fp64": su32 / su64,
+        "fp32_energy_rel_err_vs_fp64": err,
+        "spectrum": {"fp32_ms": spec32, "fp64_ms": spec64,
+                     "fp32_GBps_algorithmic": fft_bytes(4) / (spec32 * 1e-3) / 1e9,
+                     "fp64_GBps_algorithmic": fft_bytes(8) / (spec64 * 1e-3) / 1e9},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,217 @@
+"""coolbubble_amd -- MI355X lattice time-stepper (ctypes binding of libcoolbubble.so).
+
+Python mirror of the reference's `Bubble` object (src/bubble.hpp:34-135) over
+the C ABI declared in include/coolbubble.h.  Every call goes to the HIP
+library; there is no CPU fallback: if libcoolbubble.so is missing or cannot
+be loaded, importing the library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcoolbubble.so")
+
+CB_FP64 = 0
+CB_FP32 = 1
+
+STATUS = {0: "CB_OK", 1: "CB_INVALID_ARG", 2: "CB_CFL_VIOLATION", 3: "CB_OUT_OF_MEMORY",
+          4: "CB_HIP_ERROR", 5: "CB_NAN_DETECTED", 6: "CB_NO_DEVICE", 7: "CB_ADMISSION_REJECTED",
+          8: "CB_NOT_READY", 9: "CB_UNSUPPORTED"}
+
+# Column names of an EnergyDensities.dat row (src/bubble.cpp:962-964).
+ENERGY_COLUMNS = ("t", "a", "Etot", "Ekin", "Egrad", "Epot", "EtotSpread", "EkinSpread",
+                  "EgradSpread", "EpotSpread", "avphi", "avphiSpread", "avphiP",
+                  "avphiPSpread", "H", "C")
+
+
+class CBError(RuntimeError):
+    def __init__(self, status, msg):
+        self.status = status
+        super().__init__("%s: %s" % (STATUS.get(status, status), msg))
+
+
+class Params(ctypes.Structure):
+    """cb_params (include/coolbubble.h)."""
+    _fields_ = [
+        ("D", ctypes.c_int), ("nGrids", ctypes.c_int), ("L", ctypes.c_double),
+        ("c", ctypes.c_double), ("readxns", ctypes.c_int), ("dt", ctypes.c_double),
+        ("nBubbles", ctypes.c_int), ("Trad", ctypes.c_double), ("DeltaR", ctypes.c_double),
+        ("test_wave", ctypes.c_int), ("expansion", ctypes.c_int), ("alpha", ctypes.c_int),
+        ("Mpl", ctypes.c_double), ("rand_seed", ctypes.c_uint),
+        ("thermal_base", ctypes.c_ulonglong), ("precision", ctypes.c_int), ("device", ctypes.c_int),
+    ]
+
+
+class Energies(ctypes.Structure):
+    _fields_ = [("col", ctypes.c_double * 16)]
+
+
+class State(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_double), ("ap", ctypes.c_double), ("H", ctypes.c_double),
+                ("C", ctypes.c_double), ("t", ctypes.c_double), ("EkinHalfPrev", ctypes.c_double),
+                ("step", ctypes.c_longlong)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libcoolbubble.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libcoolbubble.so not built: run python -m coolbubble_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        L.cb_default_params.argtypes = [ctypes.POINTER(Params)]
+        L.cb_check_stability.argtypes = [ctypes.POINTER(Params)]
+        L.cb_create.argtypes = [ctypes.POINTER(Params), dp, ctypes.c_int, ctypes.POINTER(vp)]
+        L.cb_step.argtypes = [vp, ctypes.c_int]
+        L.cb_energies_row.argtypes = [vp, ctypes.POINTER(Energies)]
+        L.cb_power_spectrum.argtypes = [vp, dp, dp, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.cb_get_fields.argtypes = [vp, dp, dp]
+        L.cb_get_state.argtypes = [vp, ctypes.POINTER(State)]
+        L.cb_set_state.argtypes = [vp, ctypes.POINTER(State), dp, dp]
+        L.cb_get_xns.argtypes = [vp, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.cb_stream.argtypes = [vp, ctypes.POINTER(vp)]
+        L.cb_kernel_variants.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.cb_select_kernel.argtypes = [vp, ctypes.c_int]
+        L.cb_autotune.argtypes = [vp, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.cb_nsites.argtypes = [vp]
+        L.cb_nsites.restype = ctypes.c_longlong
+        L.cb_destroy.argtypes = [vp]
+        L.cb_status_string.argtypes = [ctypes.c_int]
+        L.cb_status_string.restype = ctypes.c_char_p
+        L.cb_last_error.restype = ctypes.c_char_p
+        for name in ("cb_default_params", "cb_check_stability", "cb_create", "cb_step",
+                     "cb_energies_row", "cb_power_spectrum", "cb_get_fields", "cb_get_state",
+                     "cb_set_state", "cb_get_xns", "cb_stream", "cb_destroy",
+                     "cb_kernel_variants", "cb_select_kernel", "cb_autotune"):
+            getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != 0:
+        raise CBError(status, lib().cb_last_error().decode(errors="replace"))
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def default_params(**kw):
+    """Reference defaults (src/def.h), overridable by keyword ('precision': 'fp32'/'fp64')."""
+    p = Params()
+    check(lib().cb_default_params(ctypes.byref(p)))
+    for key, val in kw.items():
+        if key == "precision" and isinstance(val, str):
+            val = CB_FP32 if val.lower() == "fp32" else CB_FP64
+        setattr(p, key, val)
+    return p
+
+
+class Bubble:
+    """One simulation on the GPU: Bubble::Bubble + oneStep + outEP + spectrum."""
+
+    def __init__(self, params=None, xns=None, **kw):
+        self.p = params if params is not None else default_params(**kw)
+        h = ctypes.c_void_p()
+        if xns is not None:
+            xa = np.ascontiguousarray(np.asarray(xns, dtype=np.float64).reshape(-1, 3))
+            check(lib().cb_create(ctypes.byref(self.p), _dp(xa), xa.shape[0], ctypes.byref(h)))
+        else:
+            check(lib().cb_create(ctypes.byref(self.p), None, 0, ctypes.byref(h)))
+        self._h = h
+        self.nsites = lib().cb_nsites(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def shape(self):
+        return (self.p.nGrids,) * self.p.D
+
+    def step(self, n=1):
+        check(lib().cb_step(self._h, int(n)))
+
+    def energy_row(self):
+        e = Energies()
+        check(lib().cb_energies_row(self._h, ctypes.byref(e)))
+        return np.array(e.col[:])
+
+    def fields(self):
+        phi = np.empty(self.nsites)
+        pi = np.empty(self.nsites)
+        check(lib().cb_get_fields(self._h, _dp(phi), _dp(pi)))
+        return phi.reshape(self.shape()), pi.reshape(self.shape())
+
+    def state(self):
+        st = State()
+        check(lib().cb_get_state(self._h, ctypes.byref(st)))
+        return st
+
+    def set_state(self, st, phi=None, pi=None):
+        ph = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64).ravel()
+        pp = None if pi is None else np.ascontiguousarray(pi, dtype=np.float64).ravel()
+        check(lib().cb_set_state(self._h, ctypes.byref(st), None if ph is None else _dp(ph),
+                                 None if pp is None else _dp(pp)))
+
+    def xns(self):
+        n = ctypes.c_int()
+        check(lib().cb_get_xns(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros((max(n.value, 1), 3))
+        check(lib().cb_get_xns(self._h, _dp(out), n.value, ctypes.byref(n)))
+        return out[:n.value]
+
+    def spectrum(self):
+        cap = int(np.sqrt(self.p.D) * self.p.nGrids / 2) + 2
+        k, kp, d = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+        nb = ctypes.c_int()
+        check(lib().cb_power_spectrum(self._h, _dp(k), _dp(kp), _dp(d), cap, ctypes.byref(nb)))
+        n = nb.value
+        return k[:n].copy(), kp[:n].copy(), d[:n].copy()
+
+    def kernel_variants(self):
+        """(number of tiled variants, selected variant; -1 = general kernel)."""
+        n, sel = ctypes.c_int(), ctypes.c_int()
+        check(lib().cb_kernel_variants(self._h, ctypes.byref(n), ctypes.byref(sel)))
+        return n.value, sel.value
+
+    def select_kernel(self, variant):
+        check(lib().cb_select_kernel(self._h, int(variant)))
+
+    def autotune(self, steps=5):
+        """Time every variant (state preserved), select the fastest; returns
+        (best variant, ms per step of each variant with the general kernel last)."""
+        n, _ = self.kernel_variants()
+        ms = np.zeros(n + 1)
+        best = ctypes.c_int()
+        check(lib().cb_autotune(self._h, int(steps), _dp(ms), n + 1, ctypes.byref(best)))
+        return best.value, ms
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        check(lib().cb_stream(self._h, ctypes.byref(s)))
+        return s

@@ -198,12 +198,22 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
     s->mT = std::sqrt(2 * (c + 6));
     s->cfg = simple_cfg(s->nsites);
 
+    {   // step-kernel variants and the energy-partials capacity they need
+        cb::TiledCfg cand[64];
+        int nc = cb::tiled_candidates(k, s->fp32, cand, 64);
+        s->variants.assign(cand, cand + nc);
+        s->max_blocks = s->cfg.grid;
+        for (const auto &c : s->variants) {
+            int nb = cb::tiled_blocks(k, c);
+            if (nb > s->max_blocks) s->max_blocks = nb;
+        }
+    }
     const size_t bytes = (size_t)s->nsites * s->esz;
     for (int i = 0; i < 2; ++i) {
         HIP_TRY(hipMalloc(&s->phi[i], bytes));
         if (i == 0 || s->lagged) HIP_TRY(hipMalloc(&s->pi[i], bytes));
     }
-    HIP_TRY(hipMalloc((void **)&s->partials, sizeof(double) * cb::NQ * (size_t)s->cfg.grid));
+    HIP_TRY(hipMalloc((void **)&s->partials, sizeof(double) * cb::NQ * (size_t)s->max_blocks));
     HIP_TRY(hipMalloc((void **)&s->dtot, sizeof(double) * cb::NQ));
     HIP_TRY(hipMalloc((void **)&s->dsc, sizeof(cb::DevScalars)));
     HIP_TRY(hipMemsetAsync(s->phi[0], 0, bytes, s->st));
@@ -299,7 +309,7 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
     s->t = 0.;
     s->step = 0;
     s->row_valid = true;
-    return CB_OK;
+    return cb::select_variant(s, s->variants.empty() ? -1 : 0);
 }
 
 cb_status read_scalars(cb_sim *s, cb::DevScalars &h)
@@ -317,10 +327,6 @@ cb_status materialise(cb_sim *s)
 }
 
 } // namespace
-
-namespace cb {
-cb_status step_n(cb_sim *s, int nsteps);
-}
 
 extern "C" {
 
@@ -502,6 +508,20 @@ cb_status cb_power_spectrum(cb_sim *s, double *kmean, double *kphys, double *del
     cb_status r = materialise(s);
     if (r != CB_OK) return r;
     return cb::spectrum(s, kmean, kphys, delta, cap, nbins);
+}
+
+cb_status cb_kernel_variants(cb_sim *s, int *count, int *selected)
+{
+    if (!s) { set_error("invalid handle"); return CB_INVALID_ARG; }
+    if (count) *count = (int)s->variants.size();
+    if (selected) *selected = s->variant;
+    return CB_OK;
+}
+
+cb_status cb_select_kernel(cb_sim *s, int variant)
+{
+    if (!s) { set_error("invalid handle"); return CB_INVALID_ARG; }
+    return cb::select_variant(s, variant);
 }
 
 cb_status cb_stream(cb_sim *s, void **stream)

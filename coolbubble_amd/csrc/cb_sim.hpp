@@ -6,6 +6,7 @@
 
 #include "../../include/coolbubble.h"
 #include "cb_kernels.hpp"
+#include "cb_tiled.hpp"
 
 namespace cb {
 
@@ -32,7 +33,10 @@ struct cb_sim {
     double *dtot = nullptr;
     cb::DevScalars *dsc = nullptr;
     cb::SimpleCfg cfg{256, 1};
-    int tiled = -1;             // index of the tiled kernel config, -1 = general kernel
+    int tiled = -1;             // registry index of the tiled config in use, -1 = general kernel
+    int variant = -1;           // index into `variants`, -1 = general kernel
+    std::vector<cb::TiledCfg> variants;   // tiled configurations valid for this lattice
+    int max_blocks = 0;         // capacity of `partials`, in blocks
     double t = 0.0;
     long long step = 0;
     std::vector<double> xns;    // centres (the __xns.dat rows)
@@ -44,6 +48,8 @@ struct cb_sim {
 namespace cb {
 // spectrum of the current phi (cb_fft.hip); bins filled with the reference's
 // non-empty-shell convention
+cb_status step_n(cb_sim *s, int nsteps);
+cb_status select_variant(cb_sim *s, int variant);
 cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int cap, int *nbins);
 // thermal initial condition of one field (cb_fft.hip)
 cb_status thermal_field(cb_sim *s, void *field, double T, double m, int velocity);

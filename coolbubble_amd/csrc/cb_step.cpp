@@ -58,4 +58,84 @@ cb_status step_n(cb_sim *s, int nsteps)
     return CB_OK;
 }
 
+cb_status select_variant(cb_sim *s, int variant)
+{
+    if (variant < -1 || variant >= (int)s->variants.size()) {
+        set_error("kernel variant out of range");
+        return CB_UNSUPPORTED;
+    }
+    s->variant = variant;
+    s->tiled = variant < 0 ? -1 : tiled_register(s->variants[(size_t)variant]);
+    return CB_OK;
+}
+
 } // namespace cb
+
+extern "C" cb_status cb_autotune(cb_sim *s, int steps, double *ms_per_step, int cap, int *best)
+{
+    using namespace cb;
+    if (!s || steps < 1) { set_error("invalid argument"); return CB_INVALID_ARG; }
+    const size_t bytes = (size_t)s->nsites * s->esz;
+    void *save_phi = nullptr, *save_pi = nullptr;
+    HIP_TRY(hipMalloc(&save_phi, bytes));
+    if (hipMalloc(&save_pi, bytes) != hipSuccess) {
+        hipFree(save_phi);
+        set_error("autotune: out of device memory for the state copy");
+        return CB_OUT_OF_MEMORY;
+    }
+    DevScalars sc_save;
+    const int iphi0 = s->iphi, ipi0 = s->ipi;
+    const double t0 = s->t;
+    const long long step0 = s->step;
+    const int var0 = s->variant;
+    hipEvent_t ev[2];
+    hipError_t e = hipMemcpyAsync(save_phi, s->phi[iphi0], bytes, hipMemcpyDeviceToDevice, s->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(save_pi, s->pi[ipi0], bytes, hipMemcpyDeviceToDevice, s->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&sc_save, s->dsc, sizeof sc_save, hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    auto restore = [&]() -> hipError_t {
+        s->iphi = iphi0;
+        s->ipi = ipi0;
+        s->t = t0;
+        s->step = step0;
+        hipError_t r = hipMemcpyAsync(s->phi[iphi0], save_phi, bytes, hipMemcpyDeviceToDevice, s->st);
+        if (r == hipSuccess) r = hipMemcpyAsync(s->pi[ipi0], save_pi, bytes, hipMemcpyDeviceToDevice, s->st);
+        if (r == hipSuccess) r = hipMemcpyAsync(s->dsc, &sc_save, sizeof sc_save, hipMemcpyHostToDevice, s->st);
+        if (r == hipSuccess) r = hipStreamSynchronize(s->st);
+        return r;
+    };
+    const int nvar = (int)s->variants.size();
+    double best_ms = 1e300;
+    int best_v = -1;
+    for (int v = 0; v <= nvar && e == hipSuccess; ++v) {
+        const int var = (v == nvar) ? -1 : v;
+        select_variant(s, var);
+        e = restore();
+        if (e != hipSuccess) break;
+        if (step_n(s, 1) != CB_OK) { e = hipErrorUnknown; break; }
+        e = hipEventRecord(ev[0], s->st);
+        if (e == hipSuccess && step_n(s, steps) != CB_OK) e = hipErrorUnknown;
+        if (e == hipSuccess) e = hipEventRecord(ev[1], s->st);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        const double per = (double)ms / steps;
+        if (ms_per_step && v < cap) ms_per_step[v] = per;
+        if (per < best_ms) { best_ms = per; best_v = var; }
+    }
+    hipError_t e2 = restore();
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    hipFree(save_phi);
+    hipFree(save_pi);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        select_variant(s, var0);
+        set_error(std::string("autotune: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        return CB_HIP_ERROR;
+    }
+    select_variant(s, best_v);
+    if (best) *best = best_v;
+    return CB_OK;
+}

@@ -125,6 +125,20 @@ cb_status cb_set_state(cb_sim *s, const cb_state *st, const double *phi, const d
  * *n = rows available; min(cap, *n) rows of (x, y, z) written to xyz. */
 cb_status cb_get_xns(cb_sim *s, double *xyz, int cap, int *n);
 
+/* Step-kernel choice (no reference counterpart).  variant -1 = the general
+ * kernel (any lattice); variant k >= 0 = the k-th LDS-tiled marching
+ * configuration valid for this (D, nGrids, precision).  cb_kernel_variants
+ * reports how many tiled configurations exist and which one is selected.
+ * cb_create selects variant 0 when one exists. */
+cb_status cb_kernel_variants(cb_sim *s, int *count, int *selected);
+cb_status cb_select_kernel(cb_sim *s, int variant);
+
+/* Autotune: time each variant over `steps` steps (the lattice state is
+ * saved and restored around the trials) and select the fastest.
+ * ms_per_step (optional, capacity cap) receives each variant's time,
+ * general kernel last. */
+cb_status cb_autotune(cb_sim *s, int steps, double *ms_per_step, int cap, int *best);
+
 /* The HIP stream (hipStream_t) every kernel of this handle runs on. */
 cb_status cb_stream(cb_sim *s, void **stream);
 

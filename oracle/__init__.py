@@ -52,6 +52,8 @@ def lib():
         L.cbo_create.restype = ctypes.c_void_p
         L.cbo_create.argtypes = [ctypes.POINTER(Params), dp, ctypes.c_int, ctypes.c_int]
         L.cbo_destroy.argtypes = [ctypes.c_void_p]
+        L.cbo_create_blank.restype = ctypes.c_void_p
+        L.cbo_create_blank.argtypes = [ctypes.POINTER(Params), ctypes.c_int]
         L.cbo_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.cbo_energy_row.argtypes = [ctypes.c_void_p, dp]
         L.cbo_spectrum.argtypes = [ctypes.c_void_p, dp, dp, dp, ctypes.c_int]
@@ -100,10 +102,12 @@ def _dp(a):
 class Sim:
     """One oracle run: the reference's Bubble object (src/bubble.hpp:34-135)."""
 
-    def __init__(self, params=None, xns=None, nthreads=0, **kw):
+    def __init__(self, params=None, xns=None, nthreads=0, blank=False, **kw):
         self.p = params if params is not None else default_params(**kw)
         L = lib()
-        if xns is not None:
+        if blank:
+            self._h = L.cbo_create_blank(ctypes.byref(self.p), nthreads)
+        elif xns is not None:
             xa = np.ascontiguousarray(np.asarray(xns, dtype=np.float64).reshape(-1, 3))
             self._h = L.cbo_create(ctypes.byref(self.p), _dp(xa), xa.shape[0], nthreads)
         else:

@@ -817,6 +817,28 @@ cbo_sim *cbo_create(const cbo_params *p, const double *xns, int n_xns, int nthre
     return s;
 }
 
+cbo_sim *cbo_create_blank(const cbo_params *p, int nthreads)
+{
+    cbo_sim *s = calloc(1, sizeof *s);
+    s->p = *p;
+    s->nthreads = nthreads > 0 ? nthreads : omp_get_max_threads();
+    const int D = p->D, g = p->nGrids;
+    s->N = (long long)pow(g, D);
+    s->n_cplx = g / 2 + 1;
+    s->dx = (double)p->L / (double)g;
+    s->dxD = pow(s->dx, D);
+    s->dxLD = pow(s->dx / p->L, D);
+    s->nkmax = (int)(sqrt((double)D) * g / 2);
+    s->DeltaV = 1.; s->phi0 = 1.;
+    s->phiC = phizero(p->c); s->Rn = cradius(D, p->c); s->lw = clw(p->c);
+    s->mT = sqrt(2 * (p->c + 6));
+    s->phi = calloc(s->N, sizeof(double));
+    s->pi = calloc(s->N, sizeof(double));
+    s->dpi = calloc(s->N, sizeof(double));
+    s->a = 1.;
+    return s;
+}
+
 void cbo_destroy(cbo_sim *s)
 {
     if (!s) return;

@@ -49,6 +49,9 @@ typedef struct cbo_sim cbo_sim;
 
 /* ---- whole-run API (Bubble ctor / oneStep / outEP / spectrum) ---- */
 cbo_sim *cbo_create(const cbo_params *p, const double *xns, int n_xns, int nthreads);
+/* A lattice with the ctor's derived constants and zero fields, a = 1, no
+ * initial condition (for timing the stepper from a state set by the caller). */
+cbo_sim *cbo_create_blank(const cbo_params *p, int nthreads);
 void cbo_destroy(cbo_sim *s);
 void cbo_step(cbo_sim *s, int nsteps);
 void cbo_energy_row(const cbo_sim *s, double row[16]);

@@ -1,0 +1,72 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, validates parameters and reports errors through status
+codes (the reference exits instead, src/main.cpp:105-138,171-183)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import coolbubble_amd as cb
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                      "include", "coolbubble.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const char \*|cb_status |long long )\s*(cb_\w+)\s*\(",
+                                 text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    L = cb.lib()
+    for s in syms:
+        assert hasattr(L, s), s
+
+
+def test_default_params_are_reference_defaults():
+    p = cb.default_params()
+    # src/def.h:30-58
+    assert (p.D, p.nGrids, p.L, p.c, p.readxns, p.dt) == (2, 512, 32.0, 4.0, 1, 0.004)
+    assert (p.nBubbles, p.Trad, p.expansion, p.alpha, p.Mpl) == (5, 0.2, 0, 0, 200.0)
+    assert (p.rand_seed, p.thermal_base) == (42, 12345)
+
+
+@pytest.mark.parametrize("D,n,L,dt,ok", [
+    (2, 512, 32.0, 0.004, True),
+    (2, 512, 32.0, 0.0625 / 2 ** 0.5 * 0.999, True),
+    (2, 512, 32.0, 0.0625 / 2 ** 0.5 * 1.001, False),
+    (3, 64, 8.0, 0.125 / 3 ** 0.5, False),   # dt == dx/sqrt(D) is not < (src/main.cpp:175)
+    (1, 100, 10.0, 0.099, True),
+])
+def test_cfl_check(D, n, L, dt, ok):
+    p = cb.default_params(D=D, nGrids=n, L=L, dt=dt)
+    st = cb.lib().cb_check_stability(ctypes.byref(p))
+    assert st == (0 if ok else 2)
+
+
+@pytest.mark.parametrize("field,value", [("D", 4), ("D", 0), ("nGrids", 1), ("L", -1.0),
+                                         ("readxns", 2), ("readxns", -4), ("precision", 7),
+                                         ("c", 0.0), ("alpha", 2)])
+def test_invalid_params_rejected_without_device(field, value):
+    p = cb.default_params()
+    setattr(p, field, value)
+    h = ctypes.c_void_p()
+    st = cb.lib().cb_create(ctypes.byref(p), None, 0, ctypes.byref(h))
+    assert st == 1
+    assert not h.value
+    assert cb.lib().cb_last_error()
+
+
+def test_cfl_violation_rejected_by_create():
+    p = cb.default_params(D=2, nGrids=64, L=4.0, dt=0.1)
+    h = ctypes.c_void_p()
+    assert cb.lib().cb_create(ctypes.byref(p), None, 0, ctypes.byref(h)) == 2
+
+
+def test_status_strings():
+    for code in range(10):
+        assert cb.lib().cb_status_string(code)

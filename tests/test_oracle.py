@@ -1,0 +1,394 @@
+"""Pins of the oracle (oracle/cb_oracle.c) before it is trusted as the checker.
+
+The reference ships no tests, fixtures or golden data.  Its src/bubble.cpp
+needs FFTW3, which is absent here, so only src/gridfuncs.cpp can be built
+(oracle/_ref).  Each part of the oracle is therefore pinned by one of:
+  * the compiled reference (lattice indexing, periodic images);
+  * a known-answer test or the library routine the reference itself calls
+    (C++ standard mt19937_64 KAT, libstdc++ normal_distribution, glibc rand,
+    numpy's FFT for the DFT conventions);
+  * closed forms and invariants (potential, plane-wave Laplacian eigenvalue,
+    linear dispersion of the leapfrog, de Sitter scale factor, Parseval,
+    lattice-point counts of the spectrum shells, energy conservation).
+"""
+import ctypes
+import math
+import os
+import shutil
+import subprocess
+import tempfile
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+
+HAVE_GXX = shutil.which("g++") is not None
+
+
+# --------------------------------------------------------------------------
+# random numbers
+# --------------------------------------------------------------------------
+def test_mt19937_64_known_answer():
+    # C++ standard [rand.predef]: the 10000th output of a default-constructed
+    # mt19937_64 (seed 5489) is 9981545732273789042
+    assert int(oracle.mt64(5489, 10000)[-1]) == 9981545732273789042
+
+
+_CXX_NORMALS = r"""
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+int main(int argc, char **argv) {
+    unsigned long long seed = std::strtoull(argv[1], nullptr, 10);
+    long n = std::atol(argv[2]);
+    std::mt19937_64 gen(seed);
+    std::normal_distribution<double> d(0.0, 1.0);
+    for (long i = 0; i < n; ++i) std::printf("%a\n", d(gen));
+    return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def cxx_normals():
+    if not HAVE_GXX:
+        pytest.skip("g++ absent")
+    d = tempfile.mkdtemp()
+    src, exe = os.path.join(d, "n.cpp"), os.path.join(d, "n")
+    open(src, "w").write(_CXX_NORMALS)
+    subprocess.run(["g++", "-O2", "-std=c++17", src, "-o", exe], check=True)
+    yield exe
+    shutil.rmtree(d)
+
+
+@pytest.mark.parametrize("seed", [12345, 12345 + 7, 2 ** 63 + 11, 0])
+def test_normals_bitwise_equal_libstdcxx(cxx_normals, seed):
+    n = 3001
+    out = subprocess.run([cxx_normals, str(seed), str(n)], capture_output=True, text=True,
+                         check=True).stdout.split()
+    ref = np.array([float.fromhex(x) for x in out])
+    np.testing.assert_array_equal(oracle.normals(seed, n), ref)
+
+
+def _libc_rand_xns(D, L, c, nb, seed):
+    """Centres from glibc's own srand/rand, formatted as the reference writes
+    them, with the periodic images written by the reference's check_PBC."""
+    ref = oracle.ref_gridfuncs()
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(seed)
+    RAND_MAX = 2147483647
+    phiC, Rn, lw = oracle.bubble_fit(D, c)
+    text = ""
+    for _ in range(nb):
+        xn = [L * float(libc.rand()) / RAND_MAX for _ in range(3)]
+        text += "%.5e %.5e %.5e\n" % tuple(xn)
+        buf = ctypes.create_string_buffer(4096)
+        arr = (ctypes.c_double * 3)(*xn)
+        ref.ref_check_pbc(D, arr, L, Rn + 5 * lw, buf, 4096)
+        text += buf.value.decode()
+    vals = [float(v) for v in text.split()]
+    return np.array(vals).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("D,L,c,nb", [(2, 32.0, 4.0, 5), (3, 64.0, 4.0, 32), (2, 8.0, 4.0, 5),
+                                      (3, 4.0, 2.0, 6)])
+def test_nucleation_points_match_glibc_rand_and_reference_check_pbc(D, L, c, nb):
+    if oracle.ref_gridfuncs() is None:
+        pytest.skip("reference gridfuncs not built (no reference tree)")
+    np.testing.assert_array_equal(oracle.generate_xns(D, L, c, nb, 42), _libc_rand_xns(D, L, c, nb, 42))
+
+
+# --------------------------------------------------------------------------
+# lattice indexing (src/gridfuncs.cpp)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("D", [1, 2, 3])
+@pytest.mark.parametrize("N", [2, 3, 4, 5, 7])
+def test_indexing_matches_reference_and_brute_force(D, N):
+    L = oracle.lib()
+    ref = oracle.ref_gridfuncs()
+    total = N ** D
+    for ig in range(total):
+        idx = np.unravel_index(ig, (N,) * D)
+        for s in range(D if D > 1 else 2):
+            m = L.cbo_iGridM(D, N, ig, s)
+            if D == 1:
+                want = (idx[0] + (1 if s == 1 else -1)) % N
+                assert m == want
+            else:
+                p = L.cbo_iGridP(D, N, ig, s)
+                ip = list(idx); ip[s] = (ip[s] + 1) % N
+                im = list(idx); im[s] = (im[s] - 1) % N
+                assert p == np.ravel_multi_index(ip, (N,) * D)
+                assert m == np.ravel_multi_index(im, (N,) * D)
+                if ref is not None:
+                    assert p == ref.ref_iGridP(D, N, ig, s)
+            if ref is not None:
+                assert m == ref.ref_iGridM(D, N, ig, s)
+
+
+# --------------------------------------------------------------------------
+# potential (src/bubble.cpp:69-77)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("c", [1.0, 2.0, 4.0, 7.3])
+def test_potential_closed_forms(c):
+    L = oracle.lib()
+    assert L.cbo_V(0.0, c, 1.0) == 1.0
+    assert L.cbo_V(1.0, c, 1.0) == 0.0
+    assert L.cbo_Vpr(0.0, c) == 0.0
+    assert L.cbo_Vpr(1.0, c) == 0.0
+    assert abs(L.cbo_Vpr(c / (2 * c + 6), c)) < 1e-15        # barrier top
+    phi_zero = (2 + c - math.sqrt(4 + c)) / (3 + c)             # V(phiZero) = V(0)
+    assert abs(L.cbo_V(phi_zero, c, 1.0) - 1.0) < 1e-14
+    rng = np.random.default_rng(1)
+    for phi in rng.uniform(-0.6, 1.6, 200):
+        f, cc = Fraction(phi), Fraction(c)
+        v = cc * f ** 2 - (2 * cc + 4) * f ** 3 + (cc + 3) * f ** 4 + 1
+        vp = 2 * cc * f - 3 * (2 * cc + 4) * f ** 2 + 4 * (cc + 3) * f ** 3
+        scale_v = float(abs(cc * f ** 2) + abs((2 * cc + 4) * f ** 3) + abs((cc + 3) * f ** 4) + 1)
+        scale_p = float(abs(2 * cc * f) + abs(3 * (2 * cc + 4) * f ** 2) + abs(4 * (cc + 3) * f ** 3))
+        assert abs(L.cbo_V(phi, c, 1.0) - float(v)) <= 4e-16 * scale_v
+        assert abs(L.cbo_Vpr(phi, c) - float(vp)) <= 4e-16 * max(scale_p, 1e-300)
+
+
+# --------------------------------------------------------------------------
+# stepper
+# --------------------------------------------------------------------------
+def _vprime(phi, c):
+    return 2 * c * phi - 3 * (2 * c + 4) * phi ** 2 + 4 * (c + 3) * phi ** 3
+
+
+@pytest.mark.parametrize("D,N,m", [(1, 32, (3,)), (2, 16, (2, 5)), (3, 8, (1, 2, 3)), (3, 9, (4, 0, 2))])
+def test_laplacian_plane_wave_eigenvalue(D, N, m):
+    L = 2.0
+    s = oracle.Sim(blank=True, D=D, nGrids=N, L=L, dt=0.001, c=3.0)
+    idx = np.indices((N,) * D)
+    theta = sum(2 * np.pi * m[a] * idx[a] / N for a in range(D))
+    phi = 0.2 + 1e-3 * np.cos(theta)
+    s.phi()[...] = phi
+    dx = L / N
+    lam = -(4 / dx ** 2) * sum(math.sin(math.pi * m[a] / N) ** 2 for a in range(D))
+    lap = s.forces() + _vprime(phi, 3.0)
+    np.testing.assert_allclose(lap, lam * (phi - 0.2), atol=1e-9 * abs(lam) * 1e-3 + 1e-12)
+
+
+def test_uniform_field_follows_scalar_leapfrog():
+    """A uniform field has an exactly zero Laplacian, so every site follows
+    the scalar leapfrog phi'' = -V'(phi) started by the ctor's half back-step
+    (src/bubble.cpp:280-289) -- replayed here with Python floats and glibc pow."""
+    c, dt, phi0 = 4.0, 0.01, 0.31
+    s = oracle.Sim(readxns=0, xns=np.zeros((0, 3)), D=2, nGrids=6, L=1.0, dt=dt, c=c)
+    s.phi()[...] = phi0
+    # re-run the back-step from the uniform field by hand on the blank copy
+    b = oracle.Sim(blank=True, D=2, nGrids=6, L=1.0, dt=dt, c=c)
+    b.phi()[...] = phi0
+    vp = lambda f: 2 * c * f - 3 * (2 * c + 4) * (f * f) + 4 * (c + 3) * math.pow(f, 3)
+    F = 1.0 * 0.0 / ((1.0 / 6) * (1.0 / 6)) - 1.0 * vp(phi0)
+    np.testing.assert_array_equal(b.forces(), F)
+    pi = 0.0 - 0.5 * dt * F
+    b.pi()[...] = pi
+    f = phi0
+    for _ in range(50):
+        F = -vp(f) if False else (1.0 * 0.0 / ((1.0 / 6) * (1.0 / 6)) - 1.0 * vp(f))
+        pi = pi + dt * F
+        f = f + dt * pi * 1.0
+    b.step(50)
+    np.testing.assert_array_equal(b.phi(), f)
+    np.testing.assert_array_equal(b.pi(), pi)
+
+
+@pytest.mark.parametrize("D,N,m", [(2, 32, (1, 3)), (3, 16, (2, 1, 1))])
+def test_linear_waves_follow_discrete_dispersion(D, N, m):
+    """Small waves about the true vacuum phi = 1 (mass^2 = V''(1) = 2(c+6))
+    obey the leapfrog dispersion cos(theta) = 1 - dt^2 Omega^2 / 2 with
+    Omega^2 = m^2 + (4/dx^2) sum sin^2(k dx / 2); from the ctor's half back-step
+    the amplitude after n steps is exactly A cos(n theta)."""
+    c, L, A = 4.0, 4.0, 1e-7
+    dx = L / N
+    dt = 0.2 * dx
+    s = oracle.Sim(blank=True, D=D, nGrids=N, L=L, dt=dt, c=c)
+    idx = np.indices((N,) * D)
+    shape = np.cos(sum(2 * np.pi * m[a] * idx[a] / N for a in range(D)))
+    s.phi()[...] = 1.0 + A * shape
+    s.pi()[...] = -0.5 * dt * s.forces()
+    omega2 = 2 * (c + 6) + (4 / dx ** 2) * sum(math.sin(math.pi * m[a] / N) ** 2 for a in range(D))
+    theta = math.acos(1 - 0.5 * dt ** 2 * omega2)
+    for n in (10, 37, 80):
+        s.step(n - (0 if n == 10 else {37: 10, 80: 37}[n]))
+        amp = np.sum((s.phi() - 1.0) * shape) / np.sum(shape * shape)
+        assert abs(amp - A * math.cos(n * theta)) <= 2e-5 * A, (n, amp / A, math.cos(n * theta))
+
+
+def test_de_sitter_scale_factor_closed_form():
+    """False vacuum at rest (no bubbles: phi = 0, V = 1, K = G = 0), expanding:
+    a'' = a / (3 Mpl^2), and the leapfrog of a (src/bubble.cpp:858-865) from
+    the ctor's a' = H0 - dt a''/2 (:273-276) is a_n = cosh(n mu) + B sinh(n mu)
+    with cosh(mu) = 1 + dt^2 / (6 Mpl^2)."""
+    Mpl, dt = 10.0, 0.05
+    s = oracle.Sim(readxns=0, xns=np.zeros((0, 3)), D=3, nGrids=4, L=1.0, dt=dt, c=2.0,
+                   expansion=1, Mpl=Mpl)
+    kappa = 1.0 / (3 * Mpl ** 2)
+    H0 = math.sqrt(kappa)
+    # cosh(mu) - 1 = 2 sinh^2(mu/2) = dt^2 kappa / 2; a_1 - cosh(mu) = dt H0
+    mu = 2 * math.asinh(0.5 * dt * math.sqrt(kappa))
+    B = dt * H0 / math.sinh(mu)
+    for n in range(1, 201):
+        s.step(1)
+        a = s.energy_row()[1]
+        assert abs(a - (math.cosh(n * mu) + B * math.sinh(n * mu))) <= 1e-13 * a
+    # continuum limit: a(t) ~ exp(H0 t)
+    assert abs(a - math.exp(H0 * 200 * dt)) / a < 1e-4
+    row = s.energy_row()
+    assert row[3] == 0.0 and row[4] == 0.0 and abs(row[5] - 1.0) < 1e-14
+
+
+def test_gradient_energy_of_a_plane_wave():
+    """Forward-difference gradient energy (src/bubble.cpp:790-808) of
+    A cos(2 pi m i / N) along one axis: A^2 sin^2(pi m / N) / dx^2."""
+    N, L, A, m = 32, 4.0, 0.01, 3
+    dx = L / N
+    s = oracle.Sim(blank=True, D=2, nGrids=N, L=L, dt=0.2 * dx, c=4.0)
+    i = np.arange(N)[:, None] * np.ones((1, N))
+    s.phi()[...] = A * np.cos(2 * np.pi * m * i / N)
+    s.step(1)   # energies are measured at the incoming phi
+    eg = s.energy_row()[4]
+    assert abs(eg - A ** 2 * math.sin(math.pi * m / N) ** 2 / dx ** 2) <= 1e-13 * eg
+
+
+def test_energy_conservation_static_box():
+    s = oracle.Sim(D=2, nGrids=256, L=16.0, dt=0.0125, c=4.0, readxns=1, nBubbles=5)
+    e0 = s.energy_row()[2]
+    worst = 0.0
+    for _ in range(20):
+        s.step(40)
+        worst = max(worst, abs(s.energy_row()[2] - e0) / e0)
+    assert worst < 2e-3
+
+
+def test_empty_box_energies():
+    s = oracle.Sim(readxns=0, xns=np.zeros((0, 3)), D=3, nGrids=6, L=2.0, dt=0.05, c=4.0)
+    row = s.energy_row()
+    assert abs(row[2] - 1.0) < 1e-14 and row[3] == 0.0 and row[4] == 0.0
+    s.step(3)
+    assert np.all(s.phi() == 0.0) and np.all(s.pi() == 0.0)
+
+
+# --------------------------------------------------------------------------
+# Fourier transforms and spectrum (src/bubble.cpp:642-709)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 8, 12, 15, 16, 49, 64, 97, 100])
+def test_dft_matches_direct_sum(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    k = np.arange(n)
+    W = np.exp(-2j * np.pi * np.outer(k, k) / n)
+    np.testing.assert_allclose(oracle.dft(x), W @ x, atol=1e-13 * np.abs(x).sum())
+    np.testing.assert_allclose(oracle.dft(x, +1), W.conj() @ x, atol=1e-13 * np.abs(x).sum())
+
+
+@pytest.mark.parametrize("shape", [(16,), (15,), (16, 16), (15, 15), (8, 8, 8), (9, 9, 9), (6, 6, 6)])
+def test_r2c_matches_numpy_rfftn(shape):
+    rng = np.random.default_rng(len(shape))
+    f = rng.standard_normal(shape)
+    np.testing.assert_allclose(oracle.r2c(f), np.fft.rfftn(f), atol=1e-13 * np.abs(f).sum())
+
+
+def _shell_counts(D, N, L):
+    """Weights and k sums of each |k| shell by enumerating the half-complex
+    modes with the reference's weights (1 on j = 0 and j = N/2, else 2)."""
+    nb = int(math.sqrt(D) * N / 2) + 1
+    ws, ks = np.zeros(nb), np.zeros(nb)
+    signed = lambda i: i if i <= N // 2 else i - N
+    rows = list(np.ndindex(*((N,) * (D - 1)))) if D > 1 else [()]
+    for r in rows:
+        n2 = sum(signed(i) ** 2 for i in r)
+        for j in range(N // 2 + 1):
+            n = math.sqrt(n2 + j * j)
+            b = int(n)
+            if b >= nb:
+                continue
+            w = 1.0 if (j == 0 or j == N // 2) else 2.0
+            ws[b] += w
+            ks[b] += w * n * 2 * math.pi / L
+    return ws, ks
+
+
+@pytest.mark.parametrize("D,N,m", [(2, 16, (3, 2)), (3, 8, (1, 2, 1)), (2, 15, (4, 3)), (1, 64, (5,)),
+                                   (3, 9, (2, 0, 3))])
+def test_spectrum_of_a_single_mode(D, N, m):
+    L = 2.0
+    s = oracle.Sim(blank=True, D=D, nGrids=N, L=L, dt=0.01, c=4.0)
+    idx = np.indices((N,) * D)
+    s.phi()[...] = np.cos(sum(2 * np.pi * m[a] * idx[a] / N for a in range(D)))
+    k, kp, delta = s.spectrum()
+    ws, ks = _shell_counts(D, N, L)
+    nonempty = ws > 0
+    np.testing.assert_allclose(k, ks[nonempty] / ws[nonempty], rtol=1e-13)
+    np.testing.assert_allclose(kp, k, rtol=0)
+    dx = L / N
+    nm = math.sqrt(sum(x * x for x in m))
+    pref = {1: 1 / math.pi, 2: 1 / (2 * math.pi), 3: 1 / (2 * math.pi ** 2)}[D]
+    Pnorm = dx ** (2 * D) / L ** D
+    w = 1.0 if (m[-1] == 0 or 2 * m[-1] == N) else 2.0
+    amp2 = (N ** D / 2.0) ** 2 * (4.0 if m[-1] == 0 and all(x == 0 for x in m) else 1.0)
+    if m[-1] == 0:
+        amp2 = (N ** D / 2.0) ** 2   # the j = 0 plane stores both k and -k
+        w_total = 2.0
+    else:
+        w_total = w
+    want = w_total * pref * (nm * 2 * math.pi / L) ** D * Pnorm * amp2
+    b = int(nm)
+    full = np.zeros(len(ws))
+    full[nonempty] = delta
+    assert abs(full[b] * ws[b] - want) <= 1e-12 * want
+    others = np.delete(full * ws, b)
+    assert np.all(np.abs(others) <= 1e-20 * want + 1e-25)
+
+
+# --------------------------------------------------------------------------
+# thermal initial conditions (src/bubble.cpp:459-523)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("D,N", [(2, 32), (3, 16), (2, 15), (3, 9)])
+def test_thermal_modes_hermitian_and_c2r_matches_numpy(D, N):
+    L, T, m = 4.0, 0.6, 2.0
+    X = oracle.thermal_modes(D, N, L, T, m, velocity=False)
+    rows = X.reshape(-1, N // 2 + 1)
+    for j in sorted({0, N // 2}):
+        for r in range(rows.shape[0]):
+            idx = np.unravel_index(r, (N,) * (D - 1))
+            pr = np.ravel_multi_index(tuple((-np.array(idx)) % N), (N,) * (D - 1))
+            if pr == r:
+                assert rows[r, j].imag == 0.0
+            else:
+                assert rows[r, j] == np.conj(rows[pr, j])
+    field = oracle.thermal_field(D, N, L, T, m, velocity=False)
+    want = np.fft.irfftn(X, s=(N,) * D, axes=tuple(range(D))) * N ** D / math.sqrt(L ** D)
+    if N % 2 == 0:
+        np.testing.assert_allclose(field, want, atol=1e-13 * np.abs(want).max())
+    # Parseval: sum_x field^2 = N^D L^-D sum over the Hermitian-extended array
+    w = np.full(N // 2 + 1, 2.0)
+    w[0] = 1.0
+    if N % 2 == 0:
+        w[N // 2] = 1.0
+    want_sq = N ** D / L ** D * np.sum(w * np.abs(rows) ** 2)
+    if N % 2 == 0:
+        assert abs(np.sum(field ** 2) - want_sq) <= 1e-12 * want_sq
+
+
+def test_thermal_mode_variance_is_the_bose_einstein_spectrum():
+    D, N, L, T, m = 3, 16, 4.0, 0.8, 2.0
+    for velocity in (False, True):
+        X = oracle.thermal_modes(D, N, L, T, m, velocity=velocity).reshape(-1, N // 2 + 1)
+        ratio, count = 0.0, 0
+        signed = lambda i: i if i <= N // 2 else i - N
+        for r in range(X.shape[0]):
+            i0, i1 = divmod(r, N)
+            for j in range(1, N // 2):   # generic modes: <|X|^2> = P(k)
+                k2 = (2 * math.pi / L) ** 2 * (signed(i0) ** 2 + signed(i1) ** 2 + j * j)
+                om = math.sqrt(k2 + m * m)
+                nk = 1.0 / (math.exp(om / T) - 1.0)
+                P = nk * om if velocity else nk / om
+                ratio += abs(X[r, j]) ** 2 / P
+                count += 1
+        assert abs(ratio / count - 1.0) < 0.05, ratio / count
