@@ -64,14 +64,18 @@ class HipEvents:
         assert self.h.hipDeviceSynchronize() == 0
 
 
-def run_precision(cb, hip, precision, steps, warmup, log):
+def run_precision(cb, hip, precision, steps, warmup, log, variant=None):
     t0 = time.time()
     sim = cb.Bubble(precision=precision, **CONFIG)
     log("%s: seeded %d^3 in %.1f s" % (precision, CONFIG["nGrids"], time.time() - t0))
-    best, ms = sim.autotune(steps=4)
-    nvar, sel = sim.kernel_variants()
-    log("%s: autotune picked variant %d of %d (%.3f ms/step; general kernel %.3f ms/step)"
-        % (precision, sel, nvar, ms[sel] if sel >= 0 else ms[-1], ms[-1]))
+    if variant is None:
+        best, ms = sim.autotune(steps=4)
+        nvar, sel = sim.kernel_variants()
+        log("%s: autotune picked variant %d of %d (%.3f ms/step; general kernel %.3f ms/step)"
+            % (precision, sel, nvar, ms[sel] if sel >= 0 else ms[-1], ms[-1]))
+        log("%s: variant ms/step: %s" % (precision, " ".join("%.3f" % x for x in ms)))
+    else:
+        sim.select_kernel(variant)
     done = 0
     while done < warmup:
         n = min(ENERGY_EVERY, warmup - done)
@@ -136,6 +140,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
+                    help="profiling aid: run one precision only")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="profiling aid: fixed kernel variant instead of autotuning")
+    ap.add_argument("--no-spectrum", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice time-stepper runs on a single MI355X "
@@ -152,11 +161,23 @@ def main():
     sites = N ** 3
     K, W = args.steps, args.warmup
 
-    sim32, ms32, row32 = run_precision(cb, hip, "fp32", K, W, log)
+    copy_gbps = cb.bandwidth_probe(1 << 31, 20)
+    log("device-to-device copy: %.1f GB/s (read + write)" % copy_gbps)
+    if args.precision != "both":
+        sim, ms, row = run_precision(cb, hip, args.precision, K, W, log, args.variant)
+        if not args.no_spectrum:
+            log("%s spectrum: %.3f ms" % (args.precision, spectrum_time(sim, hip)))
+        sim.close()
+        w = 4 if args.precision == "fp32" else 8
+        print(json.dumps({"precision": args.precision, "ms_per_step": ms / K,
+                          "site_updates_per_s": sites * K / (ms * 1e-3),
+                          "achieved_GBps": 4 * w * sites / (ms * 1e-3 / K) / 1e9}), flush=True)
+        return 0
+    sim32, ms32, row32 = run_precision(cb, hip, "fp32", K, W, log, args.variant)
     spec32 = spectrum_time(sim32, hip)
     log("fp32 spectrum: %.3f ms" % spec32)
     sim32.close()
-    sim64, ms64, row64 = run_precision(cb, hip, "fp64", K, W, log)
+    sim64, ms64, row64 = run_precision(cb, hip, "fp64", K, W, log, args.variant)
     spec64 = spectrum_time(sim64, hip)
     log("fp64 spectrum: %.3f ms" % spec64)
 
@@ -201,6 +222,8 @@ def main():
         "fp64": {"value": su64, "ms_per_step": ms64 / K, "achieved_GBps": ach64 / 1e9,
                  "frac": ach64 / HBM_PEAK},
         "fp32_over_fp64": su32 / su64,
+        "measured_copy_GBps": copy_gbps,
+        "frac_of_measured_copy": ach32 / (copy_gbps * 1e9),
         "fp32_energy_rel_err_vs_fp64": err,
         "spectrum": {"fp32_ms": spec32, "fp64_ms": spec64,
                      "fp32_GBps_algorithmic": fft_bytes(4) / (spec32 * 1e-3) / 1e9,

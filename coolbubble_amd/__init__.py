@@ -80,6 +80,8 @@ def lib():
         L.cb_kernel_variants.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.cb_select_kernel.argtypes = [vp, ctypes.c_int]
         L.cb_autotune.argtypes = [vp, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.cb_bandwidth_probe.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp]
+        L.cb_bandwidth_probe.restype = ctypes.c_int
         L.cb_nsites.argtypes = [vp]
         L.cb_nsites.restype = ctypes.c_longlong
         L.cb_destroy.argtypes = [vp]
@@ -113,6 +115,13 @@ def default_params(**kw):
             val = CB_FP32 if val.lower() == "fp32" else CB_FP64
         setattr(p, key, val)
     return p
+
+
+def bandwidth_probe(nbytes=1 << 30, reps=20, device=0):
+    """GB/s (read + write) of a plain device-to-device copy kernel."""
+    g = ctypes.c_double()
+    check(lib().cb_bandwidth_probe(device, int(nbytes), int(reps), ctypes.byref(g)))
+    return g.value
 
 
 class Bubble:

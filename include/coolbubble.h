@@ -142,6 +142,11 @@ cb_status cb_autotune(cb_sim *s, int steps, double *ms_per_step, int cap, int *b
 /* The HIP stream (hipStream_t) every kernel of this handle runs on. */
 cb_status cb_stream(cb_sim *s, void **stream);
 
+/* Measurement helper (no reference counterpart): achieved bandwidth of a
+ * plain 16-byte-vector device-to-device copy of `bytes` bytes, repeated
+ * `reps` times, counting read + write bytes, in GB/s. */
+cb_status cb_bandwidth_probe(int device, long long bytes, int reps, double *gbps);
+
 /* Number of lattice sites N = nGrids^D (src/bubble.hpp:43). */
 long long cb_nsites(const cb_sim *s);
 
