@@ -73,7 +73,10 @@ def run_precision(cb, hip, precision, steps, warmup, log, variant=None):
         nvar, sel = sim.kernel_variants()
         log("%s: autotune picked variant %d of %d (%.3f ms/step; general kernel %.3f ms/step)"
             % (precision, sel, nvar, ms[sel] if sel >= 0 else ms[-1], ms[-1]))
-        log("%s: variant ms/step: %s" % (precision, " ".join("%.3f" % x for x in ms)))
+        log("%s: selected %s" % (precision, sim.variant_info(sel) if sel >= 0 else "general kernel"))
+        order = np.argsort(ms)[:5]
+        log("%s: fastest variants: %s" % (precision, "; ".join(
+            "%d %.4f ms %s" % (v, ms[v], sim.variant_info(v) if v < nvar else "general") for v in order)))
     else:
         sim.select_kernel(variant)
     done = 0

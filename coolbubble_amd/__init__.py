@@ -79,6 +79,8 @@ def lib():
         L.cb_stream.argtypes = [vp, ctypes.POINTER(vp)]
         L.cb_kernel_variants.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.cb_select_kernel.argtypes = [vp, ctypes.c_int]
+        L.cb_kernel_variant_info.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.cb_kernel_variant_info.restype = ctypes.c_int
         L.cb_autotune.argtypes = [vp, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.cb_bandwidth_probe.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp]
         L.cb_bandwidth_probe.restype = ctypes.c_int
@@ -207,6 +209,12 @@ class Bubble:
         n, sel = ctypes.c_int(), ctypes.c_int()
         check(lib().cb_kernel_variants(self._h, ctypes.byref(n), ctypes.byref(sel)))
         return n.value, sel.value
+
+    def variant_info(self, variant):
+        """dict(vw, items, nthr, TR, NI, pf, nt) of a tiled variant."""
+        info = (ctypes.c_int * 7)()
+        check(lib().cb_kernel_variant_info(self._h, int(variant), info))
+        return dict(zip(("vw", "items", "nthr", "TR", "NI", "pf", "nt"), info[:]))
 
     def select_kernel(self, variant):
         check(lib().cb_select_kernel(self._h, int(variant)))

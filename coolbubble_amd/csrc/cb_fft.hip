@@ -265,15 +265,25 @@ k_bin(const cpx<R> *__restrict__ H, long long rows, int N, int D, int nBins, dou
     for (int b = threadIdx.x; b < 3 * nBins; b += blockDim.x) dst[b] = hp[b];
 }
 
-// Fixed-order sum of the per-block bin partials.
+// Fixed-order sum of the per-block bin partials: 64 bins per block, the
+// partials split over the 4 waves, then the 4 wave sums merged in order.
 __global__ void __launch_bounds__(256)
 k_bin_reduce(const double *__restrict__ binpart, int nblk, int nq, double *__restrict__ out)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
+    __shared__ double ss[4][64], sc[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int q = blockIdx.x * 64 + lane;
     Neu s = {0.0, 0.0};
-    for (int b = 0; b < nblk; ++b) s.add(binpart[(long long)b * nq + q]);
-    out[q] = s.value();
+    if (q < nq)
+        for (int b = w; b < nblk; b += 4) s.add(binpart[(long long)b * nq + q]);
+    ss[w][lane] = s.s;
+    sc[w][lane] = s.c;
+    __syncthreads();
+    if (w == 0 && q < nq) {
+        Neu t = {ss[0][lane], sc[0][lane]};
+        for (int k = 1; k < 4; ++k) t.merge(Neu{ss[k][lane], sc[k][lane]});
+        out[q] = t.value();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -516,7 +526,7 @@ static cb_status plan_for(cb_sim *s)
     p->nBins = s->nkmax + 1;
     long long modes = p->rows * nc;
     p->binblocks = (int)((modes + 4095) / 4096);
-    if (p->binblocks > 1024) p->binblocks = 1024;
+    if (p->binblocks > 256) p->binblocks = 256;
     if (p->binblocks < 1) p->binblocks = 1;
     if (e == hipSuccess) e = hipMalloc((void **)&p->binpart, sizeof(double) * 3 * (size_t)p->nBins * p->binblocks);
     if (e == hipSuccess) e = hipMalloc((void **)&p->bins, sizeof(double) * 3 * (size_t)p->nBins);
@@ -550,7 +560,7 @@ cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int c
     if (s->fp32) k_bin<float><<<p->binblocks, 256, shm, s->st>>>((const cpx<float> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm, p->binpart);
     else k_bin<double><<<p->binblocks, 256, shm, s->st>>>((const cpx<double> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm, p->binpart);
     HIP_TRY(hipGetLastError());
-    k_bin_reduce<<<(3 * nB + 255) / 256, 256, 0, s->st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
+    k_bin_reduce<<<(3 * nB + 63) / 64, 256, 0, s->st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
     HIP_TRY(hipGetLastError());
     std::vector<double> h(3 * (size_t)nB);
     HIP_TRY(hipMemcpyAsync(h.data(), p->bins, sizeof(double) * 3 * nB, hipMemcpyDeviceToHost, s->st));

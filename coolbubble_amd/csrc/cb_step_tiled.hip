@@ -29,21 +29,41 @@ struct alignas(sizeof(T) * VW) Vec {
     T v[VW];
 };
 
-template <typename T, int D, int VW, int ITEMS, int NTHR, bool LAGGED, bool ENERGY>
-__global__ void __launch_bounds__(NTHR)
-k_step_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restrict__ phi_out,
-             const T *pi_in, T *pi_out, const DevScalars *__restrict__ sc, double *__restrict__ partials)
+// Wait for this wave's LDS traffic, then the workgroup barrier.  The kernels
+// communicate through LDS only (global data is read from buffers nobody
+// writes during the launch), so no global-memory fence is needed.
+__device__ inline void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <typename T, int VW>
+__device__ inline void store_vec(T *p, const Vec<T, VW> &v, bool nt)
+{
+    if (nt) {
+        typedef T nvec __attribute__((ext_vector_type(VW)));
+        nvec x;
+#pragma unroll
+        for (int e = 0; e < VW; ++e) x[e] = v.v[e];
+        __builtin_nontemporal_store(x, reinterpret_cast<nvec *>(p));
+    } else {
+        *reinterpret_cast<Vec<T, VW> *>(p) = v;
+    }
+}
+
+// One tile of one step: TR rows x all N columns of every plane i in
+// [lt's i-chunk].  lds holds two plane tiles (double buffer).  Ends with an
+// LDS barrier so a caller may run the next tile on the same LDS.
+template <typename T, int D, int VW, int ITEMS, int NTHR, bool LAGGED, bool ENERGY, int PF>
+__device__ inline void tile_step(const SimConst &k, const TiledGeom &g, int lt, const T *__restrict__ phi_in,
+                                 T *__restrict__ phi_out, const T *pi_in, T *pi_out,
+                                 const DevScalars *__restrict__ sc, double *__restrict__ partials, T *lds)
 {
     using V = Vec<T, VW>;
     using C = T;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *lds = reinterpret_cast<T *>(smem_raw);
-
     const int N = k.N;
     const int NV = g.NV;
     const long long NN = (D == 3) ? (long long)N * N : (long long)N;
-    const int G = gridDim.x, b = blockIdx.x;
-    const int lt = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;   // XCD-contiguous tiles
     const int TRv = (D == 3) ? g.TR : 1;
     const int rt = (D == 3) ? lt % g.ntr : 0;
     const int ic = (D == 3) ? lt / g.ntr : lt;
@@ -51,6 +71,8 @@ k_step_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restric
     const int i0 = ic * g.NI;
     const int i1 = min(i0 + g.NI, N);
     const int nvec = TRv * NV;
+    const int lds_plane = (D == 3) ? (TRv + 2) * N : N;
+    const bool nt = g.nt != 0;
 
     const C dt = (C)k.dt;
     const C s_prev = LAGGED ? (C)sc->s_pending : (C)0;
@@ -70,66 +92,93 @@ k_step_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restric
         col[u] = (qq - rowq[u] * NV) * VW;
         off[u] = (D == 3) ? (long long)(r0 + rowq[u]) * N + col[u] : (long long)col[u];
     }
+    // halo rows (3D): vector h of 2 NV, row r0-1 (h < NV) or r0+TR
+    bool hok[ITEMS];
+    long long hoff[ITEMS];
+    int hlds[ITEMS];
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+        const int h = threadIdx.x + u * NTHR;
+        hok[u] = (D == 3) && h < 2 * NV;
+        const int hh = hok[u] ? h : 0;
+        const int side = hh >= NV;
+        const int c0 = (hh - side * NV) * VW;
+        const int grow = side ? ((r0 + TRv == N) ? 0 : r0 + TRv) : (r0 == 0 ? N - 1 : r0 - 1);
+        hoff[u] = (long long)grow * N + c0;
+        hlds[u] = (side ? TRv + 1 : 0) * N + c0;
+    }
 
-    V prev[ITEMS], cur[ITEMS], nxt[ITEMS];
-    V pcur[ITEMS], pnxt[ITEMS];   // incoming pi of the current / next plane (lagged)
-
+    auto wrap = [&](int i) { return i >= N ? i - N : (i < 0 ? i + N : i); };
+    auto load_v = [&](long long idx, V &out, V &pout) {
+        V f = *reinterpret_cast<const V *>(phi_in + idx);
+        if constexpr (LAGGED) {
+            V p = *reinterpret_cast<const V *>(pi_in + idx);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) f.v[e] = f.v[e] + dt * p.v[e] * s_prev;
+            pout = p;
+        }
+        out = f;
+    };
     auto load_plane = [&](int i, V (&out)[ITEMS], V (&pout)[ITEMS]) {
         const long long base = (long long)i * NN;
 #pragma unroll
-        for (int u = 0; u < ITEMS; ++u) {
-            if (!ok[u]) continue;
-            V f = *reinterpret_cast<const V *>(phi_in + base + off[u]);
-            if constexpr (LAGGED) {
-                V p = *reinterpret_cast<const V *>(pi_in + base + off[u]);
-#pragma unroll
-                for (int e = 0; e < VW; ++e) f.v[e] = f.v[e] + dt * p.v[e] * s_prev;
-                pout[u] = p;
-            }
-            out[u] = f;
-        }
+        for (int u = 0; u < ITEMS; ++u)
+            if (ok[u]) load_v(base + off[u], out[u], pout[u]);
     };
+    auto load_halo = [&](int i, V (&out)[ITEMS]) {
+        const long long base = (long long)i * NN;
+        V dummy;
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u)
+            if (hok[u]) load_v(base + hoff[u], out[u], dummy);
+    };
+
+    V prev[ITEMS], cur[ITEMS], nxt[ITEMS], nx2[ITEMS];
+    V pcur[ITEMS], pnxt[ITEMS], pnx2[ITEMS];
+    V hcur[ITEMS], hnxt[ITEMS];
 
     double acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
 
-    load_plane(i0 == 0 ? N - 1 : i0 - 1, prev, pnxt);
+    load_plane(wrap(i0 - 1), prev, pnxt);
     load_plane(i0, cur, pcur);
+    if constexpr (PF == 2) {
+        load_plane(wrap(i0 + 1), nxt, pnxt);
+        if constexpr (D == 3) load_halo(i0, hcur);
+    }
 
+    int buf = 0;
     for (int i = i0; i < i1; ++i) {
-        const int ip = (i + 1 == N) ? 0 : i + 1;
-        load_plane(ip, nxt, pnxt);
         const long long base = (long long)i * NN;
+        if constexpr (PF == 2) {
+            if (i + 1 < i1) {
+                load_plane(wrap(i + 2), nx2, pnx2);
+                if constexpr (D == 3) load_halo(i + 1, hnxt);
+            }
+        } else {
+            load_plane(wrap(i + 1), nxt, pnxt);
+            if constexpr (D == 3) load_halo(i, hcur);
+        }
         if constexpr (!LAGGED) {
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u)
                 if (ok[u]) pcur[u] = *reinterpret_cast<const V *>(pi_in + base + off[u]);
         }
-        // stage plane i (tile rows at 1..TR, halo rows at 0 and TR+1)
+        T *L = lds + buf * lds_plane;
 #pragma unroll
         for (int u = 0; u < ITEMS; ++u)
-            if (ok[u]) *reinterpret_cast<V *>(lds + (D == 3 ? (rowq[u] + 1) * N : 0) + col[u]) = cur[u];
+            if (ok[u]) *reinterpret_cast<V *>(L + (D == 3 ? (rowq[u] + 1) * N : 0) + col[u]) = cur[u];
         if constexpr (D == 3) {
-            for (int h = threadIdx.x; h < 2 * NV; h += NTHR) {
-                const int side = h >= NV;
-                const int c0 = (h - side * NV) * VW;
-                const int grow = side ? ((r0 + TRv == N) ? 0 : r0 + TRv) : (r0 == 0 ? N - 1 : r0 - 1);
-                const long long idx = base + (long long)grow * N + c0;
-                V f = *reinterpret_cast<const V *>(phi_in + idx);
-                if constexpr (LAGGED) {
-                    V p = *reinterpret_cast<const V *>(pi_in + idx);
 #pragma unroll
-                    for (int e = 0; e < VW; ++e) f.v[e] = f.v[e] + dt * p.v[e] * s_prev;
-                }
-                *reinterpret_cast<V *>(lds + (side ? TRv + 1 : 0) * N + c0) = f;
-            }
+            for (int u = 0; u < ITEMS; ++u)
+                if (hok[u]) *reinterpret_cast<V *>(L + hlds[u]) = hcur[u];
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int u = 0; u < ITEMS; ++u) {
             if (!ok[u]) continue;
-            const T *lrow = lds + (D == 3 ? (rowq[u] + 1) * N : 0);
+            const T *lrow = L + (D == 3 ? (rowq[u] + 1) * N : 0);
             V rm, rp;
             if constexpr (D == 3) {
                 rm = *reinterpret_cast<const V *>(lrow - N + col[u]);
@@ -163,18 +212,36 @@ k_step_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restric
                     energy_terms<D>((double)c, fwd, (double)pnew, (double)po.v[e], k, sc, acc);
                 }
             }
-            *reinterpret_cast<V *>(pi_out + base + off[u]) = pn;
-            *reinterpret_cast<V *>(phi_out + base + off[u]) = phn;
+            store_vec<T, VW>(pi_out + base + off[u], pn, nt);
+            store_vec<T, VW>(phi_out + base + off[u], phn, nt);
         }
-        __syncthreads();
+        buf ^= 1;
 #pragma unroll
         for (int u = 0; u < ITEMS; ++u) {
             prev[u] = cur[u];
             cur[u] = nxt[u];
             if constexpr (LAGGED) pcur[u] = pnxt[u];
+            if constexpr (PF == 2) {
+                nxt[u] = nx2[u];
+                if constexpr (LAGGED) pnxt[u] = pnx2[u];
+                if constexpr (D == 3) hcur[u] = hnxt[u];
+            }
         }
     }
     if constexpr (ENERGY) block_partials<NQ>(acc, partials + (long long)lt * NQ);
+    lds_barrier();
+}
+
+template <typename T, int D, int VW, int ITEMS, int NTHR, bool LAGGED, bool ENERGY, int PF>
+__global__ void __launch_bounds__(NTHR)
+k_step_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restrict__ phi_out,
+             const T *pi_in, T *pi_out, const DevScalars *__restrict__ sc, double *__restrict__ partials)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int G = gridDim.x, b = blockIdx.x;
+    const int lt = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;   // XCD-contiguous tiles
+    tile_step<T, D, VW, ITEMS, NTHR, LAGGED, ENERGY, PF>(k, g, lt, phi_in, phi_out, pi_in, pi_out, sc, partials,
+                                                         reinterpret_cast<T *>(smem_raw));
 }
 
 // ---------------------------------------------------------------------------
@@ -188,6 +255,7 @@ static TiledGeom geom(const SimConst &k, const TiledCfg &c)
     g.NV = k.N / c.vw;
     g.ntr = (k.D == 3) ? k.N / c.TR : 1;
     g.nic = (k.N + c.NI - 1) / c.NI;
+    g.nt = c.nt;
     return g;
 }
 
@@ -195,7 +263,7 @@ static size_t lds_bytes(const SimConst &k, int fp32, const TiledCfg &c)
 {
     const size_t esz = fp32 ? 4 : 8;
     const int rows = (k.D == 3) ? c.TR + 2 : 1;
-    return (size_t)rows * k.N * esz;
+    return 2 * (size_t)rows * k.N * esz;     // double-buffered plane tile
 }
 
 bool tiled_valid(const SimConst &k, int fp32, const TiledCfg &c)
@@ -211,6 +279,9 @@ bool tiled_valid(const SimConst &k, int fp32, const TiledCfg &c)
     const int rows = (k.D == 3) ? c.TR : 1;
     if ((long long)rows * (k.N / c.vw) > (long long)c.items * c.nthr) return false;
     if (lds_bytes(k, fp32, c) > 65536) return false;
+    if (c.pf != 1 && c.pf != 2) return false;
+    if (c.nt != 0 && c.nt != 1) return false;
+    if (k.D == 3 && 2 * (k.N / c.vw) > c.items * c.nthr) return false;   // halo rows need owners
     return true;
 }
 
@@ -225,12 +296,15 @@ int tiled_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
     int n = 0;
     const int vw = fp32 ? 4 : 2;
     auto push = [&](int items, int nthr, int TR, int NI) {
-        TiledCfg c{vw, items, nthr, TR, NI};
-        if (n < cap && tiled_valid(k, fp32, c)) out[n++] = c;
+        for (int pf : {2, 1})
+            for (int nt : {0, 1}) {
+                TiledCfg c{vw, items, nthr, TR, NI, pf, nt};
+                if (n < cap && tiled_valid(k, fp32, c)) out[n++] = c;
+            }
     };
     if (k.D == 3) {
         const int trs[] = {8, 4, 2, 16, 1};
-        const int nis[] = {32, 16, 64, 8};
+        const int nis[] = {32, 16, 64, 8, 128};
         for (int TR : trs)
             for (int nthr : {256, 128}) {
                 const int nv = TR * (k.N / vw);
@@ -250,7 +324,7 @@ int tiled_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
     return n;
 }
 
-template <typename T, int D, int ITEMS, int NTHR>
+template <typename T, int D, int ITEMS, int NTHR, int PF>
 static void launch_t(const SimConst &k, int lagged, int energy, const TiledCfg &c, const StepPtrs &p,
                      const DevScalars *sc, double *partials, hipStream_t st, size_t shm)
 {
@@ -263,28 +337,36 @@ static void launch_t(const SimConst &k, int lagged, int energy, const TiledCfg &
     T *qout = (T *)p.pi_out;
     if constexpr (D == 3) {
         if (lagged) {
-            if (energy) k_step_tiled<T, D, VW, ITEMS, NTHR, true, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
-            else k_step_tiled<T, D, VW, ITEMS, NTHR, true, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            if (energy) k_step_tiled<T, D, VW, ITEMS, NTHR, true, true, PF><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            else k_step_tiled<T, D, VW, ITEMS, NTHR, true, false, PF><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
             return;
         }
     }
-    if (energy) k_step_tiled<T, D, VW, ITEMS, NTHR, false, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
-    else k_step_tiled<T, D, VW, ITEMS, NTHR, false, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+    if (energy) k_step_tiled<T, D, VW, ITEMS, NTHR, false, true, PF><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+    else k_step_tiled<T, D, VW, ITEMS, NTHR, false, false, PF><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+}
+
+template <typename T, int D, int PF>
+static void launch_p(const SimConst &k, int lagged, int energy, const TiledCfg &c, const StepPtrs &p,
+                     const DevScalars *sc, double *partials, hipStream_t st, size_t shm)
+{
+    if (c.nthr == 256) {
+        if (c.items == 1) launch_t<T, D, 1, 256, PF>(k, lagged, energy, c, p, sc, partials, st, shm);
+        else if (c.items == 2) launch_t<T, D, 2, 256, PF>(k, lagged, energy, c, p, sc, partials, st, shm);
+        else launch_t<T, D, 4, 256, PF>(k, lagged, energy, c, p, sc, partials, st, shm);
+    } else {
+        if (c.items == 1) launch_t<T, D, 1, 128, PF>(k, lagged, energy, c, p, sc, partials, st, shm);
+        else if (c.items == 2) launch_t<T, D, 2, 128, PF>(k, lagged, energy, c, p, sc, partials, st, shm);
+        else launch_t<T, D, 4, 128, PF>(k, lagged, energy, c, p, sc, partials, st, shm);
+    }
 }
 
 template <typename T, int D>
 static void launch_d(const SimConst &k, int lagged, int energy, const TiledCfg &c, const StepPtrs &p,
                      const DevScalars *sc, double *partials, hipStream_t st, size_t shm)
 {
-    if (c.nthr == 256) {
-        if (c.items == 1) launch_t<T, D, 1, 256>(k, lagged, energy, c, p, sc, partials, st, shm);
-        else if (c.items == 2) launch_t<T, D, 2, 256>(k, lagged, energy, c, p, sc, partials, st, shm);
-        else launch_t<T, D, 4, 256>(k, lagged, energy, c, p, sc, partials, st, shm);
-    } else {
-        if (c.items == 1) launch_t<T, D, 1, 128>(k, lagged, energy, c, p, sc, partials, st, shm);
-        else if (c.items == 2) launch_t<T, D, 2, 128>(k, lagged, energy, c, p, sc, partials, st, shm);
-        else launch_t<T, D, 4, 128>(k, lagged, energy, c, p, sc, partials, st, shm);
-    }
+    if (c.pf == 2) launch_p<T, D, 2>(k, lagged, energy, c, p, sc, partials, st, shm);
+    else launch_p<T, D, 1>(k, lagged, energy, c, p, sc, partials, st, shm);
 }
 
 hipError_t launch_step_tiled_cfg(const SimConst &k, int fp32, int lagged, int energy, const TiledCfg &c,
@@ -311,7 +393,9 @@ int tiled_register(const TiledCfg &c)
     std::lock_guard<std::mutex> lk(g_reg_mu);
     for (size_t i = 0; i < g_reg.size(); ++i) {
         const TiledCfg &o = g_reg[i];
-        if (o.vw == c.vw && o.items == c.items && o.nthr == c.nthr && o.TR == c.TR && o.NI == c.NI) return (int)i;
+        if (o.vw == c.vw && o.items == c.items && o.nthr == c.nthr && o.TR == c.TR && o.NI == c.NI &&
+            o.pf == c.pf && o.nt == c.nt)
+            return (int)i;
     }
     g_reg.push_back(c);
     return (int)g_reg.size() - 1;

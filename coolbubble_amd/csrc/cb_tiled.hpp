@@ -8,18 +8,21 @@ namespace cb {
 // One launch shape of the tiled kernel.  A block owns TR rows (3D; 1 in 2D)
 // of every plane of an i-chunk of NI planes and marches through the chunk:
 // the planes i-1, i, i+1 of its own columns sit in registers, the plane i
-// tile plus one halo row on each side sits in LDS for the j and k taps.
+// tile plus one halo row on each side sits in LDS (double-buffered) for the j
+// and k taps.
 struct TiledCfg {
     int vw;        // elements per vector load (16 B: 4 float / 2 double); 1 = scalar
     int items;     // vectors per thread per plane (template: 1, 2, 4)
     int nthr;      // threads per block (template: 128, 256)
     int TR;        // rows per tile (3D)
     int NI;        // planes per block
+    int pf;        // 2: next-but-one plane (and next halo rows) prefetched into registers; 1: none
+    int nt;        // 1: non-temporal stores of phi and pi
 };
 
 // Geometry of one launch, derived from (N, cfg).
 struct TiledGeom {
-    int TR, NI, ntr, nic, NV;
+    int TR, NI, ntr, nic, NV, nt;
 };
 
 // Is cfg usable for this lattice?  (vector width divides N, tile fits LDS,

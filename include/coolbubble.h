@@ -132,6 +132,10 @@ cb_status cb_get_xns(cb_sim *s, double *xyz, int cap, int *n);
  * cb_create selects variant 0 when one exists. */
 cb_status cb_kernel_variants(cb_sim *s, int *count, int *selected);
 cb_status cb_select_kernel(cb_sim *s, int variant);
+/* Shape of tiled variant `variant`: info = {vector width, vectors per thread,
+ * threads per block, rows per tile, planes per block, prefetch depth,
+ * non-temporal stores}. */
+cb_status cb_kernel_variant_info(cb_sim *s, int variant, int info[7]);
 
 /* Autotune: time each variant over `steps` steps (the lattice state is
  * saved and restored around the trials) and select the fastest.
