@@ -69,7 +69,7 @@ def run_precision(cb, hip, precision, steps, warmup, log, variant=None):
     sim = cb.Bubble(precision=precision, **CONFIG)
     log("%s: seeded %d^3 in %.1f s" % (precision, CONFIG["nGrids"], time.time() - t0))
     if variant is None:
-        best, ms = sim.autotune(steps=4)
+        best, ms = sim.autotune(steps=12)
         nvar, sel = sim.kernel_variants()
         log("%s: autotune picked variant %d of %d (%.3f ms/step; general kernel %.3f ms/step)"
             % (precision, sel, nvar, ms[sel] if sel >= 0 else ms[-1], ms[-1]))
@@ -111,6 +111,46 @@ def spectrum_time(sim, hip, reps=3):
     return hip.elapsed_ms(a, b) / reps
 
 
+def ensemble_members():
+    """64 thermal d=3 realisations (N=128, L=16, c in {1,2,4,8} x 16 seeds,
+    Trad = 0.1) and 16 d=2 five-bubble runs (N=1024, L=64, c ~ U[1,8] from a
+    seeded generator), static, FP32, dt = 0.2 dx."""
+    rng = np.random.default_rng(2024)
+    out = []
+    for c in (1.0, 2.0, 4.0, 8.0):
+        for s in range(16):
+            out.append(dict(D=3, nGrids=128, L=16.0, dt=0.025, c=c, readxns=-3, Trad=0.1,
+                            thermal_base=12345 + s, precision="fp32"))
+    for c in rng.uniform(1.0, 8.0, 16):
+        out.append(dict(D=2, nGrids=1024, L=64.0, dt=0.0125, c=float(c), readxns=1, nBubbles=5,
+                        precision="fp32"))
+    return out
+
+
+def run_ensemble(cb, hip, nt, log):
+    t0 = time.time()
+    batch = cb.Batch(ensemble_members())
+    log("ensemble: %d members, %d sites, created in %.1f s" % (len(batch.members), batch.sites,
+                                                                time.time() - t0))
+    batch.step(40)   # graph capture + warm
+    stream = batch.stream()
+    a, b = hip.event(), hip.event()
+    hip.sync()
+    hip.record(a, stream)
+    batch.step(nt)
+    hip.record(b, stream)
+    ms = hip.elapsed_ms(a, b)
+    rows = [m.energy_row() for m in batch.members]
+    assert all(np.all(np.isfinite(r)) for r in rows)
+    batch.close()
+    log("ensemble: %d steps of all members in %.1f ms" % (nt, ms))
+    return {"members": 80, "steps": nt, "wall_ms": ms, "runs_per_hour": 80 * 3600.0 / (ms * 1e-3),
+            "site_updates_per_s": batch.sites * nt / (ms * 1e-3),
+            "config": "64 x d=3 N=128 L=16 thermal (c in {1,2,4,8} x 16 seeds, Trad=0.1) + "
+                      "16 x d=2 N=1024 L=64 five bubbles (c ~ U[1,8]); FP32, static, "
+                      "energies every 40 steps, 40-step HIP graph segments"}
+
+
 def cpu_baseline(sim64, nsteps, log):
     """Oracle (FP64 C/OpenMP restatement, 4 passes per step with energies
     every step, like the reference) stepping the same lattice from the GPU's
@@ -148,6 +188,8 @@ def main():
     ap.add_argument("--variant", type=int, default=None,
                     help="profiling aid: fixed kernel variant instead of autotuning")
     ap.add_argument("--no-spectrum", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice time-stepper runs on a single MI355X "
@@ -203,6 +245,7 @@ def main():
                "sample": "%d oracle steps (FP64, energies every step) of the same 512^3 lattice "
                          "from the GPU-seeded state" % args.cpu_steps}
     sim64.close()
+    ens = None if args.no_ensemble else run_ensemble(cb, hip, args.ensemble_steps, log)
     out = {
         "metric": "site-updates/s (FP32 stepper, d=3 N=512 static, mode 1 32 bubbles)",
         "value": su32,
@@ -225,6 +268,7 @@ def main():
         "fp64": {"value": su64, "ms_per_step": ms64 / K, "achieved_GBps": ach64 / 1e9,
                  "frac": ach64 / HBM_PEAK},
         "fp32_over_fp64": su32 / su64,
+        "ensemble": ens,
         "measured_copy_GBps": copy_gbps,
         "frac_of_measured_copy": ach32 / (copy_gbps * 1e9),
         "fp32_energy_rel_err_vs_fp64": err,

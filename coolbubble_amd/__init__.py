@@ -84,6 +84,15 @@ def lib():
         L.cb_autotune.argtypes = [vp, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.cb_bandwidth_probe.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp]
         L.cb_bandwidth_probe.restype = ctypes.c_int
+        L.cb_batch_create.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(vp)]
+        L.cb_batch_step.argtypes = [vp, ctypes.c_int]
+        L.cb_batch_member.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+        L.cb_batch_size.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]
+        L.cb_batch_stream.argtypes = [vp, ctypes.POINTER(vp)]
+        L.cb_batch_destroy.argtypes = [vp]
+        for name in ("cb_batch_create", "cb_batch_step", "cb_batch_member", "cb_batch_size",
+                     "cb_batch_stream", "cb_batch_destroy"):
+            getattr(L, name).restype = ctypes.c_int
         L.cb_nsites.argtypes = [vp]
         L.cb_nsites.restype = ctypes.c_longlong
         L.cb_destroy.argtypes = [vp]
@@ -129,8 +138,13 @@ def bandwidth_probe(nbytes=1 << 30, reps=20, device=0):
 class Bubble:
     """One simulation on the GPU: Bubble::Bubble + oneStep + outEP + spectrum."""
 
-    def __init__(self, params=None, xns=None, **kw):
+    def __init__(self, params=None, xns=None, _handle=None, **kw):
         self.p = params if params is not None else default_params(**kw)
+        self._owned = _handle is None
+        if _handle is not None:
+            self._h = _handle
+            self.nsites = lib().cb_nsites(_handle)
+            return
         h = ctypes.c_void_p()
         if xns is not None:
             xa = np.ascontiguousarray(np.asarray(xns, dtype=np.float64).reshape(-1, 3))
@@ -142,7 +156,8 @@ class Bubble:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().cb_destroy(self._h)
+            if self._owned:
+                lib().cb_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -232,3 +247,45 @@ class Bubble:
         s = ctypes.c_void_p()
         check(lib().cb_stream(self._h, ctypes.byref(s)))
         return s
+
+
+class Batch:
+    """An ensemble stepped together (cb_batch_*): one persistent launch per
+    step over all members, 40-step segments replayed as a HIP graph."""
+
+    def __init__(self, members):
+        plist = [m if isinstance(m, Params) else default_params(**m) for m in members]
+        arr = (Params * len(plist))(*plist)
+        h = ctypes.c_void_p()
+        check(lib().cb_batch_create(arr, len(plist), ctypes.byref(h)))
+        self._h = h
+        self.params = plist
+        self.members = []
+        for i, pp in enumerate(plist):
+            mh = ctypes.c_void_p()
+            check(lib().cb_batch_member(h, i, ctypes.byref(mh)))
+            self.members.append(Bubble(params=pp, _handle=mh))
+        n, sites = ctypes.c_int(), ctypes.c_longlong()
+        check(lib().cb_batch_size(h, ctypes.byref(n), ctypes.byref(sites)))
+        self.sites = sites.value
+
+    def step(self, n):
+        check(lib().cb_batch_step(self._h, int(n)))
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        check(lib().cb_batch_stream(self._h, ctypes.byref(s)))
+        return s
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for m in self.members:
+                m._h = None
+            lib().cb_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -155,4 +155,54 @@ __device__ inline void energy_terms(double ph, const double (&fwd)[3], double pn
     acc[Q_KIN_OLD] += 0.5 * pk2 * po * po * dxLD;
 }
 
+// Energies of the step and the scalar chain of oneStep, src/bubble.cpp:839-875.
+__device__ inline void finalize_member(const SimConst &k, const double *partials, int nblocks, DevScalars *sc,
+                                       int use_old_kin)
+{
+    __shared__ double tot[NQ];
+    sum_partials_block(partials, nblocks, tot);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    DevScalars s = *sc;
+    const double EkinHalf = tot[Q_KIN];
+    if (use_old_kin) s.EkinHalfPrev = tot[Q_KIN_OLD];
+    s.Ekintot = 0.5 * (EkinHalf + s.EkinHalfPrev);
+    s.EkinHalfPrev = EkinHalf;
+    s.Epottot = tot[Q_POT];
+    s.Egradtot = tot[Q_GRAD];
+    s.avphi = tot[Q_PHI];
+    s.avphiP = tot[Q_PHIP];
+    s.EkintotVar = tot[Q_VKIN];
+    s.EpottotVar = tot[Q_VPOT];
+    s.EgradtotVar = tot[Q_VGRAD];
+    s.EtotVar = tot[Q_VTOT];
+    s.avphiVar = tot[Q_VPHI];
+    s.avphiPVar = tot[Q_VPHIP];
+    s.Etot = s.Ekintot + s.Epottot + s.Egradtot;
+    // addot, src/bubble.cpp:747-757
+    double addot = 0.0;
+    if (k.expanding) {
+        if (k.alpha == 0) addot = -s.a * (2.0 * s.Ekintot - s.Epottot) / (3.0 * k.Mpl * k.Mpl);
+        else addot = s.a * s.a * s.a * (2 * s.Epottot - s.Ekintot + s.Egradtot) / (3.0 * k.Mpl * k.Mpl);
+    }
+    const double ap_prev = s.ap;
+    s.ap += k.dt * addot;
+    const double ap_atn = (ap_prev + s.ap) / 2.0;
+    const double a_old = s.a;
+    s.a += k.dt * s.ap;
+    const double a_half = 0.5 * (s.a + a_old);
+    s.s_pending = ipow_rn(a_half, -k.D + k.alpha);
+    s.H = ap_atn / a_old;
+    s.C = k.expanding ? s.H * s.H - ipow_rn(a_old, 2 * k.alpha) * s.Etot / (3.0 * k.Mpl * k.Mpl) : 0.0;
+    // factors of the next step's force and energies (src/bubble.cpp:741-742,765)
+    s.fac_lap = ipow_rn(s.a, k.D - 2 + k.alpha);
+    s.fac_pot = ipow_rn(s.a, k.D + k.alpha);
+    s.adx = s.a * k.dx;
+    const double a_kin = s.a + 0.5 * k.dt * s.ap;
+    s.pk2 = ipow_rn(a_kin, -2 * k.D);
+    s.pk1 = ipow_rn(a_kin, -k.D);
+    if (!isfinite(s.Etot)) s.nan_flag = 1;
+    *sc = s;
+}
+
 } // namespace cb

@@ -127,7 +127,7 @@ void release(cb_sim *s)
     if (s->partials) hipFree(s->partials);
     if (s->dtot) hipFree(s->dtot);
     if (s->dsc) hipFree(s->dsc);
-    if (s->st) hipStreamDestroy(s->st);
+    if (s->st && s->own_stream) hipStreamDestroy(s->st);
     delete s;
 }
 
@@ -164,7 +164,10 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
     }
     if (p.device < 0 || p.device >= ndev) { set_error("device ordinal out of range"); return CB_INVALID_ARG; }
     HIP_TRY(hipSetDevice(p.device));
-    HIP_TRY(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    if (!s->st) {
+        HIP_TRY(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        s->own_stream = true;
+    }
 
     // lattice constants (src/bubble.hpp:43-49)
     s->nsites = (long long)std::pow((double)g, D);
@@ -327,6 +330,12 @@ cb_status materialise(cb_sim *s)
 }
 
 } // namespace
+
+namespace cb {
+cb_status construct_sim(cb_sim *s, const double *xns, int n_xns) { return construct(s, xns, n_xns); }
+void release_sim(cb_sim *s) { release(s); }
+bool params_ok(const cb_params *p, std::string &why) { return valid_params(p, why); }
+} // namespace cb
 
 extern "C" {
 
@@ -547,6 +556,10 @@ long long cb_nsites(const cb_sim *s) { return s ? s->nsites : 0; }
 
 cb_status cb_destroy(cb_sim *s)
 {
+    if (s && s->batch_owned) {
+        set_error("this simulation belongs to a batch: destroy the batch");
+        return CB_INVALID_ARG;
+    }
     release(s);
     return CB_OK;
 }

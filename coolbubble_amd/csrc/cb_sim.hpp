@@ -26,6 +26,8 @@ struct cb_sim {
     int nkmax = 0;              // src/bubble.hpp:49
     double dxD = 0, phiC = 0, Rn = 0, lw = 0, mT = 0, phi0 = 1.0;
     hipStream_t st = nullptr;
+    bool own_stream = false;    // false when the stream belongs to a batch / server
+    bool batch_owned = false;   // member of a cb_batch (destroyed with it)
     void *phi[2] = {nullptr, nullptr};
     void *pi[2] = {nullptr, nullptr};
     int iphi = 0, ipi = 0;
@@ -49,6 +51,9 @@ namespace cb {
 // spectrum of the current phi (cb_fft.hip); bins filled with the reference's
 // non-empty-shell convention
 cb_status step_n(cb_sim *s, int nsteps);
+cb_status construct_sim(cb_sim *s, const double *xns, int n_xns);
+void release_sim(cb_sim *s);
+bool params_ok(const cb_params *p, std::string &why);
 cb_status select_variant(cb_sim *s, int variant);
 cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int cap, int *nbins);
 // thermal initial condition of one field (cb_fft.hip)

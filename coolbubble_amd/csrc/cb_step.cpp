@@ -112,16 +112,21 @@ extern "C" cb_status cb_autotune(cb_sim *s, int steps, double *ms_per_step, int 
     for (int v = 0; v <= nvar && e == hipSuccess; ++v) {
         const int var = (v == nvar) ? -1 : v;
         select_variant(s, var);
-        e = restore();
-        if (e != hipSuccess) break;
-        if (step_n(s, 1) != CB_OK) { e = hipErrorUnknown; break; }
-        e = hipEventRecord(ev[0], s->st);
-        if (e == hipSuccess && step_n(s, steps) != CB_OK) e = hipErrorUnknown;
-        if (e == hipSuccess) e = hipEventRecord(ev[1], s->st);
-        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
-        float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
-        const double per = (double)ms / steps;
+        double per = 1e300;
+        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+            e = restore();
+            if (e != hipSuccess) break;
+            if (step_n(s, 1) != CB_OK) { e = hipErrorUnknown; break; }
+            e = hipEventRecord(ev[0], s->st);
+            // one call of `steps` steps: energies on its last step only, as in a
+            // run that prints every `steps` steps
+            if (e == hipSuccess && step_n(s, steps) != CB_OK) e = hipErrorUnknown;
+            if (e == hipSuccess) e = hipEventRecord(ev[1], s->st);
+            if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+            if ((double)ms / steps < per) per = (double)ms / steps;
+        }
         if (ms_per_step && v < cap) ms_per_step[v] = per;
         if (per < best_ms) { best_ms = per; best_v = var; }
     }

@@ -244,6 +244,74 @@ k_step_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restric
                                                          reinterpret_cast<T *>(smem_raw));
 }
 
+// Batched persistent step: block b takes work items lt(b), lt(b) + G, ...
+// (XCD-contiguous order), each one tile of one member (binary search on the
+// members' first work items).  Per-member D and lagging are uniform per
+// block.  The per-tile arithmetic and partials are those of k_step_tiled
+// with the same config, so a member steps bit-identically alone or batched.
+template <typename T, bool ENERGY>
+__global__ void __launch_bounds__(256)
+k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int step_parity)
+{
+    constexpr int VW = sizeof(T) == 4 ? 4 : 2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    const int G = gridDim.x, b = blockIdx.x;
+    const int lb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+    for (int w = lb; w < total; w += G) {
+        int lo = 0, hi = nmem - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (table[mid].tile_begin <= w) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchMember &m = table[lo];
+        const int lt = w - m.tile_begin;
+        const int pphi = (m.par_phi + step_parity) & 1;
+        const int ppi = m.lagged ? ((m.par_pi + step_parity) & 1) : m.par_pi;
+        const T *phi_in = (const T *)m.phi[pphi];
+        T *phi_out = (T *)m.phi[pphi ^ 1];
+        const T *pi_in = (const T *)m.pi[ppi];
+        T *pi_out = m.lagged ? (T *)m.pi[ppi ^ 1] : (T *)m.pi[ppi];
+        if (m.k.D == 3) {
+            if (m.lagged)
+                tile_step<T, 3, VW, 4, 256, true, ENERGY, 2>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
+                                                             m.partials, lds);
+            else
+                tile_step<T, 3, VW, 4, 256, false, ENERGY, 2>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
+                                                              m.partials, lds);
+        } else {
+            tile_step<T, 2, VW, 4, 256, false, ENERGY, 2>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
+                                                          m.partials, lds);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_batch_finalize(const BatchMember *__restrict__ table)
+{
+    const BatchMember &m = table[blockIdx.x];
+    finalize_member(m.k, m.partials, m.ntiles, m.sc, m.lagged ? 0 : 1);
+}
+
+hipError_t launch_batch_step(int fp32, int energy, const BatchMember *table, int nmem, int total_tiles, int grid,
+                             size_t shm, int step_parity, hipStream_t st)
+{
+    if (fp32) {
+        if (energy) k_batch_step<float, true><<<grid, 256, shm, st>>>(table, nmem, total_tiles, step_parity);
+        else k_batch_step<float, false><<<grid, 256, shm, st>>>(table, nmem, total_tiles, step_parity);
+    } else {
+        if (energy) k_batch_step<double, true><<<grid, 256, shm, st>>>(table, nmem, total_tiles, step_parity);
+        else k_batch_step<double, false><<<grid, 256, shm, st>>>(table, nmem, total_tiles, step_parity);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_finalize(const BatchMember *table, int nmem, hipStream_t st)
+{
+    if (nmem > 0) k_batch_finalize<<<nmem, 256, 0, st>>>(table);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -258,6 +326,14 @@ static TiledGeom geom(const SimConst &k, const TiledCfg &c)
     g.nt = c.nt;
     return g;
 }
+
+static size_t lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
+
+size_t tiled_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c) { return lds_bytes(k, fp32, c); }
+
+static TiledGeom geom(const SimConst &k, const TiledCfg &c);
+
+TiledGeom tiled_geom(const SimConst &k, const TiledCfg &c) { return geom(k, c); }
 
 static size_t lds_bytes(const SimConst &k, int fp32, const TiledCfg &c)
 {

@@ -45,6 +45,32 @@ hipError_t launch_step_tiled(const SimConst &k, int fp32, int lagged, int energy
                              const StepPtrs &p, const DevScalars *sc, double *partials, hipStream_t st,
                              int *nblocks);
 
+// One ensemble member as the batched step kernel sees it: constants, tile
+// geometry, both buffers of each field and the parity of the current ones at
+// the start of a launch sequence.
+struct BatchMember {
+    SimConst k;
+    TiledGeom g;
+    int lagged;
+    int tile_begin;     // first work item of this member in the batch
+    int ntiles;
+    int par_phi, par_pi;
+    void *phi[2];
+    void *pi[2];
+    DevScalars *sc;
+    double *partials;
+};
+
+// Batched persistent step: every tile of every member, one launch.  All
+// members share the precision; the tiled configs must have 256 threads, at
+// most 4 vectors per thread and prefetch depth 2.  step_parity flips the
+// buffer roles of every member relative to the table's parities.
+hipError_t launch_batch_step(int fp32, int energy, const BatchMember *table, int nmem, int total_tiles,
+                             int grid, size_t shm, int step_parity, hipStream_t st);
+hipError_t launch_batch_finalize(const BatchMember *table, int nmem, hipStream_t st);
+size_t tiled_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
+TiledGeom tiled_geom(const SimConst &k, const TiledCfg &c);
+
 // Registry of chosen configurations (the autotuner writes, the step reads).
 int tiled_register(const TiledCfg &c);
 TiledCfg tiled_get(int which);

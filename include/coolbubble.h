@@ -146,6 +146,25 @@ cb_status cb_autotune(cb_sim *s, int steps, double *ms_per_step, int cap, int *b
 /* The HIP stream (hipStream_t) every kernel of this handle runs on. */
 cb_status cb_stream(cb_sim *s, void **stream);
 
+/* ---- Ensembles (no reference counterpart: the reference runs one
+ * simulation per process) ----
+ * A batch steps n independent simulations together: one persistent launch
+ * per step covers every tile of every member (members may differ in D,
+ * nGrids, L, c, dt, mode, seeds; they share the precision and device), the
+ * energies of all members are finalised by one launch, and runs of 40 steps
+ * are captured once as a HIP graph and replayed.  A member steps
+ * bit-identically alone (cb_create with the same params, same kernel
+ * variant) or in the batch.  cb_batch_member gives a handle for reading a
+ * member (energies, fields, spectrum, state); it must not be destroyed on
+ * its own. */
+typedef struct cb_batch cb_batch;
+cb_status cb_batch_create(const cb_params *members, int n, cb_batch **out);
+cb_status cb_batch_step(cb_batch *b, int nsteps);
+cb_status cb_batch_member(cb_batch *b, int i, cb_sim **member);
+cb_status cb_batch_size(cb_batch *b, int *n, long long *sites);
+cb_status cb_batch_stream(cb_batch *b, void **stream);
+cb_status cb_batch_destroy(cb_batch *b);
+
 /* Measurement helper (no reference counterpart): achieved bandwidth of a
  * plain 16-byte-vector device-to-device copy of `bytes` bytes, repeated
  * `reps` times, counting read + write bytes, in GB/s. */
