@@ -44,6 +44,23 @@ class Params(ctypes.Structure):
     ]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("precision", ctypes.c_int),
+                ("mem_budget", ctypes.c_longlong), ("max_active", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("params", Params), ("nt", ctypes.c_int), ("snapshots", ctypes.c_int),
+                ("tag", ctypes.c_ulonglong)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_ulonglong), ("tag", ctypes.c_ulonglong), ("status", ctypes.c_int),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("site_updates", ctypes.c_longlong), ("n_rows", ctypes.c_int), ("n_spectra", ctypes.c_int),
+                ("final_row", ctypes.c_double * 16)]
+
+
 class Energies(ctypes.Structure):
     _fields_ = [("col", ctypes.c_double * 16)]
 
@@ -92,6 +109,24 @@ def lib():
         L.cb_batch_destroy.argtypes = [vp]
         for name in ("cb_batch_create", "cb_batch_step", "cb_batch_member", "cb_batch_size",
                      "cb_batch_stream", "cb_batch_destroy"):
+            getattr(L, name).restype = ctypes.c_int
+        L.cb_batch_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp]
+        L.cb_batch_autotune.restype = ctypes.c_int
+        ull = ctypes.c_ulonglong
+        L.cb_server_create.argtypes = [ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]
+        L.cb_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ull)]
+        L.cb_server_cancel.argtypes = [vp, ull]
+        L.cb_server_poll.argtypes = [vp, ctypes.POINTER(Result), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.cb_server_rows.argtypes = [vp, ull, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.cb_server_spectrum.argtypes = [vp, ull, ctypes.c_int, dp, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.cb_server_release.argtypes = [vp, ull]
+        L.cb_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]
+        L.cb_server_drain.argtypes = [vp]
+        L.cb_server_destroy.argtypes = [vp]
+        for name in ("cb_server_create", "cb_server_submit", "cb_server_cancel", "cb_server_poll",
+                     "cb_server_rows", "cb_server_spectrum", "cb_server_release", "cb_server_stats",
+                     "cb_server_drain", "cb_server_destroy"):
             getattr(L, name).restype = ctypes.c_int
         L.cb_nsites.argtypes = [vp]
         L.cb_nsites.restype = ctypes.c_longlong
@@ -272,6 +307,13 @@ class Batch:
     def step(self, n):
         check(lib().cb_batch_step(self._h, int(n)))
 
+    def autotune(self, steps=10):
+        """Pick the batched launch shape; returns ((items, pf, nt, NI), ms per step)."""
+        shape = (ctypes.c_int * 4)()
+        ms = ctypes.c_double()
+        check(lib().cb_batch_autotune(self._h, int(steps), shape, ctypes.byref(ms)))
+        return tuple(shape[:]), ms.value
+
     def stream(self):
         s = ctypes.c_void_p()
         check(lib().cb_batch_stream(self._h, ctypes.byref(s)))
@@ -289,3 +331,85 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Continuous-batching request server (cb_server_*)."""
+
+    def __init__(self, precision="fp32", mem_budget=0, max_active=0, device=0):
+        o = ServerOpts(device=device, precision=CB_FP32 if precision == "fp32" else CB_FP64,
+                       mem_budget=int(mem_budget), max_active=int(max_active))
+        h = ctypes.c_void_p()
+        check(lib().cb_server_create(ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, nt, snapshots=5, tag=0, params=None, **kw):
+        r = Request(params=params if params is not None else default_params(**kw), nt=int(nt),
+                    snapshots=int(snapshots), tag=int(tag))
+        rid = ctypes.c_ulonglong()
+        check(lib().cb_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
+        return rid.value
+
+    def cancel(self, rid):
+        check(lib().cb_server_cancel(self._h, int(rid)))
+
+    def poll(self, cap=256):
+        out = (Result * cap)()
+        n = ctypes.c_int()
+        check(lib().cb_server_poll(self._h, out, cap, ctypes.byref(n)))
+        return [out[i] for i in range(n.value)]
+
+    def rows(self, rid):
+        n = ctypes.c_int()
+        check(lib().cb_server_rows(self._h, int(rid), None, 0, ctypes.byref(n)))
+        out = np.zeros((max(n.value, 1), 16))
+        check(lib().cb_server_rows(self._h, int(rid), _dp(out), n.value, ctypes.byref(n)))
+        return out[:n.value]
+
+    def spectrum(self, rid, which):
+        n = ctypes.c_int()
+        check(lib().cb_server_spectrum(self._h, int(rid), int(which), None, None, 0, ctypes.byref(n)))
+        k, d = np.zeros(max(n.value, 1)), np.zeros(max(n.value, 1))
+        check(lib().cb_server_spectrum(self._h, int(rid), int(which), _dp(k), _dp(d), n.value,
+                                       ctypes.byref(n)))
+        return k[:n.value], d[:n.value]
+
+    def release(self, rid):
+        check(lib().cb_server_release(self._h, int(rid)))
+
+    def stats(self):
+        q, a, d, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+        check(lib().cb_server_stats(self._h, ctypes.byref(q), ctypes.byref(a), ctypes.byref(d),
+                                    ctypes.byref(b)))
+        return dict(queued=q.value, active=a.value, done=d.value, bytes=b.value)
+
+    def drain(self):
+        check(lib().cb_server_drain(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cb_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def server_variant(sim):
+    """The tiled variant the server gives a request (256 threads, <= 4
+    vectors per thread, prefetch 1, non-temporal stores; tallest tile, chunk
+    closest to 32 planes), or -1: selecting it on a lone simulation
+    reproduces the request's results bit for bit."""
+    n, _ = sim.kernel_variants()
+    best, score = -1, -1e300
+    for v in range(n):
+        c = sim.variant_info(v)
+        if c["nthr"] != 256 or c["items"] > 4 or c["pf"] != 1 or c["nt"] != 1:
+            continue
+        s = 1000.0 * c["TR"] - abs(np.log2(c["NI"] / 32.0))
+        if s > score:
+            best, score = v, s
+    return best

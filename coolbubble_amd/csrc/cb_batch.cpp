@@ -9,6 +9,7 @@
 // once into a HIP graph and replayed; kSegment is even, so every member's
 // ping-pong buffers are back in place after a replay and the captured
 // kernel arguments stay valid.
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -31,6 +32,62 @@ namespace {
 constexpr int kSegment = 40;   // OutEnergyEvery, src/def.h:65
 }
 
+namespace cb {
+
+// The tiled variant a member uses inside a batched launch of shape
+// (items_max, pf, nt): 256 threads, at most items_max vectors per thread;
+// among those the tallest tile (most rows per block), then the chunk length
+// closest to ni_pref.  -1 if the lattice has no tiled shape.
+int pick_variant(const cb_sim *s, int items_max, int pf, int nt, int ni_pref)
+{
+    int best = -1;
+    double best_score = -1e300;
+    for (size_t v = 0; v < s->variants.size(); ++v) {
+        const TiledCfg &c = s->variants[v];
+        if (c.nthr != 256 || c.items > items_max || c.pf != pf || c.nt != nt) continue;
+        const double score = 1000.0 * c.TR - std::fabs(std::log2((double)c.NI / ni_pref));
+        if (score > best_score) { best_score = score; best = (int)v; }
+    }
+    return best;
+}
+
+int batch_variant(const cb_sim *s) { return pick_variant(s, 4, 1, 1, 32); }
+
+// Table entry of a member stepping with its selected tiled variant.
+void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size_t *shm)
+{
+    const TiledCfg c = s->variants[(size_t)s->variant];
+    std::memset(&m, 0, sizeof m);
+    m.k = s->k;
+    m.g = tiled_geom(s->k, c);
+    m.lagged = s->lagged ? 1 : 0;
+    m.tile_begin = tile_begin;
+    m.ntiles = tiled_blocks(s->k, c);
+    m.phi[0] = s->phi[0];
+    m.phi[1] = s->phi[1];
+    m.pi[0] = s->pi[0];
+    m.pi[1] = s->lagged ? s->pi[1] : s->pi[0];
+    m.sc = s->dsc;
+    m.partials = s->partials;
+    m.par_phi = s->iphi;
+    m.par_pi = s->ipi;
+    const size_t b = tiled_lds_bytes(s->k, fp32, c);
+    if (shm && b > *shm) *shm = b;
+}
+
+// Host side of n steps run by batched launches: buffer parities, t, step.
+void advance_host(cb_sim *s, int n)
+{
+    if (n & 1) {
+        s->iphi ^= 1;
+        if (s->lagged) s->ipi ^= 1;
+    }
+    for (int it = 0; it < n; ++it) s->t += s->p.dt;
+    s->step += n;
+}
+
+} // namespace cb
+
 struct cb_batch {
     std::vector<cb_sim *> mem;
     std::vector<int> tiled;        // member indices on the batched kernel, table order
@@ -45,6 +102,10 @@ struct cb_batch {
     long long sites = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // buffer parities of the general-kernel members when the graph was
+    // captured (their pointers are baked into the graph)
+    std::vector<int> cap_par;
+    int items = 4, pf = 1, nt = 1, ni = 32;   // batched launch shape (cb_batch_autotune)
 };
 
 namespace {
@@ -88,8 +149,8 @@ cb_status enqueue_steps(cb_batch *b, int n)
     for (int it = 0; it < n; ++it) {
         const bool energy = b->any_lagged || it == n - 1;
         if (!b->tiled.empty()) {
-            HIP_TRY(cb::launch_batch_step(b->fp32, energy, b->d_tab, (int)b->tiled.size(), b->total_tiles, b->grid,
-                                          b->shm, it & 1, b->st));
+            HIP_TRY(cb::launch_batch_step(b->fp32, energy, b->items, b->pf, b->d_tab, (int)b->tiled.size(),
+                                          b->total_tiles, b->grid, b->shm, it & 1, b->st));
             if (energy) HIP_TRY(cb::launch_batch_finalize(b->d_tab, (int)b->tiled.size(), b->st));
         }
     }
@@ -97,16 +158,56 @@ cb_status enqueue_steps(cb_batch *b, int n)
         cb_status r = cb::step_n(b->mem[(size_t)i], n);
         if (r != CB_OK) return r;
     }
-    for (int i : b->tiled) {
-        cb_sim *s = b->mem[(size_t)i];
-        if (n & 1) {
-            s->iphi ^= 1;
-            if (s->lagged) s->ipi ^= 1;
-        }
-        for (int it = 0; it < n; ++it) s->t += s->p.dt;
-        s->step += n;
-    }
+    for (int i : b->tiled) cb::advance_host(b->mem[(size_t)i], n);
     return CB_OK;
+}
+
+void drop_graph(cb_batch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Pick every member's variant for the batch's launch shape and lay out the
+// work table (members without a tiled shape step on their general kernel).
+cb_status build_table(cb_batch *b)
+{
+    if (b->st) HIP_TRY(hipStreamSynchronize(b->st));
+    drop_graph(b);
+    b->tiled.clear();
+    b->general.clear();
+    for (size_t i = 0; i < b->mem.size(); ++i) {
+        cb_sim *s = b->mem[i];
+        const int pick = cb::pick_variant(s, b->items, b->pf, b->nt, b->ni);
+        cb::select_variant(s, pick);
+        (pick >= 0 ? b->tiled : b->general).push_back((int)i);
+    }
+    const size_t nt = b->tiled.size();
+    if (b->d_tab) { hipFree(b->d_tab); b->d_tab = nullptr; }
+    if (b->h_tab) { hipHostFree(b->h_tab); b->h_tab = nullptr; }
+    b->total_tiles = 0;
+    b->shm = 0;
+    b->grid = 0;
+    if (!nt) return CB_OK;
+    if (hipHostMalloc((void **)&b->h_tab, sizeof(cb::BatchMember) * nt, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void **)&b->d_tab, sizeof(cb::BatchMember) * nt) != hipSuccess) {
+        set_error("batch table allocation failed");
+        return CB_OUT_OF_MEMORY;
+    }
+    int tiles = 0;
+    for (size_t t = 0; t < nt; ++t) {
+        cb::fill_member(b->h_tab[t], b->mem[(size_t)b->tiled[t]], tiles, b->fp32, &b->shm);
+        tiles += b->h_tab[t].ntiles;
+    }
+    b->total_tiles = tiles;
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int cap = cus * 8;
+    b->grid = tiles < cap ? tiles : cap;
+    return upload_table(b);
 }
 
 cb_status capture_segment(cb_batch *b)
@@ -133,6 +234,8 @@ cb_status capture_segment(cb_batch *b)
     if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return CB_HIP_ERROR; }
     b->graph = g;
     HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->cap_par.clear();
+    for (int i : b->general) b->cap_par.push_back(b->mem[(size_t)i]->iphi * 2 + b->mem[(size_t)i]->ipi);
     return CB_OK;
 }
 
@@ -175,54 +278,9 @@ cb_status cb_batch_create(const cb_params *members, int n, cb_batch **out)
         s->batch_owned = true;
         b->sites += s->nsites;
         if (s->lagged) b->any_lagged = true;
-        // the batched kernel's shape: 256 threads, <= 4 vectors per thread, prefetch 2
-        int pick = -1;
-        for (size_t v = 0; v < s->variants.size(); ++v) {
-            const cb::TiledCfg &c = s->variants[v];
-            if (c.nthr == 256 && c.items <= 4 && c.pf == 2 && c.nt == 0) { pick = (int)v; break; }
-        }
-        cb::select_variant(s, pick);
-        if (pick >= 0) b->tiled.push_back(i);
-        else b->general.push_back(i);
     }
-    const size_t nt = b->tiled.size();
-    if (nt) {
-        if (hipHostMalloc((void **)&b->h_tab, sizeof(cb::BatchMember) * nt, hipHostMallocDefault) != hipSuccess ||
-            hipMalloc((void **)&b->d_tab, sizeof(cb::BatchMember) * nt) != hipSuccess) {
-            batch_free(b);
-            set_error("batch table allocation failed");
-            return CB_OUT_OF_MEMORY;
-        }
-        int tiles = 0;
-        for (size_t t = 0; t < nt; ++t) {
-            cb_sim *s = b->mem[(size_t)b->tiled[t]];
-            const cb::TiledCfg c = s->variants[(size_t)s->variant];
-            cb::BatchMember &m = b->h_tab[t];
-            std::memset(&m, 0, sizeof m);
-            m.k = s->k;
-            m.g = cb::tiled_geom(s->k, c);
-            m.lagged = s->lagged ? 1 : 0;
-            m.tile_begin = tiles;
-            m.ntiles = cb::tiled_blocks(s->k, c);
-            m.phi[0] = s->phi[0];
-            m.phi[1] = s->phi[1];
-            m.pi[0] = s->pi[0];
-            m.pi[1] = s->lagged ? s->pi[1] : s->pi[0];
-            m.sc = s->dsc;
-            m.partials = s->partials;
-            tiles += m.ntiles;
-            const size_t shm = cb::tiled_lds_bytes(s->k, b->fp32, c);
-            if (shm > b->shm) b->shm = shm;
-        }
-        b->total_tiles = tiles;
-        int dev = 0, cus = 256;
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        const int cap = cus * 8;
-        b->grid = tiles < cap ? tiles : cap;
-        cb_status r = upload_table(b);
-        if (r != CB_OK) { batch_free(b); return r; }
-    }
+    cb_status r = build_table(b);
+    if (r != CB_OK) { batch_free(b); return r; }
     if (hipStreamSynchronize(b->st) != hipSuccess) { batch_free(b); set_error("batch setup failed"); return CB_HIP_ERROR; }
     *out = b;
     return CB_OK;
@@ -234,6 +292,17 @@ cb_status cb_batch_step(cb_batch *b, int nsteps)
     cb_status r = upload_table(b);
     if (r != CB_OK) return r;
     int left = nsteps;
+    if (b->exec) {   // general members moved to other buffers since the capture: capture again
+        bool stale = false;
+        for (size_t g = 0; g < b->general.size(); ++g) {
+            const cb_sim *s = b->mem[(size_t)b->general[g]];
+            if (s->iphi * 2 + s->ipi != b->cap_par[g]) stale = true;
+        }
+        if (stale) {
+            HIP_TRY(hipStreamSynchronize(b->st));
+            drop_graph(b);
+        }
+    }
     if (left >= kSegment && !b->exec) {
         r = capture_segment(b);
         if (r != CB_OK) return r;
@@ -269,6 +338,88 @@ cb_status cb_batch_stream(cb_batch *b, void **stream)
 {
     if (!b || !stream) { set_error("invalid argument"); return CB_INVALID_ARG; }
     *stream = (void *)b->st;
+    return CB_OK;
+}
+
+cb_status cb_batch_autotune(cb_batch *b, int steps, int shape[4], double *best_ms)
+{
+    if (!b || steps < 1) { set_error("invalid argument"); return CB_INVALID_ARG; }
+    HIP_TRY(hipStreamSynchronize(b->st));
+    // save every member's current fields, scalars and host state
+    struct Saved { void *phi, *pi; cb::DevScalars sc; int iphi, ipi; double t; long long step; };
+    std::vector<Saved> sv(b->mem.size());
+    cb_status r = CB_OK;
+    for (size_t i = 0; i < b->mem.size() && r == CB_OK; ++i) {
+        cb_sim *s = b->mem[i];
+        const size_t bytes = (size_t)s->nsites * s->esz;
+        Saved &q = sv[i];
+        q = Saved{nullptr, nullptr, {}, s->iphi, s->ipi, s->t, s->step};
+        if (hipMalloc(&q.phi, bytes) != hipSuccess || hipMalloc(&q.pi, bytes) != hipSuccess) { r = CB_OUT_OF_MEMORY; break; }
+        hipMemcpyAsync(q.phi, s->phi[s->iphi], bytes, hipMemcpyDeviceToDevice, b->st);
+        hipMemcpyAsync(q.pi, s->pi[s->ipi], bytes, hipMemcpyDeviceToDevice, b->st);
+        hipMemcpyAsync(&q.sc, s->dsc, sizeof q.sc, hipMemcpyDeviceToHost, b->st);
+    }
+    auto restore = [&]() -> hipError_t {
+        for (size_t i = 0; i < b->mem.size(); ++i) {
+            cb_sim *s = b->mem[i];
+            Saved &q = sv[i];
+            const size_t bytes = (size_t)s->nsites * s->esz;
+            s->iphi = q.iphi;
+            s->ipi = q.ipi;
+            s->t = q.t;
+            s->step = q.step;
+            hipMemcpyAsync(s->phi[s->iphi], q.phi, bytes, hipMemcpyDeviceToDevice, b->st);
+            hipMemcpyAsync(s->pi[s->ipi], q.pi, bytes, hipMemcpyDeviceToDevice, b->st);
+            hipMemcpyAsync(s->dsc, &q.sc, sizeof q.sc, hipMemcpyHostToDevice, b->st);
+        }
+        return hipStreamSynchronize(b->st);
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (r == CB_OK && (hipStreamSynchronize(b->st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                       hipEventCreate(&e1) != hipSuccess))
+        r = CB_HIP_ERROR;
+    const int shapes[][4] = {{4, 1, 1, 32}, {4, 2, 1, 32}, {4, 1, 0, 32}, {2, 1, 1, 32}, {1, 1, 1, 32},
+                             {4, 1, 1, 16}, {4, 1, 1, 64}, {1, 2, 1, 32}, {2, 2, 1, 16}, {4, 2, 0, 16}};
+    double best = 1e300;
+    int best_i = 0;
+    for (int c = 0; c < (int)(sizeof shapes / sizeof shapes[0]) && r == CB_OK; ++c) {
+        b->items = shapes[c][0];
+        b->pf = shapes[c][1];
+        b->nt = shapes[c][2];
+        b->ni = shapes[c][3];
+        r = build_table(b);
+        if (r != CB_OK) break;
+        double per = 1e300;
+        for (int rep = 0; rep < 2 && r == CB_OK; ++rep) {
+            if (restore() != hipSuccess) { r = CB_HIP_ERROR; break; }
+            r = upload_table(b);
+            if (r != CB_OK) break;
+            hipEventRecord(e0, b->st);
+            r = enqueue_steps(b, steps);
+            hipEventRecord(e1, b->st);
+            float ms = 0.f;
+            if (r == CB_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                r = CB_HIP_ERROR;
+            if (ms / steps < per) per = ms / steps;
+        }
+        if (per < best) { best = per; best_i = c; }
+    }
+    if (restore() != hipSuccess && r == CB_OK) r = CB_HIP_ERROR;
+    for (Saved &q : sv) {
+        if (q.phi) hipFree(q.phi);
+        if (q.pi) hipFree(q.pi);
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    b->items = shapes[best_i][0];
+    b->pf = shapes[best_i][1];
+    b->nt = shapes[best_i][2];
+    b->ni = shapes[best_i][3];
+    cb_status r2 = build_table(b);
+    if (r == CB_OK) r = r2;
+    if (r != CB_OK) return r;
+    if (shape) for (int i = 0; i < 4; ++i) shape[i] = shapes[best_i][i];
+    if (best_ms) *best_ms = best;
     return CB_OK;
 }
 

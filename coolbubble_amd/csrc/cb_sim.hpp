@@ -54,6 +54,10 @@ cb_status step_n(cb_sim *s, int nsteps);
 cb_status construct_sim(cb_sim *s, const double *xns, int n_xns);
 void release_sim(cb_sim *s);
 bool params_ok(const cb_params *p, std::string &why);
+int batch_variant(const cb_sim *s);
+int pick_variant(const cb_sim *s, int items_max, int pf, int nt, int ni_pref);
+void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size_t *shm);
+void advance_host(cb_sim *s, int n);
 cb_status select_variant(cb_sim *s, int variant);
 cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int cap, int *nbins);
 // thermal initial condition of one field (cb_fft.hip)

@@ -249,7 +249,7 @@ k_step_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restric
 // members' first work items).  Per-member D and lagging are uniform per
 // block.  The per-tile arithmetic and partials are those of k_step_tiled
 // with the same config, so a member steps bit-identically alone or batched.
-template <typename T, bool ENERGY>
+template <typename T, bool ENERGY, int ITEMS, int PF>
 __global__ void __launch_bounds__(256)
 k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int step_parity)
 {
@@ -275,13 +275,13 @@ k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int ste
         T *pi_out = m.lagged ? (T *)m.pi[ppi ^ 1] : (T *)m.pi[ppi];
         if (m.k.D == 3) {
             if (m.lagged)
-                tile_step<T, 3, VW, 4, 256, true, ENERGY, 2>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
+                tile_step<T, 3, VW, ITEMS, 256, true, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
                                                              m.partials, lds);
             else
-                tile_step<T, 3, VW, 4, 256, false, ENERGY, 2>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
+                tile_step<T, 3, VW, ITEMS, 256, false, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
                                                               m.partials, lds);
         } else {
-            tile_step<T, 2, VW, 4, 256, false, ENERGY, 2>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
+            tile_step<T, 2, VW, ITEMS, 256, false, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
                                                           m.partials, lds);
         }
     }
@@ -293,15 +293,32 @@ __global__ void __launch_bounds__(256) k_batch_finalize(const BatchMember *__res
     finalize_member(m.k, m.partials, m.ntiles, m.sc, m.lagged ? 0 : 1);
 }
 
-hipError_t launch_batch_step(int fp32, int energy, const BatchMember *table, int nmem, int total_tiles, int grid,
-                             size_t shm, int step_parity, hipStream_t st)
+template <typename T, bool ENERGY>
+static void batch_launch_ip(int items, int pf, const BatchMember *table, int nmem, int total, int grid, size_t shm,
+                            int parity, hipStream_t st)
+{
+#define CB_BATCH_L(I, P) k_batch_step<T, ENERGY, I, P><<<grid, 256, shm, st>>>(table, nmem, total, parity)
+    if (pf == 2) {
+        if (items == 1) CB_BATCH_L(1, 2);
+        else if (items == 2) CB_BATCH_L(2, 2);
+        else CB_BATCH_L(4, 2);
+    } else {
+        if (items == 1) CB_BATCH_L(1, 1);
+        else if (items == 2) CB_BATCH_L(2, 1);
+        else CB_BATCH_L(4, 1);
+    }
+#undef CB_BATCH_L
+}
+
+hipError_t launch_batch_step(int fp32, int energy, int items, int pf, const BatchMember *table, int nmem,
+                             int total_tiles, int grid, size_t shm, int step_parity, hipStream_t st)
 {
     if (fp32) {
-        if (energy) k_batch_step<float, true><<<grid, 256, shm, st>>>(table, nmem, total_tiles, step_parity);
-        else k_batch_step<float, false><<<grid, 256, shm, st>>>(table, nmem, total_tiles, step_parity);
+        if (energy) batch_launch_ip<float, true>(items, pf, table, nmem, total_tiles, grid, shm, step_parity, st);
+        else batch_launch_ip<float, false>(items, pf, table, nmem, total_tiles, grid, shm, step_parity, st);
     } else {
-        if (energy) k_batch_step<double, true><<<grid, 256, shm, st>>>(table, nmem, total_tiles, step_parity);
-        else k_batch_step<double, false><<<grid, 256, shm, st>>>(table, nmem, total_tiles, step_parity);
+        if (energy) batch_launch_ip<double, true>(items, pf, table, nmem, total_tiles, grid, shm, step_parity, st);
+        else batch_launch_ip<double, false>(items, pf, table, nmem, total_tiles, grid, shm, step_parity, st);
     }
     return hipGetLastError();
 }

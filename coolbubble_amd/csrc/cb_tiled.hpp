@@ -62,11 +62,14 @@ struct BatchMember {
 };
 
 // Batched persistent step: every tile of every member, one launch.  All
-// members share the precision; the tiled configs must have 256 threads, at
-// most 4 vectors per thread and prefetch depth 2.  step_parity flips the
-// buffer roles of every member relative to the table's parities.
-hipError_t launch_batch_step(int fp32, int energy, const BatchMember *table, int nmem, int total_tiles,
-                             int grid, size_t shm, int step_parity, hipStream_t st);
+// members share the precision; their tiled configs must have 256 threads and
+// at most `items` vectors per thread; `pf` is the launch's prefetch depth.
+// step_parity flips the buffer roles of every member relative to the table's
+// parities.  A member's results depend only on its (TR, NI, threads), not on
+// items, pf or the non-temporal flag, so it steps bit-identically alone with
+// a variant of the same (TR, NI, threads).
+hipError_t launch_batch_step(int fp32, int energy, int items, int pf, const BatchMember *table, int nmem,
+                             int total_tiles, int grid, size_t shm, int step_parity, hipStream_t st);
 hipError_t launch_batch_finalize(const BatchMember *table, int nmem, hipStream_t st);
 size_t tiled_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
 TiledGeom tiled_geom(const SimConst &k, const TiledCfg &c);

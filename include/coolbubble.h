@@ -163,7 +163,62 @@ cb_status cb_batch_step(cb_batch *b, int nsteps);
 cb_status cb_batch_member(cb_batch *b, int i, cb_sim **member);
 cb_status cb_batch_size(cb_batch *b, int *n, long long *sites);
 cb_status cb_batch_stream(cb_batch *b, void **stream);
+/* Autotune the batched launch: time `steps` steps of the whole batch for a
+ * set of launch shapes (vectors per thread, prefetch depth, non-temporal
+ * stores, preferred chunk length; each member then takes its tallest tile
+ * of that shape), state saved and restored around the trials; keep the
+ * fastest.  shape (optional) receives it, best_ms its time per step. */
+cb_status cb_batch_autotune(cb_batch *b, int steps, int shape[4], double *best_ms);
 cb_status cb_batch_destroy(cb_batch *b);
+
+/* ---- Serving (no reference counterpart) ----
+ * A request is one run: parameters, nt steps and a number of snapshots
+ * (spaced as SnapshotSchedule, src/main.cpp:201-223, each with a power
+ * spectrum).  The server admits queued requests in arrival order while the
+ * device-memory budget and the slot count allow, keeps their lattices
+ * resident, and advances every active request together: one batched launch
+ * per step over all of them, in ticks that end at the nearest event of any
+ * request (an energy row every 40 steps, a snapshot, completion).  Requests
+ * arriving while others run are admitted at the next tick.  Each request's
+ * results are bit-identical to running it alone with the same kernel
+ * variant.  Times are seconds on the server's steady clock. */
+typedef struct cb_server cb_server;
+typedef struct cb_server_opts {
+    int device;
+    int precision;               /* all requests run in this precision */
+    long long mem_budget;        /* bytes of device memory for lattices (0: 90% of free) */
+    int max_active;              /* slot count (0: unlimited) */
+} cb_server_opts;
+typedef struct cb_request {
+    cb_params params;            /* precision and device are taken from the server */
+    int nt;                      /* steps */
+    int snapshots;               /* snapshots after the initial state, each with a spectrum */
+    unsigned long long tag;      /* caller's tag, returned in the result */
+} cb_request;
+typedef struct cb_result {
+    unsigned long long id, tag;
+    cb_status status;            /* CB_OK, CB_ADMISSION_REJECTED (can never fit), CB_NOT_READY (cancelled) */
+    double t_submit, t_admit, t_done;
+    long long site_updates;
+    int n_rows;                  /* energy rows recorded (t = 0 and every 40 steps) */
+    int n_spectra;               /* spectra recorded (initial + snapshots) */
+    double final_row[16];        /* energy row after the last step */
+} cb_result;
+cb_status cb_server_create(const cb_server_opts *o, cb_server **out);
+cb_status cb_server_submit(cb_server *s, const cb_request *r, unsigned long long *id);
+cb_status cb_server_cancel(cb_server *s, unsigned long long id);
+/* completed requests since the last poll: up to cap results, *n written */
+cb_status cb_server_poll(cb_server *s, cb_result *out, int cap, int *n);
+/* stored outputs of a completed (polled) request */
+cb_status cb_server_rows(cb_server *s, unsigned long long id, double *rows, int cap_rows, int *n);
+cb_status cb_server_spectrum(cb_server *s, unsigned long long id, int which, double *kmean, double *delta,
+                             int cap, int *nbins);
+cb_status cb_server_release(cb_server *s, unsigned long long id);   /* drop stored outputs */
+/* queued, active, completed-but-unpolled requests; device bytes in use */
+cb_status cb_server_stats(cb_server *s, int *queued, int *active, int *done, long long *bytes);
+/* block until no request is queued or active */
+cb_status cb_server_drain(cb_server *s);
+cb_status cb_server_destroy(cb_server *s);
 
 /* Measurement helper (no reference counterpart): achieved bandwidth of a
  * plain 16-byte-vector device-to-device copy of `bytes` bytes, repeated
