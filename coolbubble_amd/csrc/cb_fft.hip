@@ -409,6 +409,56 @@ __global__ void __launch_bounds__(256) k_hermitian(cpx<double> *H, long long row
 }
 
 // ---------------------------------------------------------------------------
+// Long 1D transforms (N beyond one in-LDS transform): four-step.  The real
+// row x[N2 n1 + n2] is viewed as A[n1][n2]; transform along n1 (strided
+// columns), multiply by W_N^(k1 n2), transform along n2 (rows); then
+// X[k1 + N1 k2] = C[k1][k2].
+// ---------------------------------------------------------------------------
+template <typename T, typename R>
+__global__ void __launch_bounds__(256) k_real_to_cpx(const T *__restrict__ f, cpx<R> *__restrict__ H, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) H[i] = {(R)f[i], (R)0};
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_twiddle4(cpx<R> *H, int N1, int N2, const cpx<R> *__restrict__ twN, long long N)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)N1 * N2) return;
+    const long long k1 = i / N2, n2 = i - k1 * N2;
+    const long long e = (k1 * n2) % N;
+    H[i] = cmul(H[i], twN[e]);
+}
+
+// 1D bins: shell j holds the single mode j (n = sqrt(j^2) = j), so each
+// bin is written directly (ComputePowerSpectrum, src/bubble.cpp:680-701).
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_bin1d(const cpx<R> *__restrict__ C, int N1, int N2, int N, int nBins, double dk, double pref, double Pnorm,
+        double *__restrict__ bins)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nBins) return;
+    double pw = 0.0, ws = 0.0, ks = 0.0;
+    if (j <= N / 2) {
+        const int k1 = j % N1, k2 = j / N1;
+        const cpx<R> f = C[(long long)k1 * N2 + k2];
+        const double n = sqrt(0. + (double)j * j);
+        const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
+        const double kc = n * dk;
+        const double fx = (double)f.x, fy = (double)f.y;
+        pw = w * pref * ipow_rn(kc, 1) * Pnorm * (fx * fx + fy * fy);
+        ws = w;
+        ks = w * kc;
+    }
+    bins[j] = pw;
+    bins[nBins + j] = ws;
+    bins[2 * nBins + j] = ks;
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 struct FftPlan {
@@ -420,6 +470,10 @@ struct FftPlan {
     long long rows = 0;
     int nBins = 0, binblocks = 0;
     double *binpart = nullptr, *bins = nullptr;
+    // long 1D rows (four-step): N = N1 N2
+    int N1 = 0, N2 = 0;
+    Stages st1{}, st2{};
+    void *tw1 = nullptr, *tw2 = nullptr;
 };
 
 static bool factorise(int n, Stages &st)
@@ -486,10 +540,29 @@ static hipError_t forward_r2c(const Stages &st, const cpx<R> *tw, const T *field
     return hipGetLastError();
 }
 
+// Four-step spectrum of one long row into p->bins.
+template <typename R>
+static hipError_t long_row(FftPlan *p, const R *field, double pref, double dk, double Pnorm, hipStream_t st)
+{
+    const int N = p->N, N1 = p->N1, N2 = p->N2;
+    cpx<R> *H = (cpx<R> *)p->H;
+    k_real_to_cpx<R, R><<<(N + 255) / 256, 256, 0, st>>>(field, H, N);
+    const int CB = batch_for(N1, sizeof(cpx<R>));
+    size_t shm = 2 * (size_t)CB * N1 * sizeof(cpx<R>);
+    k_cols<R><<<(unsigned)((N2 + CB - 1) / CB), 256, shm, st>>>(H, N2, CB, p->st1, (const cpx<R> *)p->tw1, -1);
+    k_twiddle4<R><<<(N + 255) / 256, 256, 0, st>>>(H, N1, N2, (const cpx<R> *)p->tw, N);
+    shm = 2 * (size_t)N2 * sizeof(cpx<R>);
+    k_cols<R><<<(unsigned)N1, 256, shm, st>>>(H, 1, 1, p->st2, (const cpx<R> *)p->tw2, -1);
+    k_bin1d<R><<<(p->nBins + 255) / 256, 256, 0, st>>>(H, N1, N2, N, p->nBins, dk, pref, Pnorm, p->bins);
+    return hipGetLastError();
+}
+
 void fft_destroy(FftPlan *p)
 {
     if (!p) return;
     if (p->tw) hipFree(p->tw);
+    if (p->tw1) hipFree(p->tw1);
+    if (p->tw2) hipFree(p->tw2);
     if (p->H) hipFree(p->H);
     if (p->binpart) hipFree(p->binpart);
     if (p->bins) hipFree(p->bins);
@@ -513,7 +586,26 @@ static cb_status plan_for(cb_sim *s)
     p->N = N;
     p->D = D;
     p->fp32 = s->fp32;
-    if (!factorise(N, p->st) || batch_for(N, s->fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>)) < 1) {
+    const size_t csz0 = s->fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+    bool ok = factorise(N, p->st);
+    if (ok && batch_for(N, csz0) < 1) {
+        ok = false;
+        if (D == 1) {   // four-step: N1 N2 = N, both within one in-LDS transform
+            int best = 0;
+            for (int n2 = 2; n2 <= N / 2; ++n2) {
+                if (N % n2) continue;
+                const int n1 = N / n2;
+                if (batch_for(n1, csz0) < 1 || batch_for(n2, csz0) < 1) continue;
+                if (!best || std::abs(n2 - n1) < std::abs(best - N / best)) best = n2;
+            }
+            if (best) {
+                p->N2 = best;
+                p->N1 = N / best;
+                ok = factorise(p->N1, p->st1) && factorise(p->N2, p->st2);
+            }
+        }
+    }
+    if (!ok) {
         delete p;
         set_error("FFT length not supported by the in-LDS transform");
         return CB_UNSUPPORTED;
@@ -522,7 +614,12 @@ static cb_status plan_for(cb_sim *s)
     const int nc = N / 2 + 1;
     const size_t csz = s->fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
     hipError_t e = s->fp32 ? make_twiddles<float>(N, &p->tw, s->st) : make_twiddles<double>(N, &p->tw, s->st);
-    if (e == hipSuccess) e = hipMalloc(&p->H, csz * (size_t)p->rows * nc);
+    if (e == hipSuccess && p->N1) {
+        e = s->fp32 ? make_twiddles<float>(p->N1, &p->tw1, s->st) : make_twiddles<double>(p->N1, &p->tw1, s->st);
+        if (e == hipSuccess)
+            e = s->fp32 ? make_twiddles<float>(p->N2, &p->tw2, s->st) : make_twiddles<double>(p->N2, &p->tw2, s->st);
+    }
+    if (e == hipSuccess) e = hipMalloc(&p->H, csz * (p->N1 ? (size_t)N : (size_t)p->rows * nc));
     p->nBins = s->nkmax + 1;
     long long modes = p->rows * nc;
     p->binblocks = (int)((modes + 4095) / 4096);
@@ -547,21 +644,26 @@ cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int c
     const int D = s->p.D, N = s->p.nGrids, nB = p->nBins;
     const double L = s->p.L;
     const void *field = s->phi[s->iphi];
-    hipError_t e;
-    if (s->fp32) e = forward_r2c<float, float>(p->st, (const cpx<float> *)p->tw, (const float *)field, (cpx<float> *)p->H, D, s->st);
-    else e = forward_r2c<double, double>(p->st, (const cpx<double> *)p->tw, (const double *)field, (cpx<double> *)p->H, D, s->st);
-    HIP_TRY(e);
     const double PI = 3.14159265358979323846;
     const double dk = 2.0 * PI / L;
     const double pref = (D == 1) ? 1.0 / PI : (D == 2) ? 1.0 / (2.0 * PI) : 1.0 / (2.0 * PI * PI);
     const double Pnorm = s->dxD * s->dxD / std::pow(L, D);
-    const size_t shm = sizeof(double) * 3 * (size_t)nB;
-    if (shm > kLdsBudget) { set_error("too many spectrum bins for LDS"); return CB_UNSUPPORTED; }
-    if (s->fp32) k_bin<float><<<p->binblocks, 256, shm, s->st>>>((const cpx<float> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm, p->binpart);
-    else k_bin<double><<<p->binblocks, 256, shm, s->st>>>((const cpx<double> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm, p->binpart);
-    HIP_TRY(hipGetLastError());
-    k_bin_reduce<<<(3 * nB + 63) / 64, 256, 0, s->st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
-    HIP_TRY(hipGetLastError());
+    if (p->N1) {
+        HIP_TRY(s->fp32 ? long_row<float>(p, (const float *)field, pref, dk, Pnorm, s->st)
+                        : long_row<double>(p, (const double *)field, pref, dk, Pnorm, s->st));
+    } else {
+        hipError_t e;
+        if (s->fp32) e = forward_r2c<float, float>(p->st, (const cpx<float> *)p->tw, (const float *)field, (cpx<float> *)p->H, D, s->st);
+        else e = forward_r2c<double, double>(p->st, (const cpx<double> *)p->tw, (const double *)field, (cpx<double> *)p->H, D, s->st);
+        HIP_TRY(e);
+        const size_t shm = sizeof(double) * 3 * (size_t)nB;
+        if (shm > kLdsBudget) { set_error("too many spectrum bins for LDS"); return CB_UNSUPPORTED; }
+        if (s->fp32) k_bin<float><<<p->binblocks, 256, shm, s->st>>>((const cpx<float> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm, p->binpart);
+        else k_bin<double><<<p->binblocks, 256, shm, s->st>>>((const cpx<double> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm, p->binpart);
+        HIP_TRY(hipGetLastError());
+        k_bin_reduce<<<(3 * nB + 63) / 64, 256, 0, s->st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
+        HIP_TRY(hipGetLastError());
+    }
     std::vector<double> h(3 * (size_t)nB);
     HIP_TRY(hipMemcpyAsync(h.data(), p->bins, sizeof(double) * 3 * nB, hipMemcpyDeviceToHost, s->st));
     cb::DevScalars sc;

@@ -190,3 +190,22 @@ def test_set_state_roundtrip_and_stepping_from_it():
     phi_g, pi_g = g.fields()
     assert field_err(o.phi(), phi_g) <= TOL
     assert row_errors(o.energy_row()[None], g.energy_row()[None]).max() <= TOL
+
+
+@pytest.mark.parametrize("N", [8192, 65536])
+def test_long_1d_spectrum_matches_oracle(N):
+    """1D rows longer than one in-LDS transform go through the four-step
+    path (the serving workload's d=1 requests have N = 65536)."""
+    kw = params(D=1, nGrids=N, L=N * 0.0625, dt=0.0125)
+    o, g, _, _ = run_pair(kw, [40])
+    ko, _, do = o.spectrum()
+    kg, _, dg = g.spectrum()
+    assert len(kg) == len(ko) == N // 2 + 1
+    np.testing.assert_allclose(kg, ko, rtol=1e-13)
+    floor = 1e-13 * np.max(do)
+    assert np.all(np.abs(dg - do) <= TOL * np.abs(do) + floor)
+    g32 = cb.Bubble(precision="fp32", **kw)
+    g32.step(40)
+    k32, _, d32 = g32.spectrum()
+    big = do > 1e-6 * np.max(do)
+    np.testing.assert_allclose(d32[big], do[big], rtol=1e-3)
