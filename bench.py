@@ -132,6 +132,8 @@ def run_ensemble(cb, hip, nt, log):
     batch = cb.Batch(ensemble_members())
     log("ensemble: %d members, %d sites, created in %.1f s" % (len(batch.members), batch.sites,
                                                                 time.time() - t0))
+    shape, tms = batch.autotune(steps=10)
+    log("ensemble: autotuned launch shape (items, prefetch, nt, NI) = %s, %.3f ms/step" % (shape, tms))
     batch.step(40)   # graph capture + warm
     stream = batch.stream()
     a, b = hip.event(), hip.event()
@@ -149,6 +151,54 @@ def run_ensemble(cb, hip, nt, log):
             "config": "64 x d=3 N=128 L=16 thermal (c in {1,2,4,8} x 16 seeds, Trad=0.1) + "
                       "16 x d=2 N=1024 L=64 five bubbles (c ~ U[1,8]); FP32, static, "
                       "energies every 40 steps, 40-step HIP graph segments"}
+
+
+def serving_workload(n, rate, seed=7):
+    """Poisson arrivals at `rate` per second; d in {1,2,3}; N = 2^16 (d=1),
+    N in [256, 2048] (d=2, multiples of 32), N in [64, 384] (d=3, multiples
+    of 16); c ~ U[1,8]; mode in {-2,-1,1,-3}; nt ~ U[500,5000]; 5 snapshots;
+    dx = 0.0625 (d=1,2) or 0.125 (d=3), L = N dx, dt = 0.2 dx."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for _ in range(n):
+        t += rng.exponential(1.0 / rate)
+        d = int(rng.integers(1, 4))
+        if d == 1:
+            N, dx = 65536, 0.0625
+        elif d == 2:
+            N, dx = 32 * int(rng.integers(8, 65)), 0.0625
+        else:
+            N, dx = 16 * int(rng.integers(4, 25)), 0.125
+        p = dict(D=d, nGrids=N, L=N * dx, dt=0.2 * dx, c=float(rng.uniform(1.0, 8.0)),
+                 readxns=int(rng.choice([-2, -1, 1, -3])))
+        out.append((t, p, int(rng.integers(500, 5001))))
+    return out
+
+
+def run_serving(cb, n, rate, log):
+    work = serving_workload(n, rate)
+    sv = cb.Server(precision="fp32")
+    t0 = time.time()
+    ids = {}
+    for i, (ta, p, nt) in enumerate(work):
+        delay = ta - (time.time() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ids[sv.submit(nt, 5, tag=i, **p)] = i
+    sv.drain()
+    res = sv.poll(cap=n + 8)
+    sv.close()
+    ok = [r for r in res if r.status == 0]
+    lat = np.array([r.t_done - r.t_submit for r in ok])
+    span = max(r.t_done for r in ok) - min(r.t_submit for r in ok)
+    su = sum(r.site_updates for r in ok)
+    log("serving: %d requests (%d ok) at %.1f/s: p50 %.3f s, p99 %.3f s, %.3e site-updates/s over %.1f s"
+        % (n, len(ok), rate, np.percentile(lat, 50), np.percentile(lat, 99), su / span, span))
+    return {"requests": n, "completed": len(ok), "arrival_rate_per_s": rate,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "site_updates_per_s": su / span, "makespan_s": span, "precision": "fp32",
+            "config": "Poisson arrivals; d in {1,2,3} (N=2^16 | [256,2048] | [64,384]); c ~ U[1,8]; "
+                      "mode in {-2,-1,1,-3}; nt ~ U[500,5000]; 5 snapshots with spectra"}
 
 
 def cpu_baseline(sim64, nsteps, log):
@@ -190,6 +240,9 @@ def main():
     ap.add_argument("--no-spectrum", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice time-stepper runs on a single MI355X "
@@ -246,6 +299,7 @@ def main():
                          "from the GPU-seeded state" % args.cpu_steps}
     sim64.close()
     ens = None if args.no_ensemble else run_ensemble(cb, hip, args.ensemble_steps, log)
+    srv = None if args.no_serving else run_serving(cb, args.serve_requests, args.serve_rate, log)
     out = {
         "metric": "site-updates/s (FP32 stepper, d=3 N=512 static, mode 1 32 bubbles)",
         "value": su32,
@@ -269,6 +323,7 @@ def main():
                  "frac": ach64 / HBM_PEAK},
         "fp32_over_fp64": su32 / su64,
         "ensemble": ens,
+        "serving": srv,
         "measured_copy_GBps": copy_gbps,
         "frac_of_measured_copy": ach32 / (copy_gbps * 1e9),
         "fp32_energy_rel_err_vs_fp64": err,
