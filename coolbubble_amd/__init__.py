@@ -262,9 +262,9 @@ class Bubble:
 
     def variant_info(self, variant):
         """dict(vw, items, nthr, TR, NI, pf, nt) of a tiled variant."""
-        info = (ctypes.c_int * 7)()
+        info = (ctypes.c_int * 8)()
         check(lib().cb_kernel_variant_info(self._h, int(variant), info))
-        return dict(zip(("vw", "items", "nthr", "TR", "NI", "pf", "nt"), info[:]))
+        return dict(zip(("vw", "items", "nthr", "TR", "NI", "pf", "nt", "fuse"), info[:]))
 
     def select_kernel(self, variant):
         check(lib().cb_select_kernel(self._h, int(variant)))
@@ -407,7 +407,7 @@ def server_variant(sim):
     best, score = -1, -1e300
     for v in range(n):
         c = sim.variant_info(v)
-        if c["nthr"] != 256 or c["items"] > 4 or c["pf"] != 1 or c["nt"] != 1:
+        if c["fuse"] == 2 or c["nthr"] != 256 or c["items"] > 4 or c["pf"] != 1 or c["nt"] != 1:
             continue
         s = 1000.0 * c["TR"] - abs(np.log2(c["NI"] / 32.0))
         if s > score:

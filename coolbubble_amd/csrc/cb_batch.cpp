@@ -44,7 +44,7 @@ int pick_variant(const cb_sim *s, int items_max, int pf, int nt, int ni_pref)
     double best_score = -1e300;
     for (size_t v = 0; v < s->variants.size(); ++v) {
         const TiledCfg &c = s->variants[v];
-        if (c.nthr != 256 || c.items > items_max || c.pf != pf || c.nt != nt) continue;
+        if (c.fuse == 2 || c.nthr != 256 || c.items > items_max || c.pf != pf || c.nt != nt) continue;
         const double score = 1000.0 * c.TR - std::fabs(std::log2((double)c.NI / ni_pref));
         if (score > best_score) { best_score = score; best = (int)v; }
     }
@@ -66,7 +66,7 @@ void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size
     m.phi[0] = s->phi[0];
     m.phi[1] = s->phi[1];
     m.pi[0] = s->pi[0];
-    m.pi[1] = s->lagged ? s->pi[1] : s->pi[0];
+    m.pi[1] = s->pi[1] ? s->pi[1] : s->pi[0];
     m.sc = s->dsc;
     m.partials = s->partials;
     m.par_phi = s->iphi;

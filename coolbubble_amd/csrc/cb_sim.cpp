@@ -527,15 +527,15 @@ cb_status cb_kernel_variants(cb_sim *s, int *count, int *selected)
     return CB_OK;
 }
 
-cb_status cb_kernel_variant_info(cb_sim *s, int variant, int info[7])
+cb_status cb_kernel_variant_info(cb_sim *s, int variant, int info[8])
 {
     if (!s || !info || variant < 0 || variant >= (int)s->variants.size()) {
         set_error("invalid handle or variant");
         return CB_INVALID_ARG;
     }
     const cb::TiledCfg &c = s->variants[(size_t)variant];
-    const int v[7] = {c.vw, c.items, c.nthr, c.TR, c.NI, c.pf, c.nt};
-    for (int i = 0; i < 7; ++i) info[i] = v[i];
+    const int v[8] = {c.vw, c.items, c.nthr, c.TR, c.NI, c.pf, c.nt, c.fuse};
+    for (int i = 0; i < 8; ++i) info[i] = v[i];
     return CB_OK;
 }
 

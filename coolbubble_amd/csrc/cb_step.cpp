@@ -45,9 +45,49 @@ static hipError_t launch_one(cb_sim *s, bool energy)
     return e;
 }
 
+// Two steps in one launch (static box): phi and pi both ping-pong.
+static hipError_t launch_pair(cb_sim *s, bool energy)
+{
+    StepPtrs p;
+    p.phi_in = s->phi[s->iphi];
+    p.phi_out = s->phi[s->iphi ^ 1];
+    p.pi_in = s->pi[s->ipi];
+    p.pi_out = s->pi[s->ipi ^ 1];
+    int nblocks = 0;
+    const TiledCfg c = tiled_get(s->tiled);
+    hipError_t e = launch_step2(s->k, s->fp32, energy, c, p, s->dsc, s->partials, s->st, &nblocks);
+    if (e == hipSuccess && energy) e = launch_finalize(s->k, s->partials, nblocks, s->dsc, 1, s->st);
+    return e;
+}
+
 cb_status step_n(cb_sim *s, int nsteps)
 {
-    for (int it = 0; it < nsteps; ++it) {
+    const bool fused = s->variant >= 0 && s->variants[(size_t)s->variant].fuse == 2;
+    int it = 0;
+    if (fused) {
+        for (; it + 2 <= nsteps; it += 2) {
+            const bool energy = it + 2 == nsteps;
+            HIP_TRY(launch_pair(s, energy));
+            s->iphi ^= 1;
+            s->ipi ^= 1;
+            s->t += s->p.dt;
+            s->t += s->p.dt;
+            s->step += 2;
+        }
+        if (it < nsteps) {   // odd remainder on the one-step companion variant
+            const int keep = s->tiled;
+            s->tiled = s->single_variant < 0 ? -1 : tiled_register(s->variants[(size_t)s->single_variant]);
+            hipError_t e = launch_one(s, true);
+            s->tiled = keep;
+            HIP_TRY(e);
+            s->iphi ^= 1;
+            s->t += s->p.dt;
+            s->step += 1;
+            ++it;
+        }
+        return CB_OK;
+    }
+    for (; it < nsteps; ++it) {
         const bool energy = s->lagged || it == nsteps - 1;
         HIP_TRY(launch_one(s, energy));
         s->iphi ^= 1;
@@ -63,6 +103,17 @@ cb_status select_variant(cb_sim *s, int variant)
     if (variant < -1 || variant >= (int)s->variants.size()) {
         set_error("kernel variant out of range");
         return CB_UNSUPPORTED;
+    }
+    if (variant >= 0 && s->variants[(size_t)variant].fuse == 2) {
+        if (!s->pi[1]) {   // the fused kernel needs pi double-buffered
+            if (hipMalloc(&s->pi[1], (size_t)s->nsites * s->esz) != hipSuccess) {
+                set_error("out of device memory for the second pi buffer");
+                return CB_OUT_OF_MEMORY;
+            }
+        }
+        if (s->single_variant < 0)
+            for (size_t v = 0; v < s->variants.size(); ++v)
+                if (s->variants[v].fuse != 2) { s->single_variant = (int)v; break; }
     }
     s->variant = variant;
     s->tiled = variant < 0 ? -1 : tiled_register(s->variants[(size_t)variant]);

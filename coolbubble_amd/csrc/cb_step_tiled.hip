@@ -20,36 +20,10 @@
 #include <vector>
 
 #include "cb_device.hpp"
+#include "cb_tile_common.hpp"
 #include "cb_tiled.hpp"
 
 namespace cb {
-
-template <typename T, int VW>
-struct alignas(sizeof(T) * VW) Vec {
-    T v[VW];
-};
-
-// Wait for this wave's LDS traffic, then the workgroup barrier.  The kernels
-// communicate through LDS only (global data is read from buffers nobody
-// writes during the launch), so no global-memory fence is needed.
-__device__ inline void lds_barrier()
-{
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
-
-template <typename T, int VW>
-__device__ inline void store_vec(T *p, const Vec<T, VW> &v, bool nt)
-{
-    if (nt) {
-        typedef T nvec __attribute__((ext_vector_type(VW)));
-        nvec x;
-#pragma unroll
-        for (int e = 0; e < VW; ++e) x[e] = v.v[e];
-        __builtin_nontemporal_store(x, reinterpret_cast<nvec *>(p));
-    } else {
-        *reinterpret_cast<Vec<T, VW> *>(p) = v;
-    }
-}
 
 // One tile of one step: TR rows x all N columns of every plane i in
 // [lt's i-chunk].  lds holds two plane tiles (double buffer).  Ends with an
@@ -384,14 +358,23 @@ int tiled_blocks(const SimConst &k, const TiledCfg &c)
     return g.ntr * g.nic;
 }
 
+static int single_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap);
+
 int tiled_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
+{
+    int n = single_candidates(k, fp32, out, cap);
+    if (!k.expanding && n < cap) n += step2_candidates(k, fp32, out + n, cap - n);
+    return n;
+}
+
+static int single_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
 {
     int n = 0;
     const int vw = fp32 ? 4 : 2;
     auto push = [&](int items, int nthr, int TR, int NI) {
         for (int pf : {2, 1})
             for (int nt : {0, 1}) {
-                TiledCfg c{vw, items, nthr, TR, NI, pf, nt};
+                TiledCfg c{vw, items, nthr, TR, NI, pf, nt, 1};
                 if (n < cap && tiled_valid(k, fp32, c)) out[n++] = c;
             }
     };
@@ -465,6 +448,7 @@ static void launch_d(const SimConst &k, int lagged, int energy, const TiledCfg &
 hipError_t launch_step_tiled_cfg(const SimConst &k, int fp32, int lagged, int energy, const TiledCfg &c,
                                  const StepPtrs &p, const DevScalars *sc, double *partials, hipStream_t st)
 {
+    if (c.fuse == 2) return hipErrorInvalidValue;   // two-step shapes launch through launch_step2
     if (!tiled_valid(k, fp32, c)) return hipErrorInvalidValue;
     if (lagged && k.D != 3) return hipErrorInvalidValue;
     const size_t shm = lds_bytes(k, fp32, c);
@@ -487,7 +471,7 @@ int tiled_register(const TiledCfg &c)
     for (size_t i = 0; i < g_reg.size(); ++i) {
         const TiledCfg &o = g_reg[i];
         if (o.vw == c.vw && o.items == c.items && o.nthr == c.nthr && o.TR == c.TR && o.NI == c.NI &&
-            o.pf == c.pf && o.nt == c.nt)
+            o.pf == c.pf && o.nt == c.nt && o.fuse == c.fuse)
             return (int)i;
     }
     g_reg.push_back(c);

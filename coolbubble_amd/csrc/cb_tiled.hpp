@@ -18,6 +18,7 @@ struct TiledCfg {
     int NI;        // planes per block
     int pf;        // 2: next-but-one plane (and next halo rows) prefetched into registers; 1: none
     int nt;        // 1: non-temporal stores of phi and pi
+    int fuse;      // steps per launch: 1, or 2 (temporal blocking, static box only)
 };
 
 // Geometry of one launch, derived from (N, cfg).
@@ -73,6 +74,13 @@ hipError_t launch_batch_step(int fp32, int energy, int items, int pf, const Batc
 hipError_t launch_batch_finalize(const BatchMember *table, int nmem, hipStream_t st);
 size_t tiled_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
 TiledGeom tiled_geom(const SimConst &k, const TiledCfg &c);
+
+// Two steps per launch (cb_step2.hip): static box only; pi must ping-pong.
+bool step2_valid(const SimConst &k, int fp32, const TiledCfg &c);
+int step2_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap);
+size_t step2_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
+hipError_t launch_step2(const SimConst &k, int fp32, int energy, const TiledCfg &c, const StepPtrs &p,
+                        const DevScalars *sc, double *partials, hipStream_t st, int *nblocks);
 
 // Registry of chosen configurations (the autotuner writes, the step reads).
 int tiled_register(const TiledCfg &c);

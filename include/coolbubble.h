@@ -134,8 +134,9 @@ cb_status cb_kernel_variants(cb_sim *s, int *count, int *selected);
 cb_status cb_select_kernel(cb_sim *s, int variant);
 /* Shape of tiled variant `variant`: info = {vector width, vectors per thread,
  * threads per block, rows per tile, planes per block, prefetch depth,
- * non-temporal stores}. */
-cb_status cb_kernel_variant_info(cb_sim *s, int variant, int info[7]);
+ * non-temporal stores, steps per launch (1, or 2 = two leapfrog steps per
+ * pass over the lattice, static box only)}. */
+cb_status cb_kernel_variant_info(cb_sim *s, int variant, int info[8]);
 
 /* Autotune: time each variant over `steps` steps (the lattice state is
  * saved and restored around the trials) and select the fastest.
