@@ -71,6 +71,7 @@ void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size
     m.partials = s->partials;
     m.par_phi = s->iphi;
     m.par_pi = s->ipi;
+    m.nsteps = 1 << 30;
     const size_t b = tiled_lds_bytes(s->k, fp32, c);
     if (shm && b > *shm) *shm = b;
 }
@@ -150,8 +151,8 @@ cb_status enqueue_steps(cb_batch *b, int n)
         const bool energy = b->any_lagged || it == n - 1;
         if (!b->tiled.empty()) {
             HIP_TRY(cb::launch_batch_step(b->fp32, energy, b->items, b->pf, b->d_tab, (int)b->tiled.size(),
-                                          b->total_tiles, b->grid, b->shm, it & 1, b->st));
-            if (energy) HIP_TRY(cb::launch_batch_finalize(b->d_tab, (int)b->tiled.size(), b->st));
+                                          b->total_tiles, b->grid, b->shm, it, b->st));
+            if (energy) HIP_TRY(cb::launch_batch_finalize(b->d_tab, (int)b->tiled.size(), it, b->st));
         }
     }
     for (int i : b->general) {

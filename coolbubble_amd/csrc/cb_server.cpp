@@ -5,14 +5,15 @@
 //      device-memory budget allow (a request larger than the whole budget is
 //      rejected), building each one's lattice (Bubble ctor) on the server
 //      stream and recording its initial energy row and spectrum;
-//   2. picks the tick length: the fewest steps until any active request
-//      reaches an event -- an energy row (every 40 steps, src/main.cpp:280),
-//      a snapshot (SnapshotSchedule, src/main.cpp:201-223) or its last step
-//      -- at most 40;
-//   3. steps every active request that many steps with one batched launch
-//      per step over all of their tiles (k_batch_step), energies on the
-//      tick's last step, requests without a tiled shape on their own
-//      general-kernel launches;
+//   2. gives every active request its own step count: the steps to its next
+//      event -- an energy row (every 40 steps, src/main.cpp:280), a snapshot
+//      (SnapshotSchedule, src/main.cpp:201-223) or its last step -- at most
+//      40;
+//   3. runs one batched launch per step over all of their tiles
+//      (k_batch_step); a request whose count is used up skips its tiles;
+//      energies are computed on the steps where some request stops (every
+//      step if one expands) and finalised for the requests still taking
+//      part; requests without a tiled shape run their general kernel;
 //   4. records rows / spectra of the requests at an event, retires finished
 //      ones and frees their memory.
 // A request's arithmetic does not depend on its neighbours or on where tick
@@ -202,25 +203,32 @@ void admit(cb_server *sv)
     }
 }
 
-// One tick over the active requests.
+// One tick over the active requests: each advances to its own next event
+// (at most 40 steps); the batched launches skip the tiles of requests that
+// already stopped.
 cb_status tick(cb_server *sv)
 {
     std::vector<Req *> &act = sv->active;
-    int L = 40;
+    std::vector<int> m(act.size());
+    int L = 1;
     bool any_lagged = false;
-    for (Req *q : act) {
+    for (size_t i = 0; i < act.size(); ++i) {
+        Req *q = act[i];
         const long long step = q->sim->step;
         long long ev = q->r.nt;
         const long long next40 = (step / 40 + 1) * 40;
         if (next40 < ev) ev = next40;
         if (q->next_snap < q->snaps.size() && q->snaps[q->next_snap] < ev) ev = q->snaps[q->next_snap];
-        if (ev - step < L) L = (int)(ev - step);
+        long long n = ev - step;
+        if (n < 1) n = 1;
+        if (n > 40) n = 40;
+        m[i] = (int)n;
+        if (m[i] > L) L = m[i];
         if (q->sim->lagged) any_lagged = true;
     }
-    if (L < 1) L = 1;
     // batched table of the tiled requests
-    std::vector<Req *> tiled, general;
-    for (Req *q : act) (q->sim->variant >= 0 ? tiled : general).push_back(q);
+    std::vector<size_t> tiled, general;
+    for (size_t i = 0; i < act.size(); ++i) (act[i]->sim->variant >= 0 ? tiled : general).push_back(i);
     if (tiled.size() > sv->tab_cap) {
         if (sv->h_tab) hipHostFree(sv->h_tab);
         if (sv->d_tab) hipFree(sv->d_tab);
@@ -236,8 +244,11 @@ cb_status tick(cb_server *sv)
     }
     int tiles = 0;
     size_t shm = 0;
+    std::vector<char> energy_at((size_t)L, any_lagged ? 1 : 0);
     for (size_t t = 0; t < tiled.size(); ++t) {
-        cb::fill_member(sv->h_tab[t], tiled[t]->sim, tiles, sv->o.precision == CB_FP32, &shm);
+        cb::fill_member(sv->h_tab[t], act[tiled[t]]->sim, tiles, sv->o.precision == CB_FP32, &shm);
+        sv->h_tab[t].nsteps = m[tiled[t]];
+        energy_at[(size_t)m[tiled[t]] - 1] = 1;
         tiles += sv->h_tab[t].ntiles;
     }
     hipError_t e = hipSuccess;
@@ -245,17 +256,17 @@ cb_status tick(cb_server *sv)
         e = hipMemcpyAsync(sv->d_tab, sv->h_tab, sizeof(cb::BatchMember) * tiled.size(), hipMemcpyHostToDevice, sv->st);
     const int grid = tiles < sv->cus * 8 ? tiles : sv->cus * 8;
     for (int it = 0; it < L && e == hipSuccess && !tiled.empty(); ++it) {
-        const bool energy = any_lagged || it == L - 1;
+        const bool energy = energy_at[(size_t)it] != 0;
         e = cb::launch_batch_step(sv->o.precision == CB_FP32, energy, 4, 1, sv->d_tab, (int)tiled.size(), tiles, grid,
-                                  shm, it & 1, sv->st);
-        if (e == hipSuccess && energy) e = cb::launch_batch_finalize(sv->d_tab, (int)tiled.size(), sv->st);
+                                  shm, it, sv->st);
+        if (e == hipSuccess && energy) e = cb::launch_batch_finalize(sv->d_tab, (int)tiled.size(), it, sv->st);
     }
     if (e != hipSuccess) { set_error(std::string("server tick: ") + hipGetErrorString(e)); return CB_HIP_ERROR; }
-    for (Req *q : general) {
-        cb_status r = cb::step_n(q->sim, L);
+    for (size_t i : general) {
+        cb_status r = cb::step_n(act[i]->sim, m[i]);
         if (r != CB_OK) return r;
     }
-    for (Req *q : tiled) cb::advance_host(q->sim, L);
+    for (size_t i : tiled) cb::advance_host(act[i]->sim, m[i]);
     if (hipStreamSynchronize(sv->st) != hipSuccess) { set_error("server tick: stream failed"); return CB_HIP_ERROR; }
     // events
     std::vector<Req *> keep;

@@ -202,9 +202,10 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
     s->cfg = simple_cfg(s->nsites);
 
     {   // step-kernel variants and the energy-partials capacity they need
-        cb::TiledCfg cand[64];
-        int nc = cb::tiled_candidates(k, s->fp32, cand, 64);
-        s->variants.assign(cand, cand + nc);
+        static const int kMaxVariants = 256;
+        std::vector<cb::TiledCfg> cand((size_t)kMaxVariants);
+        int nc = cb::tiled_candidates(k, s->fp32, cand.data(), kMaxVariants);
+        s->variants.assign(cand.begin(), cand.begin() + nc);
         s->max_blocks = s->cfg.grid;
         for (const auto &c : s->variants) {
             int nb = cb::tiled_blocks(k, c);

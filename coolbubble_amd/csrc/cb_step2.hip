@@ -330,7 +330,7 @@ int step2_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
         TiledCfg c{vw, items, nthr, TR, NI, 1, nt, 2};
         if (n < cap && step2_valid(k, fp32, c)) out[n++] = c;
     };
-    const int trs3[] = {6, 4, 2, 8, 14, 12, 10, 1};
+    const int trs3[] = {6, 4, 2, 8, 12, 1};
     const int nis[] = {32, 16, 64};
     if (k.D == 3) {
         for (int TR : trs3)

@@ -225,7 +225,7 @@ k_step_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restric
 // with the same config, so a member steps bit-identically alone or batched.
 template <typename T, bool ENERGY, int ITEMS, int PF>
 __global__ void __launch_bounds__(256)
-k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int step_parity)
+k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int step_it)
 {
     constexpr int VW = sizeof(T) == 4 ? 4 : 2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -240,9 +240,10 @@ k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int ste
             else hi = mid - 1;
         }
         const BatchMember &m = table[lo];
+        if (step_it >= m.nsteps) continue;     // this member stopped at an event earlier in the sequence
         const int lt = w - m.tile_begin;
-        const int pphi = (m.par_phi + step_parity) & 1;
-        const int ppi = m.lagged ? ((m.par_pi + step_parity) & 1) : m.par_pi;
+        const int pphi = (m.par_phi + step_it) & 1;
+        const int ppi = m.lagged ? ((m.par_pi + step_it) & 1) : m.par_pi;
         const T *phi_in = (const T *)m.phi[pphi];
         T *phi_out = (T *)m.phi[pphi ^ 1];
         const T *pi_in = (const T *)m.pi[ppi];
@@ -261,17 +262,18 @@ k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int ste
     }
 }
 
-__global__ void __launch_bounds__(256) k_batch_finalize(const BatchMember *__restrict__ table)
+__global__ void __launch_bounds__(256) k_batch_finalize(const BatchMember *__restrict__ table, int step_it)
 {
     const BatchMember &m = table[blockIdx.x];
+    if (step_it >= m.nsteps) return;
     finalize_member(m.k, m.partials, m.ntiles, m.sc, m.lagged ? 0 : 1);
 }
 
 template <typename T, bool ENERGY>
 static void batch_launch_ip(int items, int pf, const BatchMember *table, int nmem, int total, int grid, size_t shm,
-                            int parity, hipStream_t st)
+                            int step_it, hipStream_t st)
 {
-#define CB_BATCH_L(I, P) k_batch_step<T, ENERGY, I, P><<<grid, 256, shm, st>>>(table, nmem, total, parity)
+#define CB_BATCH_L(I, P) k_batch_step<T, ENERGY, I, P><<<grid, 256, shm, st>>>(table, nmem, total, step_it)
     if (pf == 2) {
         if (items == 1) CB_BATCH_L(1, 2);
         else if (items == 2) CB_BATCH_L(2, 2);
@@ -285,21 +287,21 @@ static void batch_launch_ip(int items, int pf, const BatchMember *table, int nme
 }
 
 hipError_t launch_batch_step(int fp32, int energy, int items, int pf, const BatchMember *table, int nmem,
-                             int total_tiles, int grid, size_t shm, int step_parity, hipStream_t st)
+                             int total_tiles, int grid, size_t shm, int step_it, hipStream_t st)
 {
     if (fp32) {
-        if (energy) batch_launch_ip<float, true>(items, pf, table, nmem, total_tiles, grid, shm, step_parity, st);
-        else batch_launch_ip<float, false>(items, pf, table, nmem, total_tiles, grid, shm, step_parity, st);
+        if (energy) batch_launch_ip<float, true>(items, pf, table, nmem, total_tiles, grid, shm, step_it, st);
+        else batch_launch_ip<float, false>(items, pf, table, nmem, total_tiles, grid, shm, step_it, st);
     } else {
-        if (energy) batch_launch_ip<double, true>(items, pf, table, nmem, total_tiles, grid, shm, step_parity, st);
-        else batch_launch_ip<double, false>(items, pf, table, nmem, total_tiles, grid, shm, step_parity, st);
+        if (energy) batch_launch_ip<double, true>(items, pf, table, nmem, total_tiles, grid, shm, step_it, st);
+        else batch_launch_ip<double, false>(items, pf, table, nmem, total_tiles, grid, shm, step_it, st);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_batch_finalize(const BatchMember *table, int nmem, hipStream_t st)
+hipError_t launch_batch_finalize(const BatchMember *table, int nmem, int step_it, hipStream_t st)
 {
-    if (nmem > 0) k_batch_finalize<<<nmem, 256, 0, st>>>(table);
+    if (nmem > 0) k_batch_finalize<<<nmem, 256, 0, st>>>(table, step_it);
     return hipGetLastError();
 }
 
@@ -379,8 +381,8 @@ static int single_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap
             }
     };
     if (k.D == 3) {
-        const int trs[] = {8, 4, 2, 16, 1};
-        const int nis[] = {32, 16, 64, 8, 128};
+        const int trs[] = {8, 4, 2, 1};
+        const int nis[] = {32, 16, 64, 8};
         for (int TR : trs)
             for (int nthr : {256, 128}) {
                 const int nv = TR * (k.N / vw);

@@ -56,6 +56,8 @@ struct BatchMember {
     int tile_begin;     // first work item of this member in the batch
     int ntiles;
     int par_phi, par_pi;
+    int nsteps;         // steps this member takes in the current launch sequence
+    int pad_;
     void *phi[2];
     void *pi[2];
     DevScalars *sc;
@@ -69,9 +71,12 @@ struct BatchMember {
 // parities.  A member's results depend only on its (TR, NI, threads), not on
 // items, pf or the non-temporal flag, so it steps bit-identically alone with
 // a variant of the same (TR, NI, threads).
+// step_it = index of this launch in the sequence: a member takes part while
+// step_it < its nsteps (its buffers flip with step_it & 1).
 hipError_t launch_batch_step(int fp32, int energy, int items, int pf, const BatchMember *table, int nmem,
-                             int total_tiles, int grid, size_t shm, int step_parity, hipStream_t st);
-hipError_t launch_batch_finalize(const BatchMember *table, int nmem, hipStream_t st);
+                             int total_tiles, int grid, size_t shm, int step_it, hipStream_t st);
+// Finalise the energies of the members taking part in launch step_it.
+hipError_t launch_batch_finalize(const BatchMember *table, int nmem, int step_it, hipStream_t st);
 size_t tiled_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
 TiledGeom tiled_geom(const SimConst &k, const TiledCfg &c);
 
