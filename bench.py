@@ -272,6 +272,8 @@ def main():
                           "achieved_GBps": 4 * w * sites / (ms * 1e-3 / K) / 1e9}), flush=True)
         return 0
     sim32, ms32, row32 = run_precision(cb, hip, "fp32", K, W, log, args.variant)
+    sel32 = sim32.kernel_variants()[1]
+    info32 = sim32.variant_info(sel32) if sel32 >= 0 else None
     spec32 = spectrum_time(sim32, hip)
     log("fp32 spectrum: %.3f ms" % spec32)
     sim32.close()
@@ -280,6 +282,14 @@ def main():
     log("fp64 spectrum: %.3f ms" % spec64)
 
     su32 = sites * K / (ms32 * 1e-3)
+    traffic32, steps_per_launch32 = None, info32.get("fuse", 1) if info32 else 1
+    try:
+        rec = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                          "hbm_traffic.json")))["fp32"]
+        if info32 and rec["variant"] == sel32:
+            traffic32 = rec["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        pass
     su64 = sites * K / (ms64 * 1e-3)
     t_step32 = ms32 * 1e-3 / K
     bytes32 = 16.0 * sites          # read phi, read pi, write pi, write phi (FP32)
@@ -317,7 +327,12 @@ def main():
                                "dt=0.025 static, energies every 40 steps",
                    "sites": sites, "energy_every": ENERGY_EVERY},
         "roofline": {"bound": "hbm", "achieved": ach32 / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": ach32 / HBM_PEAK, "traffic": None},
+                     "frac": ach32 / HBM_PEAK, "traffic": traffic32,
+                     "steps_per_launch": steps_per_launch32,
+                     "note": "achieved = 16 B per site-update x site-updates/s (one step per pass over "
+                             "phi and pi); variants that fuse two steps per pass read and write each "
+                             "byte once per two steps, so this can exceed 1. traffic = measured HBM "
+                             "bytes per launch of the profiled variant (profiles/hbm_traffic.json)"},
         "cpu_baseline": cpu,
         "fp64": {"value": su64, "ms_per_step": ms64 / K, "achieved_GBps": ach64 / 1e9,
                  "frac": ach64 / HBM_PEAK},

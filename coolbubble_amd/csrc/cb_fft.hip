@@ -22,35 +22,16 @@
 #include <vector>
 
 #include "cb_device.hpp"
+#include "cb_fft.hpp"
 #include "cb_sim.hpp"
 
 namespace cb {
-
-template <typename R>
-struct cpx {
-    R x, y;
-};
-
-static constexpr int kMaxStages = 40;
-static constexpr size_t kLdsBudget = 65536;
-
-struct Stages {
-    int n;
-    int ns;
-    int radix[kMaxStages];
-};
 
 // ---------------------------------------------------------------------------
 // Device: batched Stockham FFT in LDS.  Sequence b, element e at
 // buf[b * bs + e * es].  sign = -1 forward, +1 backward; tw[e] =
 // exp(-2 pi i e / n).  Returns the buffer holding the result.
 // ---------------------------------------------------------------------------
-template <typename R>
-__device__ inline cpx<R> cmul(cpx<R> a, cpx<R> b)
-{
-    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-
 template <typename R>
 __device__ inline cpx<R> twid(const cpx<R> *tw, int e, int n, int sign)
 {
@@ -522,6 +503,13 @@ static hipError_t forward_r2c(const Stages &st, const cpx<R> *tw, const T *field
 {
     const int N = st.n, nc = N / 2 + 1;
     const long long rows = (D == 3) ? (long long)N * N : (D == 2 ? N : 1);
+    if (p2_supported(N, sizeof(cpx<R>))) {
+        const int f32 = sizeof(R) == 4;
+        hipError_t e = p2_rows_r2c(f32, field, H, rows, N, tw, s);
+        if (e == hipSuccess && D == 3) e = p2_cols(f32, H, N, nc, N, tw, -1, s);        // axis 1
+        if (e == hipSuccess && D >= 2) e = p2_cols(f32, H, 1, rows / N * nc, N, tw, -1, s); // axis 0
+        return e;
+    }
     const int RB = batch_for(N, sizeof(cpx<R>));
     if (RB < 1) return hipErrorInvalidValue;
     size_t shm = 2 * (size_t)RB * N * sizeof(cpx<R>);
@@ -724,6 +712,16 @@ cb_status thermal_field(cb_sim *s, void *field, double T, double m, int velocity
     k_hermitian<<<nb, 256, 0, s->st>>>((cpx<double> *)H, rows, N, D);
     const int CB = batch_for(N, sizeof(cpx<double>));
     const size_t shm = 2 * (size_t)CB * N * sizeof(cpx<double>);
+    const double norm = 1.0 / std::sqrt(std::pow(s->p.L, D));
+    if (p2_supported(N, sizeof(cpx<double>))) {
+        e = p2_cols(0, H, 1, rows / N * nc, N, tw, +1, s->st);                 // axis 0
+        if (e == hipSuccess && D == 3) e = p2_cols(0, H, N, nc, N, tw, +1, s->st);   // axis 1
+        if (e == hipSuccess) e = p2_rows_c2r(s->fp32, H, field, rows, N, tw, norm, 0.0, s->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+        cleanup();
+        if (e != hipSuccess) { set_error(std::string("thermal IC: ") + hipGetErrorString(e)); return CB_HIP_ERROR; }
+        return CB_OK;
+    }
     {   // axis 0 then (3D) axis 1, backward
         const long long inner = rows / N * nc;
         const long long tiles = (inner + CB - 1) / CB;
@@ -734,7 +732,6 @@ cb_status thermal_field(cb_sim *s, void *field, double T, double m, int velocity
         }
     }
     const int RB = CB;
-    const double norm = 1.0 / std::sqrt(std::pow(s->p.L, D));
     if (s->fp32)
         k_rows_c2r<float, double><<<(unsigned)((rows + RB - 1) / RB), 256, shm, s->st>>>((const cpx<double> *)H, (float *)field, rows, RB, st, (const cpx<double> *)tw, norm, 0.0);
     else

@@ -1,0 +1,43 @@
+// cb_fft.hpp -- types shared by the FFT kernels (internal).
+#pragma once
+
+#include "cb_common.hpp"
+
+namespace cb {
+
+template <typename R>
+struct cpx {
+    R x, y;
+};
+
+static constexpr int kMaxStages = 40;
+static constexpr size_t kLdsBudget = 65536;
+
+struct Stages {
+    int n;
+    int ns;
+    int radix[kMaxStages];
+};
+
+template <typename R>
+__device__ inline cpx<R> cmul(cpx<R> a, cpx<R> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+
+// Power-of-two lengths 8..2048: compile-time radix-8/4/2 transforms
+// (cb_fft_p2.hip).  tw = exp(-2 pi i e / N), e < N, in the transform's
+// precision.  Each returns hipErrorInvalidValue for an unsupported N.
+bool p2_supported(int N, size_t complex_bytes);
+// real rows (T = R) -> half-complex rows, forward
+hipError_t p2_rows_r2c(int fp32, const void *field, void *H, long long rows, int N, const void *tw, hipStream_t st);
+// in-place transform along the middle axis of [outer][N][inner]
+hipError_t p2_cols(int fp32, void *H, long long outer, long long inner, int N, const void *tw, int sign,
+                   hipStream_t st);
+// half-complex rows (double) -> real rows (float or double), backward,
+// field = x * norm + shift
+hipError_t p2_rows_c2r(int field_fp32, const void *H, void *field, long long rows, int N, const void *tw,
+                       double norm, double shift, hipStream_t st);
+
+} // namespace cb
