@@ -12,6 +12,9 @@
 #include "cb_sim.hpp"
 #include "cb_tiled.hpp"
 
+#include <algorithm>
+#include <vector>
+
 using cb::set_error;
 
 #define HIP_TRY(expr)                                                                    \
@@ -158,29 +161,41 @@ extern "C" cb_status cb_autotune(cb_sim *s, int steps, double *ms_per_step, int 
         return r;
     };
     const int nvar = (int)s->variants.size();
-    double best_ms = 1e300;
-    int best_v = -1;
-    for (int v = 0; v <= nvar && e == hipSuccess; ++v) {
-        const int var = (v == nvar) ? -1 : v;
+    // time `reps` runs of `nsteps` steps of one variant (-1 = general kernel); min per step
+    auto trial = [&](int var, int nsteps, int reps) -> double {
         select_variant(s, var);
         double per = 1e300;
-        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+        for (int rep = 0; rep < reps && e == hipSuccess; ++rep) {
             e = restore();
             if (e != hipSuccess) break;
-            if (step_n(s, 1) != CB_OK) { e = hipErrorUnknown; break; }
+            if (step_n(s, 2) != CB_OK) { e = hipErrorUnknown; break; }
             e = hipEventRecord(ev[0], s->st);
-            // one call of `steps` steps: energies on its last step only, as in a
-            // run that prints every `steps` steps
-            if (e == hipSuccess && step_n(s, steps) != CB_OK) e = hipErrorUnknown;
+            // one call: energies on its last step only, as in a run printing every `nsteps`
+            if (e == hipSuccess && step_n(s, nsteps) != CB_OK) e = hipErrorUnknown;
             if (e == hipSuccess) e = hipEventRecord(ev[1], s->st);
             if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
             float ms = 0.f;
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
-            if ((double)ms / steps < per) per = (double)ms / steps;
+            if ((double)ms / nsteps < per) per = (double)ms / nsteps;
         }
-        if (ms_per_step && v < cap) ms_per_step[v] = per;
-        if (per < best_ms) { best_ms = per; best_v = var; }
+        return per;
+    };
+    // phase 1: every variant, short runs; phase 2: the 8 fastest, runs 4x longer
+    std::vector<double> quick((size_t)nvar + 1, 1e300);
+    for (int v = 0; v <= nvar && e == hipSuccess; ++v) quick[(size_t)v] = trial(v == nvar ? -1 : v, steps, 2);
+    std::vector<int> order((size_t)nvar + 1);
+    for (int v = 0; v <= nvar; ++v) order[(size_t)v] = v;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return quick[(size_t)a] < quick[(size_t)b]; });
+    std::vector<double> final_ms(quick);
+    double best_ms = 1e300;
+    int best_v = -1;
+    for (int i = 0; i < (int)order.size() && i < 8 && e == hipSuccess; ++i) {
+        const int v = order[(size_t)i];
+        final_ms[(size_t)v] = trial(v == nvar ? -1 : v, 4 * steps, 3);
+        if (final_ms[(size_t)v] < best_ms) { best_ms = final_ms[(size_t)v]; best_v = v == nvar ? -1 : v; }
     }
+    if (ms_per_step)
+        for (int v = 0; v <= nvar && v < cap; ++v) ms_per_step[v] = final_ms[(size_t)v];
     hipError_t e2 = restore();
     hipEventDestroy(ev[0]);
     hipEventDestroy(ev[1]);

@@ -138,10 +138,11 @@ cb_status cb_select_kernel(cb_sim *s, int variant);
  * pass over the lattice, static box only)}. */
 cb_status cb_kernel_variant_info(cb_sim *s, int variant, int info[8]);
 
-/* Autotune: time each variant over `steps` steps (the lattice state is
- * saved and restored around the trials) and select the fastest.
- * ms_per_step (optional, capacity cap) receives each variant's time,
- * general kernel last. */
+/* Autotune: time every variant over `steps` steps, then the 8 fastest again
+ * over 4 x `steps` steps (the lattice state is saved and restored around
+ * the trials), and select the fastest.  ms_per_step (optional, capacity
+ * cap) receives each variant's time per step (the longer trial where there
+ * was one), general kernel last. */
 cb_status cb_autotune(cb_sim *s, int steps, double *ms_per_step, int cap, int *best);
 
 /* The HIP stream (hipStream_t) every kernel of this handle runs on. */
