@@ -202,7 +202,9 @@ k_cols(cpx<R> *H, long long inner, int CB, Stages st, const cpx<R> *__restrict__
 // Radial binning of the half-complex spectrum, ComputePowerSpectrum
 // (src/bubble.cpp:668-709): per block an LDS histogram of (power, weight,
 // weighted k) over a run of rows, written out as one partial per block.
-template <typename R>
+// GEOM: weights and weighted k only (they do not depend on the field; H is
+// not read).
+template <typename R, bool GEOM = false>
 __global__ void __launch_bounds__(256)
 k_bin(const cpx<R> *__restrict__ H, long long rows, int N, int D, int nBins, double dk, double pref,
       double Pnorm, double *__restrict__ binpart)
@@ -234,10 +236,12 @@ k_bin(const cpx<R> *__restrict__ H, long long rows, int N, int D, int nBins, dou
         const int bin = (int)n;
         if (bin >= nBins) continue;
         const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
-        const cpx<R> f = H[m];
         const double kc = n * dk;
-        const double fx = (double)f.x, fy = (double)f.y;
-        atomicAdd(&hp[bin], w * pref * ipow_rn(kc, D) * Pnorm * (fx * fx + fy * fy));
+        if (!GEOM) {
+            const cpx<R> f = H[m];
+            const double fx = (double)f.x, fy = (double)f.y;
+            atomicAdd(&hp[bin], w * pref * ipow_rn(kc, D) * Pnorm * (fx * fx + fy * fy));
+        }
         atomicAdd(&hw[bin], w);
         atomicAdd(&hk[bin], w * kc);
     }
@@ -451,6 +455,11 @@ struct FftPlan {
     long long rows = 0;
     int nBins = 0, binblocks = 0;
     double *binpart = nullptr, *bins = nullptr;
+    // power-of-two D = 2 / 3: binning fused into the last column pass, the
+    // field-independent weights and weighted k of each bin computed once
+    bool fused = false;
+    int fused_blocks = 0;
+    std::vector<double> wsks;   // [nBins] weights, [nBins] weighted k
     // long 1D rows (four-step): N = N1 N2
     int N1 = 0, N2 = 0;
     Stages st1{}, st2{};
@@ -503,7 +512,7 @@ static hipError_t forward_r2c(const Stages &st, const cpx<R> *tw, const T *field
 {
     const int N = st.n, nc = N / 2 + 1;
     const long long rows = (D == 3) ? (long long)N * N : (D == 2 ? N : 1);
-    if (p2_supported(N, sizeof(cpx<R>))) {
+    if (p2_supported(N)) {
         const int f32 = sizeof(R) == 4;
         hipError_t e = p2_rows_r2c(f32, field, H, rows, N, tw, s);
         if (e == hipSuccess && D == 3) e = p2_cols(f32, H, N, nc, N, tw, -1, s);        // axis 1
@@ -613,8 +622,32 @@ static cb_status plan_for(cb_sim *s)
     p->binblocks = (int)((modes + 4095) / 4096);
     if (p->binblocks > 256) p->binblocks = 256;
     if (p->binblocks < 1) p->binblocks = 1;
-    if (e == hipSuccess) e = hipMalloc((void **)&p->binpart, sizeof(double) * 3 * (size_t)p->nBins * p->binblocks);
+    p->fused = D >= 2 && !p->N1 && p2_supported(N);
+    size_t part = 3 * (size_t)p->nBins * p->binblocks;
+    if (p->fused) {
+        p->fused_blocks = p2_bin_blocks_max();
+        part = std::max(part, (size_t)p->nBins * p->fused_blocks);
+    }
+    if (e == hipSuccess) e = hipMalloc((void **)&p->binpart, sizeof(double) * part);
     if (e == hipSuccess) e = hipMalloc((void **)&p->bins, sizeof(double) * 3 * (size_t)p->nBins);
+    if (e == hipSuccess && p->fused) {
+        // weights and weighted k per bin, once
+        const int nB = p->nBins;
+        const size_t shm = sizeof(double) * 3 * (size_t)nB;
+        if (shm > kLdsBudget) {
+            fft_destroy(p);
+            set_error("too many spectrum bins for LDS");
+            return CB_UNSUPPORTED;
+        }
+        const double dk = 2.0 * 3.14159265358979323846 / s->p.L;
+        k_bin<double, true><<<p->binblocks, 256, shm, s->st>>>(nullptr, p->rows, N, D, nB, dk, 0.0, 0.0, p->binpart);
+        k_bin_reduce<<<(3 * nB + 63) / 64, 256, 0, s->st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
+        std::vector<double> h(3 * (size_t)nB);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), p->bins, sizeof(double) * 3 * nB, hipMemcpyDeviceToHost, s->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+        p->wsks.assign(h.begin() + nB, h.end());
+    }
     if (e != hipSuccess) {
         fft_destroy(p);
         set_error(std::string("FFT plan: ") + hipGetErrorString(e));
@@ -636,7 +669,20 @@ cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int c
     const double dk = 2.0 * PI / L;
     const double pref = (D == 1) ? 1.0 / PI : (D == 2) ? 1.0 / (2.0 * PI) : 1.0 / (2.0 * PI * PI);
     const double Pnorm = s->dxD * s->dxD / std::pow(L, D);
-    if (p->N1) {
+    std::vector<double> h(3 * (size_t)nB);
+    if (p->fused) {
+        // rows r2c, (3D) axis 1, then axis 0 fused with the binning
+        const int f32 = s->fp32 ? 1 : 0, nc = N / 2 + 1;
+        HIP_TRY(p2_rows_r2c(f32, field, p->H, p->rows, N, p->tw, s->st));
+        if (D == 3) HIP_TRY(p2_cols(f32, p->H, N, nc, N, p->tw, -1, s->st));
+        int nblk = 0;
+        HIP_TRY(p2_cols_bin(f32, p->H, p->rows / N * nc, N, p->tw, D, nB, dk, pref, Pnorm, p->binpart,
+                            p->fused_blocks, &nblk, s->st));
+        k_bin_reduce<<<(nB + 63) / 64, 256, 0, s->st>>>(p->binpart, nblk, nB, p->bins);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(h.data(), p->bins, sizeof(double) * nB, hipMemcpyDeviceToHost, s->st));
+        std::copy(p->wsks.begin(), p->wsks.end(), h.begin() + nB);
+    } else if (p->N1) {
         HIP_TRY(s->fp32 ? long_row<float>(p, (const float *)field, pref, dk, Pnorm, s->st)
                         : long_row<double>(p, (const double *)field, pref, dk, Pnorm, s->st));
     } else {
@@ -652,8 +698,7 @@ cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int c
         k_bin_reduce<<<(3 * nB + 63) / 64, 256, 0, s->st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
         HIP_TRY(hipGetLastError());
     }
-    std::vector<double> h(3 * (size_t)nB);
-    HIP_TRY(hipMemcpyAsync(h.data(), p->bins, sizeof(double) * 3 * nB, hipMemcpyDeviceToHost, s->st));
+    if (!p->fused) HIP_TRY(hipMemcpyAsync(h.data(), p->bins, sizeof(double) * 3 * nB, hipMemcpyDeviceToHost, s->st));
     cb::DevScalars sc;
     HIP_TRY(hipMemcpyAsync(&sc, s->dsc, sizeof sc, hipMemcpyDeviceToHost, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
@@ -713,7 +758,7 @@ cb_status thermal_field(cb_sim *s, void *field, double T, double m, int velocity
     const int CB = batch_for(N, sizeof(cpx<double>));
     const size_t shm = 2 * (size_t)CB * N * sizeof(cpx<double>);
     const double norm = 1.0 / std::sqrt(std::pow(s->p.L, D));
-    if (p2_supported(N, sizeof(cpx<double>))) {
+    if (p2_supported(N)) {
         e = p2_cols(0, H, 1, rows / N * nc, N, tw, +1, s->st);                 // axis 0
         if (e == hipSuccess && D == 3) e = p2_cols(0, H, N, nc, N, tw, +1, s->st);   // axis 1
         if (e == hipSuccess) e = p2_rows_c2r(s->fp32, H, field, rows, N, tw, norm, 0.0, s->st);

@@ -29,12 +29,19 @@ __device__ inline cpx<R> cmul(cpx<R> a, cpx<R> b)
 // Power-of-two lengths 8..2048: compile-time radix-8/4/2 transforms
 // (cb_fft_p2.hip).  tw = exp(-2 pi i e / N), e < N, in the transform's
 // precision.  Each returns hipErrorInvalidValue for an unsupported N.
-bool p2_supported(int N, size_t complex_bytes);
+bool p2_supported(int N);
 // real rows (T = R) -> half-complex rows, forward
 hipError_t p2_rows_r2c(int fp32, const void *field, void *H, long long rows, int N, const void *tw, hipStream_t st);
 // in-place transform along the middle axis of [outer][N][inner]
 hipError_t p2_cols(int fp32, void *H, long long outer, long long inner, int N, const void *tw, int sign,
                    hipStream_t st);
+// last forward pass of a D = 2 / 3 spectrum: transform axis 0 of [N][inner]
+// (inner = nc, or N nc) and bin the modes instead of storing them; writes
+// *nblk (<= cap_blocks) rows of nBins power partials to binpart.
+hipError_t p2_cols_bin(int fp32, const void *H, long long inner, int N, const void *tw, int D, int nBins, double dk,
+                       double pref, double Pnorm, double *binpart, int cap_blocks, int *nblk, hipStream_t st);
+// largest block count p2_cols_bin uses
+int p2_bin_blocks_max();
 // half-complex rows (double) -> real rows (float or double), backward,
 // field = x * norm + shift
 hipError_t p2_rows_c2r(int field_fp32, const void *H, void *field, long long rows, int N, const void *tw,

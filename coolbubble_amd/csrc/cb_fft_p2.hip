@@ -4,16 +4,24 @@
 // Stockham autosort with the radices fixed at compile time: radix-8 stages,
 // then one radix-4 or radix-2 stage.  A sequence is processed by N/8 threads;
 // in each stage a thread loads the inputs of its 8/R butterflies from LDS
-// into registers, applies the twiddles (from an LDS copy of the table), then
-// after a barrier writes the butterfly outputs back in place.  A 256-thread
-// block runs 2048/N sequences at a time.
+// into registers, applies the twiddles, then after a barrier writes the
+// butterfly outputs back in place.  A 256-thread block transforms 2048/N
+// sequences at a time (16 KiB of FP32 or 32 KiB of FP64 data), so several
+// blocks share a CU and one block's loads overlap another's arithmetic.
+// LDS positions carry one padding slot per 16 elements, which spreads the
+// power-of-two strides of the Stockham writes over the banks.
 //   rows r2c: two real rows per complex sequence (z = a + i b), split after
 //             the transform: A_k = (Z_k + conj Z_{N-k}) / 2,
 //             B_k = (Z_k - conj Z_{N-k}) / 2i;
 //   rows c2r: two Hermitian rows per sequence (Z = A + i B), real and
 //             imaginary parts of the backward transform;
-//   columns : CB adjacent columns of a strided axis, transposed into LDS
-//             with one element of padding per sequence.
+//   columns : 2048/N adjacent columns of a strided axis, transposed into LDS;
+//             the last forward pass of a spectrum bins the modes instead of
+//             writing them back (ComputePowerSpectrum, src/bubble.cpp:668-709).
+// The kernels are persistent: a grid of as many blocks as are resident at
+// once walks the work items, split into 8 contiguous ranges, one per XCD
+// (blocks are dispatched to the XCDs round-robin), so blocks that share cache
+// lines (neighbouring columns) run at the same time behind the same L2.
 #include "cb_fft.hpp"
 
 namespace cb {
@@ -23,6 +31,20 @@ namespace {
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 constexpr int nstages(int lg) { return lg / 3 + (lg % 3 ? 1 : 0); }
 constexpr int stage_radix(int lg, int s) { return s < lg / 3 ? 8 : (lg % 3 == 2 ? 4 : 2); }
+
+// LDS slot of element x of a sequence
+__host__ __device__ constexpr int pidx(int x) { return x + (x >> 4); }
+// sequences per block: one round of 256 threads at N/8 threads per sequence
+template <int N>
+constexpr int kSeq = (2048 / N) < 1 ? 1 : 2048 / N;
+// LDS stride between sequences (rows / transposed columns)
+template <int N>
+constexpr int kRowStride = pidx(N) + 1;
+template <typename R, int N>
+constexpr int kColStride = pidx(N) + (sizeof(R) == 4 ? (N >= 256 ? 4 : 1) : (N >= 512 ? 2 : 1));
+// twiddle table staged in LDS when it is small, else read through the caches
+template <typename R, int N>
+constexpr bool kTwLds = (size_t)N * 2 * sizeof(R) <= 8192;
 
 // multiply by -i (forward) or +i (backward)
 template <typename R>
@@ -76,7 +98,7 @@ __device__ inline void dft8(cpx<R> *v, int sign)
     v[7] = csub(e3, w3o);
 }
 
-// One Stockham stage on one sequence (contiguous, in place).
+// One Stockham stage on one sequence (in place, padded slots).
 template <typename R, int N, int RADIX, int NS>
 __device__ inline void p2_stage(cpx<R> *seq, int t, const cpx<R> *twl, int sign, bool active)
 {
@@ -92,7 +114,7 @@ __device__ inline void p2_stage(cpx<R> *seq, int t, const cpx<R> *twl, int sign,
             const int k = j & (NS - 1);
 #pragma unroll
             for (int r = 0; r < RADIX; ++r) {
-                cpx<R> a = seq[j + r * M];
+                cpx<R> a = seq[pidx(j + r * M)];
                 if (NS > 1 && r > 0) {
                     cpx<R> w = twl[r * k * TSTEP];
                     if (sign > 0) w.y = -w.y;
@@ -114,7 +136,7 @@ __device__ inline void p2_stage(cpx<R> *seq, int t, const cpx<R> *twl, int sign,
             else dft2(x);
             const int dst = (j / NS) * NS * RADIX + k;
 #pragma unroll
-            for (int q = 0; q < RADIX; ++q) seq[dst + q * NS] = x[q];
+            for (int q = 0; q < RADIX; ++q) seq[pidx(dst + q * NS)] = x[q];
         }
     }
     __syncthreads();
@@ -131,204 +153,363 @@ __device__ inline void p2_stages(cpx<R> *seq, int t, const cpx<R> *twl, int sign
     }
 }
 
-// Transform nseq sequences at buf + s * bs; all 256 threads of the block call it.
+// Transform the kSeq<N> sequences at buf + s * bs; all 256 threads call it.
 template <typename R, int N>
-__device__ inline void p2_fft(cpx<R> *buf, int nseq, int bs, const cpx<R> *twl, int sign)
+__device__ inline void p2_fft(cpx<R> *buf, int bs, const cpx<R> *twl, int sign)
 {
     constexpr int TPS = N / 8;
-    constexpr int SEQ = 256 / TPS;
     const int g = threadIdx.x / TPS, t = threadIdx.x - g * TPS;
-    for (int s0 = 0; s0 < nseq; s0 += SEQ) {
-        const int s = s0 + g;
-        const bool active = s < nseq;
-        p2_stages<R, N, 0, 1>(buf + (size_t)(active ? s : 0) * bs, t, twl, sign, active);
+    const bool active = g < kSeq<N>;
+    p2_stages<R, N, 0, 1>(buf + (size_t)(active ? g : 0) * bs, t, twl, sign, active);
+}
+
+// LDS layout: [twiddles (when staged)] [data] [bins]
+template <typename R, int N>
+__device__ inline const cpx<R> *stage_twiddles(unsigned char *smem, const cpx<R> *tw)
+{
+    if constexpr (kTwLds<R, N>) {
+        cpx<R> *twl = reinterpret_cast<cpx<R> *>(smem);
+        for (int e = threadIdx.x; e < N; e += blockDim.x) twl[e] = tw[e];
+        return twl;
+    } else {
+        return tw;
     }
 }
 
 template <typename R, int N>
-__device__ inline cpx<R> *load_twiddles(unsigned char *smem, const cpx<R> *tw)
+__device__ inline cpx<R> *data_area(unsigned char *smem)
 {
-    cpx<R> *twl = reinterpret_cast<cpx<R> *>(smem);
-    for (int e = threadIdx.x; e < N; e += blockDim.x) twl[e] = tw[e];
-    return twl;
-}
-
-template <typename T, typename R, int N>
-__global__ void __launch_bounds__(256)
-k_p2_rows_r2c(const T *__restrict__ field, cpx<R> *__restrict__ H, long long rows, int nseq,
-              const cpx<R> *__restrict__ tw)
-{
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int nc = N / 2 + 1;
-    cpx<R> *twl = load_twiddles<R, N>(smem_raw, tw);
-    cpx<R> *buf = twl + N;
-    const long long r0 = (long long)blockIdx.x * 2 * nseq;
-    const long long left = rows - r0;
-    const int nr = (int)(left < 2 * nseq ? left : 2 * nseq);
-    const T *src = field + r0 * N;
-    for (int idx = threadIdx.x; idx < 2 * nseq * N; idx += blockDim.x) {
-        const int row = idx / N, e = idx - row * N;
-        const R v = (row < nr) ? (R)src[idx] : (R)0;
-        R *slot = reinterpret_cast<R *>(&buf[(row >> 1) * N + e]);
-        slot[row & 1] = v;
-    }
-    __syncthreads();
-    p2_fft<R, N>(buf, nseq, N, twl, -1);
-    for (int idx = threadIdx.x; idx < nseq * nc; idx += blockDim.x) {
-        const int s = idx / nc, k = idx - s * nc;
-        const cpx<R> z = buf[s * N + k], zn = buf[s * N + ((N - k) & (N - 1))];
-        const cpx<R> A = {(R)0.5 * (z.x + zn.x), (R)0.5 * (z.y - zn.y)};
-        const cpx<R> B = {(R)0.5 * (z.y + zn.y), (R)0.5 * (zn.x - z.x)};
-        const int ra = 2 * s, rb = 2 * s + 1;
-        if (ra < nr) H[(r0 + ra) * nc + k] = A;
-        if (rb < nr) H[(r0 + rb) * nc + k] = B;
-    }
-}
-
-template <typename T, typename R, int N>
-__global__ void __launch_bounds__(256)
-k_p2_rows_c2r(const cpx<R> *__restrict__ H, T *__restrict__ field, long long rows, int nseq,
-              const cpx<R> *__restrict__ tw, double norm, double shift)
-{
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int nc = N / 2 + 1;
-    cpx<R> *twl = load_twiddles<R, N>(smem_raw, tw);
-    cpx<R> *buf = twl + N;
-    const long long r0 = (long long)blockIdx.x * 2 * nseq;
-    const long long left = rows - r0;
-    const int nr = (int)(left < 2 * nseq ? left : 2 * nseq);
-    auto ext = [&](int row, int k) -> cpx<R> {
-        if (row >= nr) return {(R)0, (R)0};
-        const cpx<R> *h = H + (r0 + row) * nc;
-        if (k == 0) return {h[0].x, (R)0};
-        if (2 * k < N) return h[k];
-        if (2 * k == N) return {h[k].x, (R)0};
-        const cpx<R> c = h[N - k];
-        return {c.x, -c.y};
-    };
-    for (int idx = threadIdx.x; idx < nseq * N; idx += blockDim.x) {
-        const int s = idx / N, k = idx - s * N;
-        const cpx<R> A = ext(2 * s, k), B = ext(2 * s + 1, k);
-        buf[idx] = {A.x - B.y, A.y + B.x};
-    }
-    __syncthreads();
-    p2_fft<R, N>(buf, nseq, N, twl, +1);
-    T *dst = field + r0 * N;
-    for (int idx = threadIdx.x; idx < 2 * nseq * N; idx += blockDim.x) {
-        const int row = idx / N, e = idx - row * N;
-        if (row >= nr) continue;
-        const cpx<R> z = buf[(row >> 1) * N + e];
-        const R v = (row & 1) ? z.y : z.x;
-        dst[idx] = (T)((double)v * norm + shift);
-    }
+    return reinterpret_cast<cpx<R> *>(smem) + (kTwLds<R, N> ? N : 0);
 }
 
 template <typename R, int N>
+constexpr size_t twiddle_bytes() { return kTwLds<R, N> ? (size_t)N * sizeof(cpx<R>) : 0; }
+
+// Persistent walk over `items`: 8 contiguous ranges, one per XCD.
+struct Sched {
+    long long begin, end, stride;
+};
+__device__ inline Sched xcd_sched(long long items)
+{
+    const long long G = gridDim.x, b = blockIdx.x;
+    if (G % 8 != 0 || items < G) return {b, items, G};
+    const long long per = G / 8;
+    const long long chunk = (items + 7) / 8;
+    const long long lo = (b % 8) * chunk;
+    const long long hi = lo + chunk < items ? lo + chunk : items;
+    return {lo + b / 8, hi, per};
+}
+
+template <typename T>
+struct Vec16 {
+    T v[16 / sizeof(T)];
+};
+
+template <typename T, typename R, int N>
 __global__ void __launch_bounds__(256)
-k_p2_cols(cpx<R> *H, long long inner, int CB, const cpx<R> *__restrict__ tw, int sign)
+k_p2_rows_r2c(const T *__restrict__ field, cpx<R> *__restrict__ H, long long rows, const cpx<R> *__restrict__ tw)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int bs = N + 1;
-    cpx<R> *twl = load_twiddles<R, N>(smem_raw, tw);
-    cpx<R> *buf = twl + N;
+    constexpr int nc = N / 2 + 1, NSQ = kSeq<N>, BS = kRowStride<N>;
+    constexpr int VW = 16 / sizeof(T);
+    const cpx<R> *twl = stage_twiddles<R, N>(smem_raw, tw);
+    cpx<R> *buf = data_area<R, N>(smem_raw);
+    const long long items = (rows + 2 * NSQ - 1) / (2 * NSQ);
+    const Sched sc = xcd_sched(items);
+    for (long long it = sc.begin; it < sc.end; it += sc.stride) {
+        const long long r0 = it * 2 * NSQ;
+        const long long left = rows - r0;
+        const int nr = (int)(left < 2 * NSQ ? left : 2 * NSQ);
+        const Vec16<T> *src = reinterpret_cast<const Vec16<T> *>(field + r0 * N);
+        __syncthreads();   // the previous item's readers are done with buf
+        for (int q = threadIdx.x; q < 2 * NSQ * N / VW; q += 256) {
+            const int idx = q * VW;
+            const int row = idx / N, e = idx - row * N;
+            Vec16<T> x;
+            if (row < nr) {
+                x = src[q];
+            } else {
+#pragma unroll
+                for (int u = 0; u < VW; ++u) x.v[u] = (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < VW; ++u) {
+                R *slot = reinterpret_cast<R *>(&buf[(row >> 1) * BS + pidx(e + u)]);
+                slot[row & 1] = (R)x.v[u];
+            }
+        }
+        __syncthreads();
+        p2_fft<R, N>(buf, BS, twl, -1);
+        for (int idx = threadIdx.x; idx < NSQ * nc; idx += 256) {
+            const int s = idx / nc, k = idx - s * nc;
+            const cpx<R> z = buf[s * BS + pidx(k)], zn = buf[s * BS + pidx((N - k) & (N - 1))];
+            const cpx<R> A = {(R)0.5 * (z.x + zn.x), (R)0.5 * (z.y - zn.y)};
+            const cpx<R> B = {(R)0.5 * (z.y + zn.y), (R)0.5 * (zn.x - z.x)};
+            const int ra = 2 * s, rb = 2 * s + 1;
+            if (ra < nr) H[(r0 + ra) * nc + k] = A;
+            if (rb < nr) H[(r0 + rb) * nc + k] = B;
+        }
+    }
+}
+
+template <typename T, typename R, int N>
+__global__ void __launch_bounds__(256)
+k_p2_rows_c2r(const cpx<R> *__restrict__ H, T *__restrict__ field, long long rows, const cpx<R> *__restrict__ tw,
+              double norm, double shift)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int nc = N / 2 + 1, NSQ = kSeq<N>, BS = kRowStride<N>;
+    const cpx<R> *twl = stage_twiddles<R, N>(smem_raw, tw);
+    cpx<R> *buf = data_area<R, N>(smem_raw);
+    const long long items = (rows + 2 * NSQ - 1) / (2 * NSQ);
+    const Sched sc = xcd_sched(items);
+    for (long long it = sc.begin; it < sc.end; it += sc.stride) {
+        const long long r0 = it * 2 * NSQ;
+        const long long left = rows - r0;
+        const int nr = (int)(left < 2 * NSQ ? left : 2 * NSQ);
+        auto ext = [&](int row, int k) -> cpx<R> {
+            if (row >= nr) return {(R)0, (R)0};
+            const cpx<R> *h = H + (r0 + row) * nc;
+            if (k == 0) return {h[0].x, (R)0};
+            if (2 * k < N) return h[k];
+            if (2 * k == N) return {h[k].x, (R)0};
+            const cpx<R> c = h[N - k];
+            return {c.x, -c.y};
+        };
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < NSQ * N; idx += 256) {
+            const int s = idx / N, k = idx - s * N;
+            const cpx<R> A = ext(2 * s, k), B = ext(2 * s + 1, k);
+            buf[s * BS + pidx(k)] = {A.x - B.y, A.y + B.x};
+        }
+        __syncthreads();
+        p2_fft<R, N>(buf, BS, twl, +1);
+        T *dst = field + r0 * N;
+        for (int idx = threadIdx.x; idx < 2 * NSQ * N; idx += 256) {
+            const int row = idx / N, e = idx - row * N;
+            if (row >= nr) continue;
+            const cpx<R> z = buf[(row >> 1) * BS + pidx(e)];
+            const R v = (row & 1) ? z.y : z.x;
+            dst[idx] = (T)((double)v * norm + shift);
+        }
+    }
+}
+
+// Radial binning of the fused last pass.  D = 2: the columns are j of
+// [N][nc]; D = 3: the columns are (i1, j) of [N][N][nc] (axis 0 transformed).
+struct BinArgs {
+    int D, nBins;
+    double dk, pref, Pnorm;
+    double *binpart;   // [gridDim.x][nBins] power partials
+};
+
+template <typename R, int N, bool BIN>
+__global__ void __launch_bounds__(256)
+k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict__ tw, int sign, BinArgs ba)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int CB = kSeq<N>, BS = kColStride<R, N>, nc = N / 2 + 1;
+    const cpx<R> *twl = stage_twiddles<R, N>(smem_raw, tw);
+    cpx<R> *buf = data_area<R, N>(smem_raw);
+    double *hist = reinterpret_cast<double *>(buf + CB * BS);
+    if constexpr (BIN)
+        for (int b = threadIdx.x; b < ba.nBins; b += 256) hist[b] = 0.0;
     const long long tiles = (inner + CB - 1) / CB;
-    const long long o = blockIdx.x / tiles;
-    const long long c0 = (blockIdx.x - o * tiles) * CB;
-    const int ncol = (int)((inner - c0) < CB ? (inner - c0) : CB);
-    cpx<R> *base = H + o * (long long)N * inner + c0;
-    for (int idx = threadIdx.x; idx < N * CB; idx += blockDim.x) {
-        const int e = idx / CB, cc = idx - e * CB;
-        buf[cc * bs + e] = (cc < ncol) ? base[(long long)e * inner + cc] : cpx<R>{(R)0, (R)0};
+    const Sched sc = xcd_sched(outer * tiles);
+    for (long long it = sc.begin; it < sc.end; it += sc.stride) {
+        const long long o = it / tiles;
+        const long long c0 = (it - o * tiles) * CB;
+        const int ncol = (int)((inner - c0) < CB ? (inner - c0) : CB);
+        cpx<R> *base = H + o * (long long)N * inner + c0;
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < N * CB; idx += 256) {
+            const int e = idx / CB, cc = idx - e * CB;
+            buf[cc * BS + pidx(e)] = (cc < ncol) ? base[(long long)e * inner + cc] : cpx<R>{(R)0, (R)0};
+        }
+        __syncthreads();
+        p2_fft<R, N>(buf, BS, twl, sign);
+        if constexpr (!BIN) {
+            for (int idx = threadIdx.x; idx < N * CB; idx += 256) {
+                const int e = idx / CB, cc = idx - e * CB;
+                if (cc < ncol) base[(long long)e * inner + cc] = buf[cc * BS + pidx(e)];
+            }
+        } else {
+            // runs of up to 8 adjacent columns per thread: along j the bin
+            // index never decreases, so a run accumulates in a register and
+            // adds to the LDS histogram when its bin changes
+            constexpr int CL = CB < 8 ? CB : 8;
+            for (int q = threadIdx.x; q < N * (CB / CL); q += 256) {
+                const int e = q & (N - 1);
+                const int cc0 = (q / N) * CL;
+                const int se = (e <= N / 2) ? e : e - N;
+                const double n2e = (double)se * (double)se;
+                const long long c = c0 + cc0;
+                int j = (int)(ba.D == 3 ? c % nc : c);
+                int i1 = (int)(ba.D == 3 ? c / nc : 0);
+                int cur = -1;
+                double acc = 0.0;
+                for (int cc = cc0; cc < cc0 + CL && cc < ncol; ++cc) {
+                    double n2rows = n2e;
+                    if (ba.D == 3) {
+                        const int s1 = (i1 <= N / 2) ? i1 : i1 - N;
+                        n2rows = n2e + (double)s1 * (double)s1;
+                    }
+                    const double n = sqrt(n2rows + (double)j * j);
+                    const int bin = (int)n;
+                    if (bin < ba.nBins) {
+                        const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
+                        const double kc = n * ba.dk;
+                        const cpx<R> f = buf[cc * BS + pidx(e)];
+                        const double fx = (double)f.x, fy = (double)f.y;
+                        const double p = w * ba.pref * ipow_rn(kc, ba.D) * ba.Pnorm * (fx * fx + fy * fy);
+                        if (bin == cur) {
+                            acc += p;
+                        } else {
+                            if (cur >= 0) atomicAdd(&hist[cur], acc);
+                            cur = bin;
+                            acc = p;
+                        }
+                    }
+                    if (++j == nc) {
+                        j = 0;
+                        ++i1;
+                    }
+                }
+                if (cur >= 0) atomicAdd(&hist[cur], acc);
+            }
+        }
     }
-    __syncthreads();
-    p2_fft<R, N>(buf, CB, bs, twl, sign);
-    for (int idx = threadIdx.x; idx < N * CB; idx += blockDim.x) {
-        const int e = idx / CB, cc = idx - e * CB;
-        if (cc < ncol) base[(long long)e * inner + cc] = buf[cc * bs + e];
+    if constexpr (BIN) {
+        __syncthreads();
+        double *dst = ba.binpart + (long long)blockIdx.x * ba.nBins;
+        for (int b = threadIdx.x; b < ba.nBins; b += 256) dst[b] = hist[b];
     }
 }
 
-// sequences per block for an LDS budget (table of N entries + data)
-int seqs_for(int N, size_t csz, int pad)
+int current_cus()
 {
-    const size_t avail = kLdsBudget - (size_t)N * csz;
-    int n = (int)(avail / ((size_t)(N + pad) * csz));
-    if (n > 16) n = 16;
-    return n;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    return cus;
+}
+
+// Blocks of a persistent launch: as many as are resident at once (at most
+// `cap`), a multiple of 8 (one share per XCD), at most `items`.
+template <typename K>
+unsigned persistent_grid(K kernel, size_t shm, long long items, long long cap)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, shm) != hipSuccess || per_cu < 1) per_cu = 1;
+    long long g = (long long)per_cu * current_cus();
+    if (g > cap) g = cap;
+    if (g > items) g = items;
+    if (g >= 8) g -= g % 8;
+    return (unsigned)(g < 1 ? 1 : g);
 }
 
 template <typename R, int N>
-void rows_r2c_n(const void *field, void *H, long long rows, const void *tw, hipStream_t st)
+constexpr size_t rows_shm() { return twiddle_bytes<R, N>() + (size_t)kSeq<N> * kRowStride<N> * sizeof(cpx<R>); }
+template <typename R, int N>
+constexpr size_t cols_shm() { return twiddle_bytes<R, N>() + (size_t)kSeq<N> * kColStride<R, N> * sizeof(cpx<R>); }
+
+constexpr long long kNoCap = 1LL << 40;
+
+template <typename R, int N>
+hipError_t rows_r2c_n(const void *field, void *H, long long rows, const void *tw, hipStream_t st)
 {
-    const int nseq = seqs_for(N, sizeof(cpx<R>), 0);
-    const size_t shm = (size_t)(N + nseq * N) * sizeof(cpx<R>);
-    const long long blocks = (rows + 2 * nseq - 1) / (2 * nseq);
-    k_p2_rows_r2c<R, R, N><<<(unsigned)blocks, 256, shm, st>>>((const R *)field, (cpx<R> *)H, rows, nseq,
-                                                              (const cpx<R> *)tw);
+    const size_t shm = rows_shm<R, N>();
+    const long long items = (rows + 2 * kSeq<N> - 1) / (2 * kSeq<N>);
+    const unsigned grid = persistent_grid(k_p2_rows_r2c<R, R, N>, shm, items, kNoCap);
+    k_p2_rows_r2c<R, R, N><<<grid, 256, shm, st>>>((const R *)field, (cpx<R> *)H, rows, (const cpx<R> *)tw);
+    return hipGetLastError();
 }
 
 template <typename T, int N>
-void rows_c2r_n(const void *H, void *field, long long rows, const void *tw, double norm, double shift,
-                hipStream_t st)
+hipError_t rows_c2r_n(const void *H, void *field, long long rows, const void *tw, double norm, double shift,
+                      hipStream_t st)
 {
-    const int nseq = seqs_for(N, sizeof(cpx<double>), 0);
-    const size_t shm = (size_t)(N + nseq * N) * sizeof(cpx<double>);
-    const long long blocks = (rows + 2 * nseq - 1) / (2 * nseq);
-    k_p2_rows_c2r<T, double, N><<<(unsigned)blocks, 256, shm, st>>>((const cpx<double> *)H, (T *)field, rows, nseq,
-                                                                    (const cpx<double> *)tw, norm, shift);
+    const size_t shm = rows_shm<double, N>();
+    const long long items = (rows + 2 * kSeq<N> - 1) / (2 * kSeq<N>);
+    const unsigned grid = persistent_grid(k_p2_rows_c2r<T, double, N>, shm, items, kNoCap);
+    k_p2_rows_c2r<T, double, N><<<grid, 256, shm, st>>>((const cpx<double> *)H, (T *)field, rows,
+                                                        (const cpx<double> *)tw, norm, shift);
+    return hipGetLastError();
 }
 
 template <typename R, int N>
-void cols_n(void *H, long long outer, long long inner, const void *tw, int sign, hipStream_t st)
+hipError_t cols_n(void *H, long long outer, long long inner, const void *tw, int sign, hipStream_t st)
 {
-    const int CB = seqs_for(N, sizeof(cpx<R>), 1);
-    const size_t shm = (size_t)(N + CB * (N + 1)) * sizeof(cpx<R>);
-    const long long tiles = (inner + CB - 1) / CB;
-    k_p2_cols<R, N><<<(unsigned)(outer * tiles), 256, shm, st>>>((cpx<R> *)H, inner, CB, (const cpx<R> *)tw, sign);
+    const size_t shm = cols_shm<R, N>();
+    const long long items = outer * ((inner + kSeq<N> - 1) / kSeq<N>);
+    const unsigned grid = persistent_grid(k_p2_cols<R, N, false>, shm, items, kNoCap);
+    k_p2_cols<R, N, false><<<grid, 256, shm, st>>>((cpx<R> *)H, outer, inner, (const cpx<R> *)tw, sign, BinArgs{});
+    return hipGetLastError();
 }
 
-#define CB_P2_SWITCH(N_, CALL)                  \
-    switch (N_) {                               \
-    case 8: { constexpr int NN = 8; CALL; break; }       \
-    case 16: { constexpr int NN = 16; CALL; break; }     \
-    case 32: { constexpr int NN = 32; CALL; break; }     \
-    case 64: { constexpr int NN = 64; CALL; break; }     \
-    case 128: { constexpr int NN = 128; CALL; break; }   \
-    case 256: { constexpr int NN = 256; CALL; break; }   \
-    case 512: { constexpr int NN = 512; CALL; break; }   \
-    case 1024: { constexpr int NN = 1024; CALL; break; } \
-    case 2048: { constexpr int NN = 2048; CALL; break; } \
-    default: return hipErrorInvalidValue;       \
+template <typename R, int N>
+hipError_t cols_bin_n(const void *H, long long inner, const void *tw, BinArgs ba, int cap_blocks, int *nblk,
+                      hipStream_t st)
+{
+    const size_t shm = cols_shm<R, N>() + sizeof(double) * (size_t)ba.nBins;
+    if (shm > kLdsBudget) return hipErrorInvalidValue;
+    const long long items = (inner + kSeq<N> - 1) / kSeq<N>;
+    const unsigned grid = persistent_grid(k_p2_cols<R, N, true>, shm, items, cap_blocks);
+    *nblk = (int)grid;
+    k_p2_cols<R, N, true><<<grid, 256, shm, st>>>((cpx<R> *)H, 1, inner, (const cpx<R> *)tw, -1, ba);
+    return hipGetLastError();
+}
+
+#define CB_P2_SWITCH(N_, CALL)                    \
+    switch (N_) {                                 \
+    case 8: { constexpr int NN = 8; CALL; }       \
+    case 16: { constexpr int NN = 16; CALL; }     \
+    case 32: { constexpr int NN = 32; CALL; }     \
+    case 64: { constexpr int NN = 64; CALL; }     \
+    case 128: { constexpr int NN = 128; CALL; }   \
+    case 256: { constexpr int NN = 256; CALL; }   \
+    case 512: { constexpr int NN = 512; CALL; }   \
+    case 1024: { constexpr int NN = 1024; CALL; } \
+    case 2048: { constexpr int NN = 2048; CALL; } \
+    default: return hipErrorInvalidValue;         \
     }
 
 } // namespace
 
-bool p2_supported(int N, size_t csz)
+bool p2_supported(int N)
 {
-    return N >= 8 && N <= 2048 && (N & (N - 1)) == 0 && seqs_for(N, csz, 1) >= 1;
+    return N >= 8 && N <= 2048 && (N & (N - 1)) == 0;
 }
+
+int p2_bin_blocks_max() { return 8 * current_cus(); }
 
 hipError_t p2_rows_r2c(int fp32, const void *field, void *H, long long rows, int N, const void *tw, hipStream_t st)
 {
-    if (fp32) { CB_P2_SWITCH(N, (rows_r2c_n<float, NN>(field, H, rows, tw, st))) }
-    else { CB_P2_SWITCH(N, (rows_r2c_n<double, NN>(field, H, rows, tw, st))) }
-    return hipGetLastError();
+    if (fp32) { CB_P2_SWITCH(N, return (rows_r2c_n<float, NN>(field, H, rows, tw, st))) }
+    CB_P2_SWITCH(N, return (rows_r2c_n<double, NN>(field, H, rows, tw, st)))
 }
 
 hipError_t p2_cols(int fp32, void *H, long long outer, long long inner, int N, const void *tw, int sign,
                    hipStream_t st)
 {
-    if (fp32) { CB_P2_SWITCH(N, (cols_n<float, NN>(H, outer, inner, tw, sign, st))) }
-    else { CB_P2_SWITCH(N, (cols_n<double, NN>(H, outer, inner, tw, sign, st))) }
-    return hipGetLastError();
+    if (fp32) { CB_P2_SWITCH(N, return (cols_n<float, NN>(H, outer, inner, tw, sign, st))) }
+    CB_P2_SWITCH(N, return (cols_n<double, NN>(H, outer, inner, tw, sign, st)))
+}
+
+hipError_t p2_cols_bin(int fp32, const void *H, long long inner, int N, const void *tw, int D, int nBins, double dk,
+                       double pref, double Pnorm, double *binpart, int cap_blocks, int *nblk, hipStream_t st)
+{
+    const BinArgs ba{D, nBins, dk, pref, Pnorm, binpart};
+    if (fp32) { CB_P2_SWITCH(N, return (cols_bin_n<float, NN>(H, inner, tw, ba, cap_blocks, nblk, st))) }
+    CB_P2_SWITCH(N, return (cols_bin_n<double, NN>(H, inner, tw, ba, cap_blocks, nblk, st)))
 }
 
 hipError_t p2_rows_c2r(int field_fp32, const void *H, void *field, long long rows, int N, const void *tw,
                        double norm, double shift, hipStream_t st)
 {
-    if (field_fp32) { CB_P2_SWITCH(N, (rows_c2r_n<float, NN>(H, field, rows, tw, norm, shift, st))) }
-    else { CB_P2_SWITCH(N, (rows_c2r_n<double, NN>(H, field, rows, tw, norm, shift, st))) }
-    return hipGetLastError();
+    if (field_fp32) { CB_P2_SWITCH(N, return (rows_c2r_n<float, NN>(H, field, rows, tw, norm, shift, st))) }
+    CB_P2_SWITCH(N, return (rows_c2r_n<double, NN>(H, field, rows, tw, norm, shift, st)))
 }
 
 } // namespace cb

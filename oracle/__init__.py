@@ -69,6 +69,9 @@ def lib():
         L.cbo_get_scalars.argtypes = [ctypes.c_void_p, dp]
         L.cbo_set_scalars.argtypes = [ctypes.c_void_p, dp]
         L.cbo_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.cbo_ctor.argtypes = [ctypes.c_void_p]
+        L.cbo_seed_sites.argtypes = [ctypes.POINTER(Params), dp, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_longlong), ctypes.c_int, dp]
         L.cbo_forces.argtypes = [ctypes.c_void_p, dp]
         L.cbo_V.restype = ctypes.c_double
         L.cbo_V.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
@@ -153,6 +156,11 @@ class Sim:
         s = np.ascontiguousarray(s, dtype=np.float64)
         lib().cbo_set_scalars(self._h, _dp(s))
 
+    def ctor(self):
+        """Run the ctor's sequence (src/bubble.cpp:239-291) on the current
+        phi and raw t = 0 pi (e.g. fields copied in from elsewhere)."""
+        lib().cbo_ctor(self._h)
+
     def forces(self):
         out = np.zeros(self.n)
         lib().cbo_forces(self._h, _dp(out))
@@ -169,6 +177,18 @@ class Sim:
         k, kp, d = np.zeros(cap), np.zeros(cap), np.zeros(cap)
         nb = lib().cbo_spectrum(self._h, _dp(k), _dp(kp), _dp(d), cap)
         return k[:nb].copy(), kp[:nb].copy(), d[:nb].copy()
+
+
+def seed_sites(xns, idx, **kw):
+    """phi after seeding the centres xns (n x 3, in order) at the flat
+    lattice indices idx, without building the lattice (src/bubble.cpp:420-453)."""
+    p = default_params(**kw)
+    xa = np.ascontiguousarray(np.asarray(xns, dtype=np.float64).reshape(-1, 3))
+    ia = np.ascontiguousarray(np.asarray(idx, dtype=np.int64))
+    out = np.zeros(ia.shape[0])
+    lib().cbo_seed_sites(ctypes.byref(p), _dp(xa), xa.shape[0],
+                         ia.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), ia.shape[0], _dp(out))
+    return out
 
 
 def mt64(seed, n):

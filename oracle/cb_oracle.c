@@ -8,8 +8,12 @@
  * Each function follows the reference file:line given in its comment, in the
  * reference's evaluation order.  OpenMP loops use a static split with
  * per-thread partial sums combined in thread order, so a fixed thread count
- * gives a deterministic result (the reference combines its reduction
- * partials in a nondeterministic order, src/bubble.cpp:776).
+ * gives a deterministic result.  The box sums (energies, averages, spreads)
+ * are Neumaier-compensated: the reference combines its OpenMP reduction
+ * partials in an unspecified order (src/bubble.cpp:776), so over 1e8 sites
+ * its own sums scatter by ~sqrt(n) ulp relative (~1e-12 at 512^3); the
+ * oracle evaluates each sum of the same per-site terms to within a few ulp
+ * of the exact sum, the value every summation order approximates.
  */
 #define _GNU_SOURCE
 #include "cb_oracle.h"
@@ -283,20 +287,26 @@ static void fft_rec(const twtab *tw, int n, int sign, const ld *xr, const ld *xi
         }
 }
 
-/* one line of n elements at stride st, transformed in place; work holds
- * 6n + 2n long doubles (copy, output, scratch, twiddle table) */
-static void fft_line_ld(int n, int sign, ld *re, ld *im, long long st, ld *work)
+/* table of cos / sin (2 pi e / n), e < n, in long double: tab[0..n) cos,
+ * tab[n..2n) sin */
+static void make_twtab(int n, ld *tab, twtab *tw)
 {
-    ld *xr = work, *xi = work + n, *yr = work + 2 * n, *yi = work + 3 * n;
-    ld *tr = work + 4 * n, *ti = work + 5 * n, *tc = work + 6 * n, *ts = work + 7 * n;
     const ld twopi = 2.0L * 3.141592653589793238462643383279502884L;
     for (int e = 0; e < n; ++e) {
         ld ang = twopi * (ld)e / (ld)n;
-        tc[e] = cosl(ang); ts[e] = sinl(ang);
-        xr[e] = re[e * st]; xi[e] = im[e * st];
+        tab[e] = cosl(ang); tab[n + e] = sinl(ang);
     }
-    twtab tw = {n, tc, ts};
-    fft_rec(&tw, n, sign, xr, xi, 1, yr, yi, tr, ti);
+    tw->n = n; tw->c = tab; tw->s = tab + n;
+}
+
+/* one line of n elements at stride st, transformed in place with the
+ * table tw of length n; work holds 6n long doubles (copy, output, scratch) */
+static void fft_line_ld(int n, int sign, ld *re, ld *im, long long st, ld *work, const twtab *tw)
+{
+    ld *xr = work, *xi = work + n, *yr = work + 2 * n, *yi = work + 3 * n;
+    ld *tr = work + 4 * n, *ti = work + 5 * n;
+    for (int e = 0; e < n; ++e) { xr[e] = re[e * st]; xi[e] = im[e * st]; }
+    fft_rec(tw, n, sign, xr, xi, 1, yr, yi, tr, ti);
     for (int e = 0; e < n; ++e) { re[e * st] = yr[e]; im[e * st] = yi[e]; }
 }
 
@@ -304,11 +314,13 @@ void cbo_dft_ld(int n, int sign, const double *in_re, const double *in_im,
                 double *out_re, double *out_im)
 {
     ld *re = malloc(sizeof(ld) * n), *im = malloc(sizeof(ld) * n);
-    ld *work = malloc(sizeof(ld) * 8 * n);
+    ld *work = malloc(sizeof(ld) * 6 * n), *tab = malloc(sizeof(ld) * 2 * n);
+    twtab tw;
+    make_twtab(n, tab, &tw);
     for (int e = 0; e < n; ++e) { re[e] = in_re[e]; im[e] = in_im ? in_im[e] : 0; }
-    fft_line_ld(n, sign, re, im, 1, work);
+    fft_line_ld(n, sign, re, im, 1, work, &tw);
     for (int e = 0; e < n; ++e) { out_re[e] = (double)re[e]; out_im[e] = (double)im[e]; }
-    free(re); free(im); free(work);
+    free(re); free(im); free(work); free(tab);
 }
 
 /* all-axes transform of a D-dim complex array of side N (row-major, last
@@ -318,22 +330,26 @@ static void fft_axes_ld(int D, int N, ld *re, ld *im, int first_axis, int last_a
 {
     long long total = 1;
     for (int d = 0; d < D; ++d) total *= N;
+    ld *tab = malloc(sizeof(ld) * 2 * N);
+    twtab tw;
+    make_twtab(N, tab, &tw);
     for (int ax = first_axis; ax < last_axis; ++ax) {
         long long st = 1;
         for (int d = ax + 1; d < D; ++d) st *= N;
         long long lines = total / N;
 #pragma omp parallel num_threads(nthreads)
         {
-            ld *work = malloc(sizeof(ld) * 8 * N);
+            ld *work = malloc(sizeof(ld) * 6 * N);
 #pragma omp for schedule(static)
             for (long long l = 0; l < lines; ++l) {
                 long long outer = l / st, inner = l % st;
                 long long base = outer * st * N + inner;
-                fft_line_ld(N, sign, re + base, im + base, st, work);
+                fft_line_ld(N, sign, re + base, im + base, st, work, &tw);
             }
             free(work);
         }
     }
+    free(tab);
 }
 
 /* r2c of a real D-dim field (FFTW_FORWARD, unnormalised), half-complex
@@ -366,36 +382,48 @@ static void c2r_ld(int D, int N, const double *hre, const double *him, double *f
     long long rows = total / N;
     /* leading axes on the half array: treat as (D-1)-dim batch of nc columns */
     ld *re = malloc(sizeof(ld) * rows * nc), *im = malloc(sizeof(ld) * rows * nc);
+    ld *tab = malloc(sizeof(ld) * 2 * N);
+    twtab tw;
+    make_twtab(N, tab, &tw);
     for (long long i = 0; i < rows * nc; ++i) { re[i] = hre[i]; im[i] = him[i]; }
     if (D >= 2) {
         for (int ax = 0; ax < D - 1; ++ax) {
             long long st = nc;
             for (int d = ax + 1; d < D - 1; ++d) st *= N;
             long long lines = rows * nc / N;
-            ld *work = malloc(sizeof(ld) * 8 * N);
-            for (long long l = 0; l < lines; ++l) {
-                long long outer = l / st, inner = l % st;
-                long long base = outer * st * N + inner;
-                fft_line_ld(N, +1, re + base, im + base, st, work);
+#pragma omp parallel
+            {
+                ld *work = malloc(sizeof(ld) * 6 * N);
+#pragma omp for schedule(static)
+                for (long long l = 0; l < lines; ++l) {
+                    long long outer = l / st, inner = l % st;
+                    long long base = outer * st * N + inner;
+                    fft_line_ld(N, +1, re + base, im + base, st, work, &tw);
+                }
+                free(work);
             }
-            free(work);
         }
     }
     /* last axis: Hermitian-extended complex line, real part of the inverse */
-    ld *lr = malloc(sizeof(ld) * N), *li = malloc(sizeof(ld) * N);
-    ld *work = malloc(sizeof(ld) * 8 * N);
-    for (long long r = 0; r < rows; ++r) {
-        ld *xr = re + r * nc, *xi = im + r * nc;
-        lr[0] = xr[0]; li[0] = 0;
-        for (int j = 1; j < N; ++j) {
-            if (2 * j < N)      { lr[j] = xr[j];     li[j] = xi[j]; }
-            else if (2 * j == N) { lr[j] = xr[j];     li[j] = 0; }
-            else                 { lr[j] = xr[N - j]; li[j] = -xi[N - j]; }
+#pragma omp parallel
+    {
+        ld *lr = malloc(sizeof(ld) * N), *li = malloc(sizeof(ld) * N);
+        ld *work = malloc(sizeof(ld) * 6 * N);
+#pragma omp for schedule(static)
+        for (long long r = 0; r < rows; ++r) {
+            ld *xr = re + r * nc, *xi = im + r * nc;
+            lr[0] = xr[0]; li[0] = 0;
+            for (int j = 1; j < N; ++j) {
+                if (2 * j < N)      { lr[j] = xr[j];     li[j] = xi[j]; }
+                else if (2 * j == N) { lr[j] = xr[j];     li[j] = 0; }
+                else                 { lr[j] = xr[N - j]; li[j] = -xi[N - j]; }
+            }
+            fft_line_ld(N, +1, lr, li, 1, work, &tw);
+            for (int k = 0; k < N; ++k) field[r * N + k] = (double)lr[k];
         }
-        fft_line_ld(N, +1, lr, li, 1, work);
-        for (int k = 0; k < N; ++k) field[r * N + k] = (double)lr[k];
+        free(lr); free(li); free(work);
     }
-    free(lr); free(li); free(work); free(re); free(im);
+    free(tab); free(re); free(im);
 }
 
 /* ------------------------------------------------------------------ */
@@ -562,6 +590,18 @@ static double addot(const cbo_sim *s)                        /* src/bubble.cpp:7
     return a * a * a * (2 * s->Epottot - s->Ekintot + s->Egradtot) / (3.0 * Mpl * Mpl);
 }
 
+/* Neumaier-compensated running sum */
+typedef struct { double s, c; } ksum;
+static inline void ks_add(ksum *k, double x)
+{
+    double t = k->s + x;
+    if (fabs(k->s) >= fabs(x)) k->c += (k->s - t) + x;
+    else k->c += (x - t) + k->s;
+    k->s = t;
+}
+static inline void ks_merge(ksum *k, ksum o) { ks_add(k, o.s); ks_add(k, o.c); }
+static inline double ks_val(ksum k) { return k.s + k.c; }
+
 enum { E_KIN, E_POT, E_GRAD, E_PHI, E_PHIP, V_KIN, V_POT, V_GRAD, V_TOT, V_PHI, V_PHIP, E_NQ };
 
 static void measure_energies(cbo_sim *s)                     /* src/bubble.cpp:761-844 */
@@ -570,10 +610,10 @@ static void measure_energies(cbo_sim *s)                     /* src/bubble.cpp:7
     const double *phi = s->phi, *pi = s->pi;
     const double a = s->a, dx = s->dx, dxLD = s->dxLD, c = s->p.c, DeltaV = s->DeltaV;
     const double a_kin = a + 0.5 * s->p.dt * s->ap;          /* :765 */
-    double *part = calloc((size_t)nth * E_NQ, sizeof(double));
+    ksum *part = calloc((size_t)nth * E_NQ, sizeof(ksum));
 #pragma omp parallel num_threads(nth)
     {
-        double acc[E_NQ] = {0};
+        ksum acc[E_NQ] = {{0, 0}};
         int tid = omp_get_thread_num();
 #pragma omp for schedule(static)
         for (long long i = 0; i < s->N; ++i) {
@@ -589,23 +629,26 @@ static void measure_energies(cbo_sim *s)                     /* src/bubble.cpp:7
             double Eg = 0.5 * (gx2 + gy2 + gz2);
             double Ep = cbo_V(phi[ig], c, DeltaV);
             double Ek = 0.5 * pow(a_kin, -2 * D) * pi[ig] * pi[ig];
-            acc[E_KIN] += Ek * dxLD;
-            acc[E_POT] += Ep * dxLD;
-            acc[E_GRAD] += Eg * dxLD;
-            acc[E_PHI] += phi[ig] * dxLD;
-            acc[E_PHIP] += pi[ig] * pow(a_kin, -D) * dxLD;
-            acc[V_KIN] += Ek * dxLD * Ek * dxLD;
-            acc[V_POT] += Ep * dxLD * Ep * dxLD;
-            acc[V_GRAD] += Eg * dxLD * Eg * dxLD;
-            acc[V_TOT] += (Ek + Eg + Ep) * dxLD * (Ek + Eg + Ep) * dxLD;
-            acc[V_PHI] += phi[ig] * dxLD * phi[ig] * dxLD;
-            acc[V_PHIP] += pi[ig] * pow(a_kin, -D) * dxLD * pi[ig] * pow(a_kin, -D) * dxLD;
+            ks_add(&acc[E_KIN], Ek * dxLD);
+            ks_add(&acc[E_POT], Ep * dxLD);
+            ks_add(&acc[E_GRAD], Eg * dxLD);
+            ks_add(&acc[E_PHI], phi[ig] * dxLD);
+            ks_add(&acc[E_PHIP], pi[ig] * pow(a_kin, -D) * dxLD);
+            ks_add(&acc[V_KIN], Ek * dxLD * Ek * dxLD);
+            ks_add(&acc[V_POT], Ep * dxLD * Ep * dxLD);
+            ks_add(&acc[V_GRAD], Eg * dxLD * Eg * dxLD);
+            ks_add(&acc[V_TOT], (Ek + Eg + Ep) * dxLD * (Ek + Eg + Ep) * dxLD);
+            ks_add(&acc[V_PHI], phi[ig] * dxLD * phi[ig] * dxLD);
+            ks_add(&acc[V_PHIP], pi[ig] * pow(a_kin, -D) * dxLD * pi[ig] * pow(a_kin, -D) * dxLD);
         }
         memcpy(part + (size_t)tid * E_NQ, acc, sizeof acc);
     }
-    double tot[E_NQ] = {0};
-    for (int t = 0; t < nth; ++t)
-        for (int q = 0; q < E_NQ; ++q) tot[q] += part[(size_t)t * E_NQ + q];
+    double tot[E_NQ];
+    for (int q = 0; q < E_NQ; ++q) {
+        ksum k = {0, 0};
+        for (int t = 0; t < nth; ++t) ks_merge(&k, part[(size_t)t * E_NQ + q]);
+        tot[q] = ks_val(k);
+    }
     free(part);
     s->Epottot = tot[E_POT]; s->Egradtot = tot[E_GRAD];
     s->avphi = tot[E_PHI]; s->avphiP = tot[E_PHIP];
@@ -665,6 +708,8 @@ static void seed_centres(cbo_sim *s, const double *xyz, int n)   /* :420-453 */
         }
     }
 }
+
+static void ctor_sequence(cbo_sim *s);
 
 cbo_sim *cbo_create(const cbo_params *p, const double *xns, int n_xns, int nthreads)
 {
@@ -744,15 +789,26 @@ cbo_sim *cbo_create(const cbo_params *p, const double *xns, int n_xns, int nthre
         cbo_thermal_field(D, g, L, Temp, mF, 0, p->thermal_base, s->phi);
         cbo_thermal_field(D, g, L, Temp, mF, 1, p->thermal_base, s->pi);
     }
+    ctor_sequence(s);
+    return s;
+}
 
+/* The ctor's sequence after the fields are set (src/bubble.cpp:239-291):
+ * initial energies, H0, a', the pi back-step and EkinHalfPrev. */
+static void ctor_sequence(cbo_sim *s)
+{
+    const cbo_params *p = &s->p;
+    const int D = p->D, g = p->nGrids;
+    const double c = p->c;
+    s->a = 1.; s->ap = 0.; s->H = 0.; s->C = 0.; s->t = 0.;
     /* initial energies, :239-269 */
     {
         const int nth = s->nthreads;
-        double *part = calloc((size_t)nth * 5, sizeof(double));
+        ksum *part = calloc((size_t)nth * 5, sizeof(ksum));
         const double dx = s->dx, dxLD = s->dxLD;
 #pragma omp parallel num_threads(nth)
         {
-            double acc[5] = {0};
+            ksum acc[5] = {{0, 0}};
             int tid = omp_get_thread_num();
 #pragma omp for schedule(static)
             for (long long i = 0; i < s->N; ++i) {
@@ -770,17 +826,20 @@ cbo_sim *cbo_create(const cbo_params *p, const double *xns, int n_xns, int nthre
                 double egrad = 0.5 * tmp;
                 double epot = cbo_V(s->phi[ig], c, s->DeltaV);
                 double ekin = 0.5 * s->pi[ig] * s->pi[ig];
-                acc[0] += s->phi[ig] * dxLD;
-                acc[1] += s->pi[ig] * dxLD;
-                acc[2] += epot * dxLD;
-                acc[3] += egrad * dxLD;
-                acc[4] += ekin * dxLD;
+                ks_add(&acc[0], s->phi[ig] * dxLD);
+                ks_add(&acc[1], s->pi[ig] * dxLD);
+                ks_add(&acc[2], epot * dxLD);
+                ks_add(&acc[3], egrad * dxLD);
+                ks_add(&acc[4], ekin * dxLD);
             }
             memcpy(part + (size_t)tid * 5, acc, sizeof acc);
         }
-        double tot[5] = {0};
-        for (int t = 0; t < nth; ++t)
-            for (int q = 0; q < 5; ++q) tot[q] += part[(size_t)t * 5 + q];
+        double tot[5];
+        for (int q = 0; q < 5; ++q) {
+            ksum k = {0, 0};
+            for (int t = 0; t < nth; ++t) ks_merge(&k, part[(size_t)t * 5 + q]);
+            tot[q] = ks_val(k);
+        }
         free(part);
         s->avphi = tot[0]; s->avphiP = tot[1]; s->Epottot = tot[2];
         s->Egradtot = tot[3]; s->Ekintot = tot[4];
@@ -795,26 +854,49 @@ cbo_sim *cbo_create(const cbo_params *p, const double *xns, int n_xns, int nthre
     double a_mhalf = s->a - 0.5 * p->dt * s->ap;
     {
         const int nth = s->nthreads;
-        double *part = calloc((size_t)nth, sizeof(double));
+        ksum *part = calloc((size_t)nth, sizeof(ksum));
         const double dt = p->dt, dxLD = s->dxLD;
 #pragma omp parallel num_threads(nth)
         {
-            double acc = 0;
+            ksum acc = {0, 0};
             int tid = omp_get_thread_num();
 #pragma omp for schedule(static)
             for (long long i = 0; i < s->N; ++i) {
                 s->pi[i] -= 0.5 * dt * s->dpi[i];
-                acc += 0.5 * pow(a_mhalf, -2 * D) * s->pi[i] * s->pi[i] * dxLD;
+                ks_add(&acc, 0.5 * pow(a_mhalf, -2 * D) * s->pi[i] * s->pi[i] * dxLD);
             }
             part[tid] = acc;
         }
-        double EkinHalf = 0;
-        for (int t = 0; t < nth; ++t) EkinHalf += part[t];
+        ksum k = {0, 0};
+        for (int t = 0; t < nth; ++t) ks_merge(&k, part[t]);
+        double EkinHalf = ks_val(k);
         free(part);
         s->EkinHalfPrev = EkinHalf;
     }
     s->H = H0;
-    return s;
+}
+
+void cbo_ctor(cbo_sim *s) { ctor_sequence(s); }
+
+/* phi after seeding the centres xyz[0..n) in order (src/bubble.cpp:420-453)
+ * at the flat lattice indices idx[0..m), without building the lattice. */
+void cbo_seed_sites(const cbo_params *p, const double *xyz, int n, const long long *idx, int m, double *out)
+{
+    const int D = p->D, g = p->nGrids;
+    const double L = p->L;
+    const double phiC = phizero(p->c), Rn = cradius(D, p->c), lw = clw(p->c);
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < m; ++q) {
+        double x[3]; xfunc(D, g, L, idx[q], x);
+        double phi = 0.0;
+        for (int b = 0; b < n; ++b) {
+            const double *xn = xyz + 3 * b;
+            double d0 = x[0] - xn[0], d1 = x[1] - xn[1], d2 = x[2] - xn[2];
+            if (D == 2) phi = dmin(phi + thick_bubble(d0 * d0 + d1 * d1, phiC, Rn, lw), phiC);
+            else if (D == 3) phi = dmin(phi + thick_bubble(d0 * d0 + d1 * d1 + d2 * d2, phiC, Rn, lw), phiC);
+        }
+        out[q] = phi;
+    }
 }
 
 cbo_sim *cbo_create_blank(const cbo_params *p, int nthreads)

@@ -52,6 +52,13 @@ cbo_sim *cbo_create(const cbo_params *p, const double *xns, int n_xns, int nthre
 /* A lattice with the ctor's derived constants and zero fields, a = 1, no
  * initial condition (for timing the stepper from a state set by the caller). */
 cbo_sim *cbo_create_blank(const cbo_params *p, int nthreads);
+/* Run the ctor's sequence after seeding (initial energies, H0, a', pi
+ * back-step, EkinHalfPrev; src/bubble.cpp:239-291) on the current phi and
+ * the raw t = 0 pi of s. */
+void cbo_ctor(cbo_sim *s);
+/* phi after seeding the n centres xyz (modes 0 / 1) at the flat indices
+ * idx[0..m), without building the lattice. */
+void cbo_seed_sites(const cbo_params *p, const double *xyz, int n, const long long *idx, int m, double *out);
 void cbo_destroy(cbo_sim *s);
 void cbo_step(cbo_sim *s, int nsteps);
 void cbo_energy_row(const cbo_sim *s, double row[16]);

@@ -192,6 +192,35 @@ def test_set_state_roundtrip_and_stepping_from_it():
     assert row_errors(o.energy_row()[None], g.energy_row()[None]).max() <= TOL
 
 
+@pytest.mark.parametrize("D,N", [(2, 8), (2, 16), (3, 16), (3, 64), (3, 128), (2, 1024), (2, 2048)])
+def test_power_of_two_sizes_thermal_fields_and_spectra(D, N):
+    """Power-of-two lattices take the compile-time radix-8/4/2 transforms
+    (thermal c2r, r2c, column passes, the last one fused with the binning):
+    FP64 fields and spectrum vs the oracle, FP32 spectrum within 2e-3."""
+    dx = 0.0625 if D == 2 else 0.125
+    kw = dict(D=D, nGrids=N, L=N * dx, dt=0.2 * dx, c=2.0, readxns=-3, Trad=0.2)
+    print("case", kw, flush=True)
+    o, g, ro, rg = run_pair(kw, [3])
+    assert row_errors(ro, rg).max() <= TOL
+    phi_g, _ = g.fields()
+    assert field_err(o.phi(), phi_g) <= TOL
+    ko, _, do = o.spectrum()
+    kg, _, dg = g.spectrum()
+    assert len(kg) == len(ko)
+    np.testing.assert_allclose(kg, ko, rtol=1e-13, atol=0)
+    floor = 1e-13 * np.max(do)
+    assert np.all(np.abs(dg - do) <= TOL * np.abs(do) + floor), np.max(np.abs(dg - do) / (np.abs(do) + floor))
+    g32 = cb.Bubble(precision="fp32", **kw)
+    g32.step(3)
+    k32, _, d32 = g32.spectrum()
+    np.testing.assert_allclose(k32, ko, rtol=1e-13)
+    big = do > 1e-6 * np.max(do)
+    np.testing.assert_allclose(d32[big], do[big], rtol=2e-3)
+    # a second spectrum of the same state gives the same bins (to rounding)
+    _, _, dg2 = g.spectrum()
+    np.testing.assert_allclose(dg2, dg, rtol=1e-13, atol=1e-13 * np.max(dg))
+
+
 @pytest.mark.parametrize("N", [8192, 65536])
 def test_long_1d_spectrum_matches_oracle(N):
     """1D rows longer than one in-LDS transform go through the four-step

@@ -392,3 +392,43 @@ def test_thermal_mode_variance_is_the_bose_einstein_spectrum():
                 ratio += abs(X[r, j]) ** 2 / P
                 count += 1
         assert abs(ratio / count - 1.0) < 0.05, ratio / count
+
+
+def test_box_sums_are_exact_to_a_few_ulp_whatever_the_thread_split():
+    """The oracle's box sums are compensated: the ctor's <phi> (sum of
+    phi dxLD over the lattice) equals the exactly rounded sum (math.fsum) of
+    the same per-site products to 2 ulp, with 1 or 7 threads."""
+    kw = dict(D=3, nGrids=48, L=6.0, c=2.0, readxns=-3, Trad=0.3, dt=0.025)
+    dxLD = (1.0 / 48) ** 3
+    vals = []
+    for nth in (1, 7):
+        o = oracle.Sim(nthreads=nth, **kw)
+        exact = math.fsum((o.phi().ravel() * dxLD).tolist())
+        got = o.energy_row()[10]
+        assert abs(got - exact) <= 2 * np.spacing(abs(exact)), (nth, got, exact)
+        vals.append(got)
+    assert abs(vals[0] - vals[1]) <= 2 * np.spacing(abs(vals[0]))
+
+
+@pytest.mark.parametrize("kw", [
+    dict(D=2, nGrids=48, L=6.0, c=4.0, readxns=1, nBubbles=5, dt=0.025),
+    dict(D=3, nGrids=24, L=4.0, c=4.0, readxns=1, nBubbles=4, dt=0.025),
+    dict(D=3, nGrids=20, L=2.5, c=2.0, readxns=1, nBubbles=3, dt=0.025, expansion=1, Mpl=10.0),
+])
+def test_seed_sites_and_ctor_pieces_reproduce_the_whole_ctor(kw):
+    """The pieces the full-size parity tests use -- seeding evaluated at
+    chosen sites, and the ctor sequence run on fields copied in -- give the
+    whole ctor's phi, pi, energy row and scalars bit for bit."""
+    o = oracle.Sim(**kw)
+    phi0 = o.phi().copy()
+    sites = oracle.seed_sites(o.xns(), np.arange(o.n), **kw)
+    np.testing.assert_array_equal(sites.reshape(o.shape()), phi0)
+    b = oracle.Sim(blank=True, **kw)
+    b.phi()[...] = phi0                 # mode 1 starts at rest: raw pi = 0
+    b.ctor()
+    np.testing.assert_array_equal(b.pi(), o.pi())
+    np.testing.assert_array_equal(b.energy_row(), o.energy_row())
+    np.testing.assert_array_equal(b.scalars(), o.scalars())
+    o.step(3)
+    b.step(3)
+    np.testing.assert_array_equal(b.phi(), o.phi())
