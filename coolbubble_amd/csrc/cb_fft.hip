@@ -514,7 +514,7 @@ static hipError_t forward_r2c(const Stages &st, const cpx<R> *tw, const T *field
     const long long rows = (D == 3) ? (long long)N * N : (D == 2 ? N : 1);
     if (p2_supported(N)) {
         const int f32 = sizeof(R) == 4;
-        hipError_t e = p2_rows_r2c(f32, field, H, rows, N, tw, s);
+        hipError_t e = p2_rows_r2c(f32, field, H, rows, nc, N, tw, s);
         if (e == hipSuccess && D == 3) e = p2_cols(f32, H, N, nc, N, tw, -1, s);        // axis 1
         if (e == hipSuccess && D >= 2) e = p2_cols(f32, H, 1, rows / N * nc, N, tw, -1, s); // axis 0
         return e;
@@ -616,17 +616,20 @@ static cb_status plan_for(cb_sim *s)
         if (e == hipSuccess)
             e = s->fp32 ? make_twiddles<float>(p->N2, &p->tw2, s->st) : make_twiddles<double>(p->N2, &p->tw2, s->st);
     }
-    if (e == hipSuccess) e = hipMalloc(&p->H, csz * (p->N1 ? (size_t)N : (size_t)p->rows * nc));
+    p->fused = D >= 2 && !p->N1 && p2_supported(N);
+    // the fused path pads rows to p2_row_stride(N); the padding stays zero
+    const size_t ldh = p->fused ? (size_t)p2_row_stride(N) : (size_t)nc;
+    if (e == hipSuccess) e = hipMalloc(&p->H, csz * (p->N1 ? (size_t)N : (size_t)p->rows * ldh));
+    if (e == hipSuccess && p->fused) e = hipMemsetAsync(p->H, 0, csz * (size_t)p->rows * ldh, s->st);
     p->nBins = s->nkmax + 1;
     long long modes = p->rows * nc;
     p->binblocks = (int)((modes + 4095) / 4096);
     if (p->binblocks > 256) p->binblocks = 256;
     if (p->binblocks < 1) p->binblocks = 1;
-    p->fused = D >= 2 && !p->N1 && p2_supported(N);
     size_t part = 3 * (size_t)p->nBins * p->binblocks;
     if (p->fused) {
         p->fused_blocks = p2_bin_blocks_max();
-        part = std::max(part, (size_t)p->nBins * p->fused_blocks);
+        part = std::max(part, (size_t)p->nBins * 64);   // the fused pass's partial rows
     }
     if (e == hipSuccess) e = hipMalloc((void **)&p->binpart, sizeof(double) * part);
     if (e == hipSuccess) e = hipMalloc((void **)&p->bins, sizeof(double) * 3 * (size_t)p->nBins);
@@ -672,11 +675,11 @@ cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int c
     std::vector<double> h(3 * (size_t)nB);
     if (p->fused) {
         // rows r2c, (3D) axis 1, then axis 0 fused with the binning
-        const int f32 = s->fp32 ? 1 : 0, nc = N / 2 + 1;
-        HIP_TRY(p2_rows_r2c(f32, field, p->H, p->rows, N, p->tw, s->st));
-        if (D == 3) HIP_TRY(p2_cols(f32, p->H, N, nc, N, p->tw, -1, s->st));
+        const int f32 = s->fp32 ? 1 : 0, ncp = p2_row_stride(N);
+        HIP_TRY(p2_rows_r2c(f32, field, p->H, p->rows, ncp, N, p->tw, s->st));
+        if (D == 3) HIP_TRY(p2_cols(f32, p->H, N, ncp, N, p->tw, -1, s->st));
         int nblk = 0;
-        HIP_TRY(p2_cols_bin(f32, p->H, p->rows / N * nc, N, p->tw, D, nB, dk, pref, Pnorm, p->binpart,
+        HIP_TRY(p2_cols_bin(f32, p->H, p->rows / N * ncp, N, p->tw, D, nB, dk, pref, Pnorm, p->binpart,
                             p->fused_blocks, &nblk, s->st));
         k_bin_reduce<<<(nB + 63) / 64, 256, 0, s->st>>>(p->binpart, nblk, nB, p->bins);
         HIP_TRY(hipGetLastError());

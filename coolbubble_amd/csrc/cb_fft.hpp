@@ -30,14 +30,19 @@ __device__ inline cpx<R> cmul(cpx<R> a, cpx<R> b)
 // (cb_fft_p2.hip).  tw = exp(-2 pi i e / N), e < N, in the transform's
 // precision.  Each returns hipErrorInvalidValue for an unsupported N.
 bool p2_supported(int N);
-// real rows (T = R) -> half-complex rows, forward
-hipError_t p2_rows_r2c(int fp32, const void *field, void *H, long long rows, int N, const void *tw, hipStream_t st);
+// row length of the padded half-complex array the fused spectrum uses
+// (N/2 + 1 rounded up to a multiple of 8 elements)
+int p2_row_stride(int N);
+// real rows (T = R) -> half-complex rows of stride ldh (>= N/2 + 1), forward
+hipError_t p2_rows_r2c(int fp32, const void *field, void *H, long long rows, long long ldh, int N, const void *tw,
+                       hipStream_t st);
 // in-place transform along the middle axis of [outer][N][inner]
 hipError_t p2_cols(int fp32, void *H, long long outer, long long inner, int N, const void *tw, int sign,
                    hipStream_t st);
 // last forward pass of a D = 2 / 3 spectrum: transform axis 0 of [N][inner]
-// (inner = nc, or N nc) and bin the modes instead of storing them; writes
-// *nblk (<= cap_blocks) rows of nBins power partials to binpart.
+// (inner = ncp, or N ncp, with ncp = p2_row_stride(N)) and bin the modes
+// instead of storing them; writes *nblk rows of nBins power partials to
+// binpart (which holds at least 64 rows).
 hipError_t p2_cols_bin(int fp32, const void *H, long long inner, int N, const void *tw, int D, int nBins, double dk,
                        double pref, double Pnorm, double *binpart, int cap_blocks, int *nblk, hipStream_t st);
 // largest block count p2_cols_bin uses

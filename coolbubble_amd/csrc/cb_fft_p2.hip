@@ -42,6 +42,10 @@ template <int N>
 constexpr int kRowStride = pidx(N) + 1;
 template <typename R, int N>
 constexpr int kColStride = pidx(N) + (sizeof(R) == 4 ? (N >= 256 ? 4 : 1) : (N >= 512 ? 2 : 1));
+// padded row length of a half-complex array: a multiple of 8 elements, so
+// the row segments the column passes load start on 64-byte boundaries
+template <int N>
+constexpr int kNcp = (N / 2 + 1 + 7) / 8 * 8;
 // twiddle table staged in LDS when it is small, else read through the caches
 template <typename R, int N>
 constexpr bool kTwLds = (size_t)N * 2 * sizeof(R) <= 8192;
@@ -153,14 +157,17 @@ __device__ inline void p2_stages(cpx<R> *seq, int t, const cpx<R> *twl, int sign
     }
 }
 
-// Transform the kSeq<N> sequences at buf + s * bs; all 256 threads call it.
-template <typename R, int N>
+// Transform RND x kSeq<N> sequences at buf + s * bs, kSeq<N> at a time; all
+// 256 threads call it.
+template <typename R, int N, int RND = 1>
 __device__ inline void p2_fft(cpx<R> *buf, int bs, const cpx<R> *twl, int sign)
 {
     constexpr int TPS = N / 8;
     const int g = threadIdx.x / TPS, t = threadIdx.x - g * TPS;
     const bool active = g < kSeq<N>;
-    p2_stages<R, N, 0, 1>(buf + (size_t)(active ? g : 0) * bs, t, twl, sign, active);
+#pragma unroll 1
+    for (int r = 0; r < RND; ++r)
+        p2_stages<R, N, 0, 1>(buf + (size_t)((active ? g : 0) + r * kSeq<N>) * bs, t, twl, sign, active);
 }
 
 // LDS layout: [twiddles (when staged)] [data] [bins]
@@ -207,7 +214,8 @@ struct Vec16 {
 
 template <typename T, typename R, int N>
 __global__ void __launch_bounds__(256)
-k_p2_rows_r2c(const T *__restrict__ field, cpx<R> *__restrict__ H, long long rows, const cpx<R> *__restrict__ tw)
+k_p2_rows_r2c(const T *__restrict__ field, cpx<R> *__restrict__ H, long long rows, long long ldh,
+              const cpx<R> *__restrict__ tw)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int nc = N / 2 + 1, NSQ = kSeq<N>, BS = kRowStride<N>;
@@ -246,8 +254,8 @@ k_p2_rows_r2c(const T *__restrict__ field, cpx<R> *__restrict__ H, long long row
             const cpx<R> A = {(R)0.5 * (z.x + zn.x), (R)0.5 * (z.y - zn.y)};
             const cpx<R> B = {(R)0.5 * (z.y + zn.y), (R)0.5 * (zn.x - z.x)};
             const int ra = 2 * s, rb = 2 * s + 1;
-            if (ra < nr) H[(r0 + ra) * nc + k] = A;
-            if (rb < nr) H[(r0 + rb) * nc + k] = B;
+            if (ra < nr) H[(r0 + ra) * ldh + k] = A;
+            if (rb < nr) H[(r0 + rb) * ldh + k] = B;
         }
     }
 }
@@ -296,19 +304,28 @@ k_p2_rows_c2r(const cpx<R> *__restrict__ H, T *__restrict__ field, long long row
 }
 
 // Radial binning of the fused last pass.  D = 2: the columns are j of
-// [N][nc]; D = 3: the columns are (i1, j) of [N][N][nc] (axis 0 transformed).
+// [N][ncp]; D = 3: the columns are (i1, j) of [N][N][ncp] (axis 0
+// transformed), rows padded to ncp = kNcp<N> >= nc; columns j >= nc are
+// padding and skipped.
+constexpr int kBinRows = 64;   // partial histograms, block b adds into row b % kBinRows
 struct BinArgs {
     int D, nBins;
     double dk, pref, Pnorm;
-    double *binpart;   // [gridDim.x][nBins] power partials
+    double *binpart;   // [kBinRows][nBins] power partials, zeroed before the launch
 };
+
+// Columns per block: RND rounds of kSeq<N> sequences (about 32 KiB of data
+// per block keeps 4-5 blocks resident per CU with 64-byte row segments).
+template <typename R, int N, bool BIN>
+constexpr int kColRounds = (!BIN && sizeof(R) == 4) ? 2 : 1;
 
 template <typename R, int N, bool BIN>
 __global__ void __launch_bounds__(256)
 k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict__ tw, int sign, BinArgs ba)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int CB = kSeq<N>, BS = kColStride<R, N>, nc = N / 2 + 1;
+    constexpr int RND = kColRounds<R, N, BIN>;
+    constexpr int CB = kSeq<N> * RND, BS = kColStride<R, N>, nc = N / 2 + 1;
     const cpx<R> *twl = stage_twiddles<R, N>(smem_raw, tw);
     cpx<R> *buf = data_area<R, N>(smem_raw);
     double *hist = reinterpret_cast<double *>(buf + CB * BS);
@@ -327,52 +344,57 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
             buf[cc * BS + pidx(e)] = (cc < ncol) ? base[(long long)e * inner + cc] : cpx<R>{(R)0, (R)0};
         }
         __syncthreads();
-        p2_fft<R, N>(buf, BS, twl, sign);
+        p2_fft<R, N, RND>(buf, BS, twl, sign);
         if constexpr (!BIN) {
             for (int idx = threadIdx.x; idx < N * CB; idx += 256) {
                 const int e = idx / CB, cc = idx - e * CB;
                 if (cc < ncol) base[(long long)e * inner + cc] = buf[cc * BS + pidx(e)];
             }
         } else {
-            // runs of up to 8 adjacent columns per thread: along j the bin
-            // index never decreases, so a run accumulates in a register and
-            // adds to the LDS histogram when its bin changes
+            // runs of up to 8 adjacent columns per thread: the terms of a run
+            // are independent (computed first), then along j the bin index
+            // never decreases, so the run accumulates in a register and adds
+            // to the LDS histogram when its bin changes.  kc^D is a product
+            // of D factors (within 2 ulp of the reference's pow(kc, D)).
             constexpr int CL = CB < 8 ? CB : 8;
             for (int q = threadIdx.x; q < N * (CB / CL); q += 256) {
                 const int e = q & (N - 1);
                 const int cc0 = (q / N) * CL;
                 const int se = (e <= N / 2) ? e : e - N;
-                const double n2e = (double)se * (double)se;
-                const long long c = c0 + cc0;
-                int j = (int)(ba.D == 3 ? c % nc : c);
-                int i1 = (int)(ba.D == 3 ? c / nc : 0);
-                int cur = -1;
-                double acc = 0.0;
-                for (int cc = cc0; cc < cc0 + CL && cc < ncol; ++cc) {
-                    double n2rows = n2e;
-                    if (ba.D == 3) {
-                        const int s1 = (i1 <= N / 2) ? i1 : i1 - N;
-                        n2rows = n2e + (double)s1 * (double)s1;
-                    }
+                const int c = (int)c0 + cc0;
+                int j = ba.D == 3 ? c % kNcp<N> : c;
+                int i1 = ba.D == 3 ? c / kNcp<N> : 0;
+                double pv[CL];
+                int bv[CL];
+#pragma unroll
+                for (int u = 0; u < CL; ++u) {
+                    const int s1 = (ba.D == 3) ? ((i1 <= N / 2) ? i1 : i1 - N) : 0;
+                    const double n2rows = (double)se * (double)se + (double)s1 * (double)s1;
                     const double n = sqrt(n2rows + (double)j * j);
                     const int bin = (int)n;
-                    if (bin < ba.nBins) {
-                        const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
-                        const double kc = n * ba.dk;
-                        const cpx<R> f = buf[cc * BS + pidx(e)];
-                        const double fx = (double)f.x, fy = (double)f.y;
-                        const double p = w * ba.pref * ipow_rn(kc, ba.D) * ba.Pnorm * (fx * fx + fy * fy);
-                        if (bin == cur) {
-                            acc += p;
-                        } else {
-                            if (cur >= 0) atomicAdd(&hist[cur], acc);
-                            cur = bin;
-                            acc = p;
-                        }
-                    }
-                    if (++j == nc) {
+                    const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
+                    const double kc = n * ba.dk;
+                    const double kd = (ba.D == 3) ? kc * kc * kc : kc * kc;
+                    const cpx<R> f = buf[(cc0 + u) * BS + pidx(e)];
+                    const double fx = (double)f.x, fy = (double)f.y;
+                    pv[u] = w * ba.pref * kd * ba.Pnorm * (fx * fx + fy * fy);
+                    bv[u] = (cc0 + u < ncol && j < nc && bin < ba.nBins) ? bin : -1;
+                    if (++j == kNcp<N>) {
                         j = 0;
                         ++i1;
+                    }
+                }
+                int cur = -1;
+                double acc = 0.0;
+#pragma unroll
+                for (int u = 0; u < CL; ++u) {
+                    if (bv[u] < 0) continue;
+                    if (bv[u] == cur) {
+                        acc += pv[u];
+                    } else {
+                        if (cur >= 0) atomicAdd(&hist[cur], acc);
+                        cur = bv[u];
+                        acc = pv[u];
                     }
                 }
                 if (cur >= 0) atomicAdd(&hist[cur], acc);
@@ -381,9 +403,18 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
     }
     if constexpr (BIN) {
         __syncthreads();
-        double *dst = ba.binpart + (long long)blockIdx.x * ba.nBins;
-        for (int b = threadIdx.x; b < ba.nBins; b += 256) dst[b] = hist[b];
+        double *dst = ba.binpart + (long long)(blockIdx.x % kBinRows) * ba.nBins;
+        for (int b = threadIdx.x; b < ba.nBins; b += 256) {
+            const double v = hist[b];
+            if (v != 0.0) atomicAdd(&dst[b], v);
+        }
     }
+}
+
+__global__ void __launch_bounds__(256) k_zero(double *p, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.0;
 }
 
 int current_cus()
@@ -411,18 +442,23 @@ unsigned persistent_grid(K kernel, size_t shm, long long items, long long cap)
 
 template <typename R, int N>
 constexpr size_t rows_shm() { return twiddle_bytes<R, N>() + (size_t)kSeq<N> * kRowStride<N> * sizeof(cpx<R>); }
-template <typename R, int N>
-constexpr size_t cols_shm() { return twiddle_bytes<R, N>() + (size_t)kSeq<N> * kColStride<R, N> * sizeof(cpx<R>); }
+template <typename R, int N, bool BIN>
+constexpr int kCols = kSeq<N> * kColRounds<R, N, BIN>;
+template <typename R, int N, bool BIN>
+constexpr size_t cols_shm()
+{
+    return twiddle_bytes<R, N>() + (size_t)kCols<R, N, BIN> * kColStride<R, N> * sizeof(cpx<R>);
+}
 
 constexpr long long kNoCap = 1LL << 40;
 
 template <typename R, int N>
-hipError_t rows_r2c_n(const void *field, void *H, long long rows, const void *tw, hipStream_t st)
+hipError_t rows_r2c_n(const void *field, void *H, long long rows, long long ldh, const void *tw, hipStream_t st)
 {
     const size_t shm = rows_shm<R, N>();
     const long long items = (rows + 2 * kSeq<N> - 1) / (2 * kSeq<N>);
     const unsigned grid = persistent_grid(k_p2_rows_r2c<R, R, N>, shm, items, kNoCap);
-    k_p2_rows_r2c<R, R, N><<<grid, 256, shm, st>>>((const R *)field, (cpx<R> *)H, rows, (const cpx<R> *)tw);
+    k_p2_rows_r2c<R, R, N><<<grid, 256, shm, st>>>((const R *)field, (cpx<R> *)H, rows, ldh, (const cpx<R> *)tw);
     return hipGetLastError();
 }
 
@@ -441,8 +477,9 @@ hipError_t rows_c2r_n(const void *H, void *field, long long rows, const void *tw
 template <typename R, int N>
 hipError_t cols_n(void *H, long long outer, long long inner, const void *tw, int sign, hipStream_t st)
 {
-    const size_t shm = cols_shm<R, N>();
-    const long long items = outer * ((inner + kSeq<N> - 1) / kSeq<N>);
+    constexpr int CB = kCols<R, N, false>;
+    const size_t shm = cols_shm<R, N, false>();
+    const long long items = outer * ((inner + CB - 1) / CB);
     const unsigned grid = persistent_grid(k_p2_cols<R, N, false>, shm, items, kNoCap);
     k_p2_cols<R, N, false><<<grid, 256, shm, st>>>((cpx<R> *)H, outer, inner, (const cpx<R> *)tw, sign, BinArgs{});
     return hipGetLastError();
@@ -452,11 +489,14 @@ template <typename R, int N>
 hipError_t cols_bin_n(const void *H, long long inner, const void *tw, BinArgs ba, int cap_blocks, int *nblk,
                       hipStream_t st)
 {
-    const size_t shm = cols_shm<R, N>() + sizeof(double) * (size_t)ba.nBins;
+    constexpr int CB = kCols<R, N, true>;
+    const size_t shm = cols_shm<R, N, true>() + sizeof(double) * (size_t)ba.nBins;
     if (shm > kLdsBudget) return hipErrorInvalidValue;
-    const long long items = (inner + kSeq<N> - 1) / kSeq<N>;
+    const long long items = (inner + CB - 1) / CB;
     const unsigned grid = persistent_grid(k_p2_cols<R, N, true>, shm, items, cap_blocks);
-    *nblk = (int)grid;
+    const int nz = kBinRows * ba.nBins;
+    k_zero<<<(nz + 255) / 256, 256, 0, st>>>(ba.binpart, nz);
+    *nblk = kBinRows;
     k_p2_cols<R, N, true><<<grid, 256, shm, st>>>((cpx<R> *)H, 1, inner, (const cpx<R> *)tw, -1, ba);
     return hipGetLastError();
 }
@@ -482,12 +522,15 @@ bool p2_supported(int N)
     return N >= 8 && N <= 2048 && (N & (N - 1)) == 0;
 }
 
+int p2_row_stride(int N) { return (N / 2 + 1 + 7) / 8 * 8; }
+
 int p2_bin_blocks_max() { return 8 * current_cus(); }
 
-hipError_t p2_rows_r2c(int fp32, const void *field, void *H, long long rows, int N, const void *tw, hipStream_t st)
+hipError_t p2_rows_r2c(int fp32, const void *field, void *H, long long rows, long long ldh, int N, const void *tw,
+                       hipStream_t st)
 {
-    if (fp32) { CB_P2_SWITCH(N, return (rows_r2c_n<float, NN>(field, H, rows, tw, st))) }
-    CB_P2_SWITCH(N, return (rows_r2c_n<double, NN>(field, H, rows, tw, st)))
+    if (fp32) { CB_P2_SWITCH(N, return (rows_r2c_n<float, NN>(field, H, rows, ldh, tw, st))) }
+    CB_P2_SWITCH(N, return (rows_r2c_n<double, NN>(field, H, rows, ldh, tw, st)))
 }
 
 hipError_t p2_cols(int fp32, void *H, long long outer, long long inner, int N, const void *tw, int sign,

@@ -79,7 +79,8 @@ long long footprint_of(const cb_params &p)
     const bool lagged = p.expansion && p.D == 3;
     const long long fields = sites * esz * (lagged ? 4 : 3);
     const long long rows = sites / N, nc = N / 2 + 1;
-    const long long fft = rows * nc * 2 * esz + N * 2 * esz + 256 * 3 * 2048 * 8;
+    const long long ncp = (nc + 7) / 8 * 8;   // padded rows of the fused power-of-two spectrum
+    const long long fft = rows * ncp * 2 * esz + N * 2 * esz + 256 * 3 * 2048 * 8;
     const long long partials = (sites / 64 + 8192) * 12 * 8;
     long long transient = 0;
     if (p.readxns == -3 && p.D >= 2) transient = rows * 312 * 8 + rows * nc * 16;
