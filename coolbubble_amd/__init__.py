@@ -399,7 +399,7 @@ class Server:
 
 
 def server_variant(sim):
-    """The tiled variant the server gives a request (256 threads, <= 4
+    """The tiled variant the server gives a request (256 threads, <= 2
     vectors per thread, prefetch 1, non-temporal stores; tallest tile, chunk
     closest to 32 planes), or -1: selecting it on a lone simulation
     reproduces the request's results bit for bit."""
@@ -407,7 +407,7 @@ def server_variant(sim):
     best, score = -1, -1e300
     for v in range(n):
         c = sim.variant_info(v)
-        if c["fuse"] == 2 or c["nthr"] != 256 or c["items"] > 4 or c["pf"] != 1 or c["nt"] != 1:
+        if c["fuse"] == 2 or c["nthr"] != 256 or c["items"] > 2 or c["pf"] != 1 or c["nt"] != 1:
             continue
         s = 1000.0 * c["TR"] - abs(np.log2(c["NI"] / 32.0))
         if s > score:

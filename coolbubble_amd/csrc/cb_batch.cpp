@@ -51,7 +51,7 @@ int pick_variant(const cb_sim *s, int items_max, int pf, int nt, int ni_pref)
     return best;
 }
 
-int batch_variant(const cb_sim *s) { return pick_variant(s, 4, 1, 1, 32); }
+int batch_variant(const cb_sim *s) { return pick_variant(s, kBatchItems, 1, 1, 32); }
 
 // Table entry of a member stepping with its selected tiled variant.
 void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size_t *shm)
@@ -72,6 +72,7 @@ void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size
     m.par_phi = s->iphi;
     m.par_pi = s->ipi;
     m.nsteps = 1 << 30;
+    m.energy_it = -1;
     const size_t b = tiled_lds_bytes(s->k, fp32, c);
     if (shm && b > *shm) *shm = b;
 }
@@ -106,7 +107,7 @@ struct cb_batch {
     // buffer parities of the general-kernel members when the graph was
     // captured (their pointers are baked into the graph)
     std::vector<int> cap_par;
-    int items = 4, pf = 1, nt = 1, ni = 32;   // batched launch shape (cb_batch_autotune)
+    int items = cb::kBatchItems, pf = 1, nt = 1, ni = 32;   // batched launch shape (cb_batch_autotune)
 };
 
 namespace {

@@ -249,6 +249,7 @@ cb_status tick(cb_server *sv)
     for (size_t t = 0; t < tiled.size(); ++t) {
         cb::fill_member(sv->h_tab[t], act[tiled[t]]->sim, tiles, sv->o.precision == CB_FP32, &shm);
         sv->h_tab[t].nsteps = m[tiled[t]];
+        sv->h_tab[t].energy_it = m[tiled[t]] - 1;   // energies at its own event only
         energy_at[(size_t)m[tiled[t]] - 1] = 1;
         tiles += sv->h_tab[t].ntiles;
     }
@@ -258,7 +259,7 @@ cb_status tick(cb_server *sv)
     const int grid = tiles < sv->cus * 8 ? tiles : sv->cus * 8;
     for (int it = 0; it < L && e == hipSuccess && !tiled.empty(); ++it) {
         const bool energy = energy_at[(size_t)it] != 0;
-        e = cb::launch_batch_step(sv->o.precision == CB_FP32, energy, 4, 1, sv->d_tab, (int)tiled.size(), tiles, grid,
+        e = cb::launch_batch_step(sv->o.precision == CB_FP32, energy, cb::kBatchItems, 1, sv->d_tab, (int)tiled.size(), tiles, grid,
                                   shm, it, sv->st);
         if (e == hipSuccess && energy) e = cb::launch_batch_finalize(sv->d_tab, (int)tiled.size(), it, sv->st);
     }

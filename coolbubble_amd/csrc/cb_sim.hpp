@@ -55,6 +55,10 @@ cb_status step_n(cb_sim *s, int nsteps);
 cb_status construct_sim(cb_sim *s, const double *xns, int n_xns);
 void release_sim(cb_sim *s);
 bool params_ok(const cb_params *p, std::string &why);
+// default batched launch shape: at most 2 vectors per thread, prefetch 1,
+// non-temporal stores, chunks near 32 planes; batch_variant is a member's
+// tile for it (the server's choice)
+constexpr int kBatchItems = 2;
 int batch_variant(const cb_sim *s);
 int pick_variant(const cb_sim *s, int items_max, int pf, int nt, int ni_pref);
 void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size_t *shm);

@@ -248,16 +248,25 @@ k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int ste
         T *phi_out = (T *)m.phi[pphi ^ 1];
         const T *pi_in = (const T *)m.pi[ppi];
         T *pi_out = m.lagged ? (T *)m.pi[ppi ^ 1] : (T *)m.pi[ppi];
+        // static members measure energies only at their own energy step
+        const bool me = ENERGY && (m.energy_it < 0 || m.energy_it == step_it);
         if (m.k.D == 3) {
             if (m.lagged)
                 tile_step<T, 3, VW, ITEMS, 256, true, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
                                                              m.partials, lds);
-            else
+            else if (me)
                 tile_step<T, 3, VW, ITEMS, 256, false, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
                                                               m.partials, lds);
+            else
+                tile_step<T, 3, VW, ITEMS, 256, false, false, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
+                                                             m.partials, lds);
         } else {
-            tile_step<T, 2, VW, ITEMS, 256, false, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
-                                                          m.partials, lds);
+            if (me)
+                tile_step<T, 2, VW, ITEMS, 256, false, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
+                                                              m.partials, lds);
+            else
+                tile_step<T, 2, VW, ITEMS, 256, false, false, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
+                                                             m.partials, lds);
         }
     }
 }
@@ -266,6 +275,7 @@ __global__ void __launch_bounds__(256) k_batch_finalize(const BatchMember *__res
 {
     const BatchMember &m = table[blockIdx.x];
     if (step_it >= m.nsteps) return;
+    if (!m.lagged && m.energy_it >= 0 && m.energy_it != step_it) return;
     finalize_member(m.k, m.partials, m.ntiles, m.sc, m.lagged ? 0 : 1);
 }
 

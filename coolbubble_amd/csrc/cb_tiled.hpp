@@ -57,7 +57,7 @@ struct BatchMember {
     int ntiles;
     int par_phi, par_pi;
     int nsteps;         // steps this member takes in the current launch sequence
-    int pad_;
+    int energy_it;      // launch index of this member's energy step; -1: every energy launch
     void *phi[2];
     void *pi[2];
     DevScalars *sc;
@@ -72,10 +72,12 @@ struct BatchMember {
 // items, pf or the non-temporal flag, so it steps bit-identically alone with
 // a variant of the same (TR, NI, threads).
 // step_it = index of this launch in the sequence: a member takes part while
-// step_it < its nsteps (its buffers flip with step_it & 1).
+// step_it < its nsteps (its buffers flip with step_it & 1).  In a launch with
+// `energy` set, a member measures energies if it expands, if its energy_it
+// is -1, or if its energy_it equals step_it; the others take the plain step.
 hipError_t launch_batch_step(int fp32, int energy, int items, int pf, const BatchMember *table, int nmem,
                              int total_tiles, int grid, size_t shm, int step_it, hipStream_t st);
-// Finalise the energies of the members taking part in launch step_it.
+// Finalise the energies of the members that measured them in launch step_it.
 hipError_t launch_batch_finalize(const BatchMember *table, int nmem, int step_it, hipStream_t st);
 size_t tiled_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
 TiledGeom tiled_geom(const SimConst &k, const TiledCfg &c);
