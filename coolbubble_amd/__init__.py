@@ -124,9 +124,10 @@ def lib():
                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]
         L.cb_server_drain.argtypes = [vp]
         L.cb_server_destroy.argtypes = [vp]
+        L.cb_server_variant.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         for name in ("cb_server_create", "cb_server_submit", "cb_server_cancel", "cb_server_poll",
                      "cb_server_rows", "cb_server_spectrum", "cb_server_release", "cb_server_stats",
-                     "cb_server_drain", "cb_server_destroy"):
+                     "cb_server_drain", "cb_server_destroy", "cb_server_variant"):
             getattr(L, name).restype = ctypes.c_int
         L.cb_nsites.argtypes = [vp]
         L.cb_nsites.restype = ctypes.c_longlong
@@ -399,17 +400,9 @@ class Server:
 
 
 def server_variant(sim):
-    """The tiled variant the server gives a request (256 threads, <= 2
-    vectors per thread, prefetch 1, non-temporal stores; tallest tile, chunk
-    closest to 32 planes), or -1: selecting it on a lone simulation
-    reproduces the request's results bit for bit."""
-    n, _ = sim.kernel_variants()
-    best, score = -1, -1e300
-    for v in range(n):
-        c = sim.variant_info(v)
-        if c["fuse"] == 2 or c["nthr"] != 256 or c["items"] > 2 or c["pf"] != 1 or c["nt"] != 1:
-            continue
-        s = 1000.0 * c["TR"] - abs(np.log2(c["NI"] / 32.0))
-        if s > score:
-            best, score = v, s
-    return best
+    """The kernel variant the server gives a request with sim's parameters
+    (cb_server_variant), or -1: selecting it on a lone simulation reproduces
+    the request's results bit for bit."""
+    v = ctypes.c_int()
+    check(lib().cb_server_variant(sim.handle, ctypes.byref(v)))
+    return v.value

@@ -9,6 +9,7 @@
 // once into a HIP graph and replayed; kSegment is even, so every member's
 // ping-pong buffers are back in place after a replay and the captured
 // kernel arguments stay valid.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -53,6 +54,36 @@ int pick_variant(const cb_sim *s, int items_max, int pf, int nt, int ni_pref)
 
 int batch_variant(const cb_sim *s) { return pick_variant(s, kBatchItems, 1, 1, 32); }
 
+// Two-step tile of a static member for the batched two-step launch: 256
+// threads, at most kBatchItems vectors per thread, at most 4 rows per tile
+// (the tallest of those), chunks nearest 16 planes (3D) or 8 (2D),
+// non-temporal stores preferred; its one-step companion must fit the
+// batched launches too.  -1 if there is none.
+int pick_variant2(const cb_sim *s)
+{
+    if (s->lagged || s->p.D < 2) return -1;
+    const double ni_pref = s->p.D == 3 ? 16.0 : 8.0;
+    int best = -1;
+    double best_score = -1e300;
+    for (size_t v = 0; v < s->variants.size(); ++v) {
+        const TiledCfg &c = s->variants[v];
+        if (c.fuse != 2 || c.nthr != 256 || c.items > kBatchItems || c.TR > 4) continue;
+        const int comp = companion_variant(s, c);
+        if (comp < 0) continue;
+        const TiledCfg &c1 = s->variants[(size_t)comp];
+        if (c1.nthr != 256 || c1.items > kBatchItems || c1.pf != 1) continue;
+        const double score = 1000.0 * c.TR - std::fabs(std::log2(c.NI / ni_pref)) + (c.nt ? 0.5 : 0.0);
+        if (score > best_score) { best_score = score; best = (int)v; }
+    }
+    return best;
+}
+
+int server_variant(const cb_sim *s)
+{
+    const int v2 = pick_variant2(s);
+    return v2 >= 0 ? v2 : batch_variant(s);
+}
+
 // Table entry of a member stepping with its selected tiled variant.
 void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size_t *shm)
 {
@@ -73,9 +104,20 @@ void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size
     m.par_pi = s->ipi;
     m.nsteps = 1 << 30;
     m.energy_it = -1;
-    const size_t b = tiled_lds_bytes(s->k, fp32, c);
+    size_t b = 0;
+    if (c.fuse == 2) {
+        const TiledCfg c1 = s->variants[(size_t)s->single_variant];
+        m.fused = 1;
+        m.g1 = tiled_geom(s->k, c1);
+        m.ntiles1 = tiled_blocks(s->k, c1);
+        b = std::max(step2_lds_bytes(s->k, fp32, c), tiled_lds_bytes(s->k, fp32, c1));
+    } else {
+        b = tiled_lds_bytes(s->k, fp32, c);
+    }
     if (shm && b > *shm) *shm = b;
 }
+
+int member_work(const BatchMember &m) { return m.fused ? std::max(m.ntiles, m.ntiles1) : m.ntiles; }
 
 // Host side of n steps run by batched launches: buffer parities, t, step.
 void advance_host(cb_sim *s, int n)
@@ -84,6 +126,17 @@ void advance_host(cb_sim *s, int n)
         s->iphi ^= 1;
         if (s->lagged) s->ipi ^= 1;
     }
+    for (int it = 0; it < n; ++it) s->t += s->p.dt;
+    s->step += n;
+}
+
+// n steps of a two-step member: n / 2 two-step launches (phi and pi flip
+// once each) and, for odd n, one one-step launch (phi flips).
+void advance_host2(cb_sim *s, int n)
+{
+    const int pairs = n / 2, odd = n & 1;
+    if ((pairs + odd) & 1) s->iphi ^= 1;
+    if (pairs & 1) s->ipi ^= 1;
     for (int it = 0; it < n; ++it) s->t += s->p.dt;
     s->step += n;
 }

@@ -20,6 +20,7 @@
 // boundaries fall (a static box never feeds energies back; an expanding one
 // computes them every step), so its results are bit-identical to a lone run
 // with the same kernel variant.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -77,7 +78,8 @@ long long footprint_of(const cb_params &p)
     for (int d = 0; d < p.D; ++d) sites *= N;
     const long long esz = p.precision == CB_FP32 ? 4 : 8;
     const bool lagged = p.expansion && p.D == 3;
-    const long long fields = sites * esz * (lagged ? 4 : 3);
+    // phi and pi double-buffered (expanding box, or a two-step variant)
+    const long long fields = sites * esz * (lagged || p.D >= 2 ? 4 : 3);
     const long long rows = sites / N, nc = N / 2 + 1;
     const long long ncp = (nc + 7) / 8 * 8;   // padded rows of the fused power-of-two spectrum
     const long long fft = rows * ncp * 2 * esz + N * 2 * esz + 256 * 3 * 2048 * 8;
@@ -194,7 +196,7 @@ void admit(cb_server *sv)
         }
         if (r != CB_OK) { finish(sv, q, r); continue; }
         q->sim = s;
-        cb::select_variant(s, cb::batch_variant(s));
+        if (cb::select_variant(s, cb::server_variant(s)) != CB_OK) cb::select_variant(s, cb::batch_variant(s));
         double row[16];
         read_row(s, row);
         q->rows.insert(q->rows.end(), row, row + 16);
@@ -227,15 +229,20 @@ cb_status tick(cb_server *sv)
         if (m[i] > L) L = m[i];
         if (q->sim->lagged) any_lagged = true;
     }
-    // batched table of the tiled requests
-    std::vector<size_t> tiled, general;
-    for (size_t i = 0; i < act.size(); ++i) (act[i]->sim->variant >= 0 ? tiled : general).push_back(i);
-    if (tiled.size() > sv->tab_cap) {
+    // batched tables: one-step tiled requests first, then two-step ones
+    std::vector<size_t> single, pair, general;
+    for (size_t i = 0; i < act.size(); ++i) {
+        const cb_sim *s = act[i]->sim;
+        if (s->variant < 0) general.push_back(i);
+        else (s->variants[(size_t)s->variant].fuse == 2 ? pair : single).push_back(i);
+    }
+    const size_t ntab = single.size() + pair.size();
+    if (ntab > sv->tab_cap) {
         if (sv->h_tab) hipHostFree(sv->h_tab);
         if (sv->d_tab) hipFree(sv->d_tab);
         sv->h_tab = nullptr;
         sv->d_tab = nullptr;
-        sv->tab_cap = tiled.size() * 2;
+        sv->tab_cap = ntab * 2;
         if (hipHostMalloc((void **)&sv->h_tab, sizeof(cb::BatchMember) * sv->tab_cap, hipHostMallocDefault) != hipSuccess ||
             hipMalloc((void **)&sv->d_tab, sizeof(cb::BatchMember) * sv->tab_cap) != hipSuccess) {
             sv->tab_cap = 0;
@@ -243,32 +250,57 @@ cb_status tick(cb_server *sv)
             return CB_OUT_OF_MEMORY;
         }
     }
-    int tiles = 0;
-    size_t shm = 0;
-    std::vector<char> energy_at((size_t)L, any_lagged ? 1 : 0);
-    for (size_t t = 0; t < tiled.size(); ++t) {
-        cb::fill_member(sv->h_tab[t], act[tiled[t]]->sim, tiles, sv->o.precision == CB_FP32, &shm);
-        sv->h_tab[t].nsteps = m[tiled[t]];
-        sv->h_tab[t].energy_it = m[tiled[t]] - 1;   // energies at its own event only
-        energy_at[(size_t)m[tiled[t]] - 1] = 1;
-        tiles += sv->h_tab[t].ntiles;
+    const int fp32 = sv->o.precision == CB_FP32;
+    // one-step table: L launches, energies where a member ends (every step if one expands)
+    int tiles1 = 0, L1 = 0;
+    size_t shm1 = 0;
+    std::vector<char> energy1((size_t)L, any_lagged ? 1 : 0);
+    for (size_t t = 0; t < single.size(); ++t) {
+        cb::fill_member(sv->h_tab[t], act[single[t]]->sim, tiles1, fp32, &shm1);
+        const int n = m[single[t]];
+        sv->h_tab[t].nsteps = n;
+        sv->h_tab[t].energy_it = n - 1;   // energies at its own event only
+        energy1[(size_t)n - 1] = 1;
+        tiles1 += cb::member_work(sv->h_tab[t]);
+        if (n > L1) L1 = n;
+    }
+    // two-step table: ceil(L2 / 2) launches, launch p covering steps 2p, 2p + 1
+    cb::BatchMember *tab2 = sv->h_tab + single.size();
+    int tiles2 = 0, L2 = 0;
+    size_t shm2 = 0;
+    std::vector<char> energy2((size_t)(L + 1) / 2, 0);
+    for (size_t t = 0; t < pair.size(); ++t) {
+        cb::fill_member(tab2[t], act[pair[t]]->sim, tiles2, fp32, &shm2);
+        const int n = m[pair[t]];
+        tab2[t].nsteps = n;
+        tab2[t].energy_it = n - 1;
+        energy2[(size_t)(n - 1) / 2] = 1;
+        tiles2 += cb::member_work(tab2[t]);
+        if (n > L2) L2 = n;
     }
     hipError_t e = hipSuccess;
-    if (!tiled.empty())
-        e = hipMemcpyAsync(sv->d_tab, sv->h_tab, sizeof(cb::BatchMember) * tiled.size(), hipMemcpyHostToDevice, sv->st);
-    const int grid = tiles < sv->cus * 8 ? tiles : sv->cus * 8;
-    for (int it = 0; it < L && e == hipSuccess && !tiled.empty(); ++it) {
-        const bool energy = energy_at[(size_t)it] != 0;
-        e = cb::launch_batch_step(sv->o.precision == CB_FP32, energy, cb::kBatchItems, 1, sv->d_tab, (int)tiled.size(), tiles, grid,
-                                  shm, it, sv->st);
-        if (e == hipSuccess && energy) e = cb::launch_batch_finalize(sv->d_tab, (int)tiled.size(), it, sv->st);
+    if (ntab) e = hipMemcpyAsync(sv->d_tab, sv->h_tab, sizeof(cb::BatchMember) * ntab, hipMemcpyHostToDevice, sv->st);
+    const int cap = sv->cus * 8;
+    for (int it = 0; it < L1 && e == hipSuccess; ++it) {
+        const bool energy = energy1[(size_t)it] != 0;
+        e = cb::launch_batch_step(fp32, energy, cb::kBatchItems, 1, sv->d_tab, (int)single.size(), tiles1,
+                                  std::min(tiles1, cap), shm1, it, sv->st);
+        if (e == hipSuccess && energy) e = cb::launch_batch_finalize(sv->d_tab, (int)single.size(), it, sv->st);
+    }
+    const cb::BatchMember *d_tab2 = sv->d_tab + single.size();
+    for (int p = 0; 2 * p < L2 && e == hipSuccess; ++p) {
+        const bool energy = energy2[(size_t)p] != 0;
+        e = cb::launch_batch_step2(fp32, energy, cb::kBatchItems, d_tab2, (int)pair.size(), tiles2,
+                                   std::min(tiles2, cap), shm2, p, sv->st);
+        if (e == hipSuccess && energy) e = cb::launch_batch_finalize2(d_tab2, (int)pair.size(), p, sv->st);
     }
     if (e != hipSuccess) { set_error(std::string("server tick: ") + hipGetErrorString(e)); return CB_HIP_ERROR; }
     for (size_t i : general) {
         cb_status r = cb::step_n(act[i]->sim, m[i]);
         if (r != CB_OK) return r;
     }
-    for (size_t i : tiled) cb::advance_host(act[i]->sim, m[i]);
+    for (size_t i : single) cb::advance_host(act[i]->sim, m[i]);
+    for (size_t i : pair) cb::advance_host2(act[i]->sim, m[i]);
     if (hipStreamSynchronize(sv->st) != hipSuccess) { set_error("server tick: stream failed"); return CB_HIP_ERROR; }
     // events
     std::vector<Req *> keep;
@@ -481,6 +513,13 @@ cb_status cb_server_drain(cb_server *sv)
     if (!sv) { set_error("invalid handle"); return CB_INVALID_ARG; }
     std::unique_lock<std::mutex> lk(sv->mu);
     sv->idle_cv.wait(lk, [&] { return sv->queue.empty() && sv->busy == 0; });
+    return CB_OK;
+}
+
+cb_status cb_server_variant(const cb_sim *s, int *variant)
+{
+    if (!s || !variant) { set_error("invalid argument"); return CB_INVALID_ARG; }
+    *variant = cb::server_variant(s);
     return CB_OK;
 }
 

@@ -61,8 +61,19 @@ bool params_ok(const cb_params *p, std::string &why);
 constexpr int kBatchItems = 2;
 int batch_variant(const cb_sim *s);
 int pick_variant(const cb_sim *s, int items_max, int pf, int nt, int ni_pref);
+// a static member's two-step tile for the batched two-step launch, or -1
+int pick_variant2(const cb_sim *s);
+// the variant the server gives a request: pick_variant2 when there is one,
+// else batch_variant
+int server_variant(const cb_sim *s);
+int companion_variant(const cb_sim *s, const TiledCfg &f);
+// table entry of a member with its selected tiled variant (one- or two-step)
 void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size_t *shm);
+// work items a member spans in a batched launch
+int member_work(const BatchMember &m);
+// host side of n steps run by batched launches (one-step / two-step members)
 void advance_host(cb_sim *s, int n);
+void advance_host2(cb_sim *s, int n);
 cb_status select_variant(cb_sim *s, int variant);
 cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int cap, int *nbins);
 // thermal initial condition of one field (cb_fft.hip)

@@ -13,6 +13,7 @@
 #include "cb_tiled.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 using cb::set_error;
@@ -101,6 +102,25 @@ cb_status step_n(cb_sim *s, int nsteps)
     return CB_OK;
 }
 
+// The one-step variant that takes the odd last step of a two-step variant f:
+// same threads per block, prefetch 1, at most 2 vectors per thread (so the
+// batched launches can run it too), the tallest tile, then the chunk length
+// nearest f's, then f's store kind; any one-step variant if none fits.
+int companion_variant(const cb_sim *s, const TiledCfg &f)
+{
+    int best = -1, first = -1;
+    double best_score = -1e300;
+    for (size_t v = 0; v < s->variants.size(); ++v) {
+        const TiledCfg &c = s->variants[v];
+        if (c.fuse == 2) continue;
+        if (first < 0) first = (int)v;
+        if (c.nthr != f.nthr || c.pf != 1 || c.items > 2) continue;
+        const double score = 1000.0 * c.TR - std::fabs(std::log2((double)c.NI / f.NI)) + (c.nt == f.nt ? 0.5 : 0.0);
+        if (score > best_score) { best_score = score; best = (int)v; }
+    }
+    return best >= 0 ? best : first;
+}
+
 cb_status select_variant(cb_sim *s, int variant)
 {
     if (variant < -1 || variant >= (int)s->variants.size()) {
@@ -114,9 +134,7 @@ cb_status select_variant(cb_sim *s, int variant)
                 return CB_OUT_OF_MEMORY;
             }
         }
-        if (s->single_variant < 0)
-            for (size_t v = 0; v < s->variants.size(); ++v)
-                if (s->variants[v].fuse != 2) { s->single_variant = (int)v; break; }
+        s->single_variant = companion_variant(s, s->variants[(size_t)variant]);
     }
     s->variant = variant;
     s->tiled = variant < 0 ? -1 : tiled_register(s->variants[(size_t)variant]);

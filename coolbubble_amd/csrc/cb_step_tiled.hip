@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "cb_device.hpp"
+#include "cb_tile2.hpp"
 #include "cb_tile_common.hpp"
 #include "cb_tiled.hpp"
 
@@ -312,6 +313,96 @@ hipError_t launch_batch_step(int fp32, int energy, int items, int pf, const Batc
 hipError_t launch_batch_finalize(const BatchMember *table, int nmem, int step_it, hipStream_t st)
 {
     if (nmem > 0) k_batch_finalize<<<nmem, 256, 0, st>>>(table, step_it);
+    return hipGetLastError();
+}
+
+// Batched two-step launch: every tile of every static two-step member.
+template <typename T, bool ENERGY, int ITEMS>
+__global__ void __launch_bounds__(256)
+k_batch_step2(const BatchMember *__restrict__ table, int nmem, int total, int pit)
+{
+    constexpr int VW = sizeof(T) == 4 ? 4 : 2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    const int G = gridDim.x, b = blockIdx.x;
+    const int lb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+    const int s0 = 2 * pit;
+    for (int w = lb; w < total; w += G) {
+        int lo = 0, hi = nmem - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (table[mid].tile_begin <= w) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchMember &m = table[lo];
+        if (s0 >= m.nsteps) continue;
+        const int lt = w - m.tile_begin;
+        const int pphi = (m.par_phi + pit) & 1;
+        const int ppi = (m.par_pi + pit) & 1;
+        const T *phi_in = (const T *)m.phi[pphi];
+        T *phi_out = (T *)m.phi[pphi ^ 1];
+        if (s0 + 1 < m.nsteps) {   // both steps: the two-step tile
+            if (lt >= m.ntiles) continue;
+            const bool me = ENERGY && (m.energy_it < 0 || m.energy_it == s0 + 1);
+            const T *pi_in = (const T *)m.pi[ppi];
+            T *pi_out = (T *)m.pi[ppi ^ 1];
+            if (m.k.D == 3) {
+                if (me) tile_step2<T, 3, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
+                else tile_step2<T, 3, ITEMS, 256, false>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
+            } else {
+                if (me) tile_step2<T, 2, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
+                else tile_step2<T, 2, ITEMS, 256, false>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
+            }
+        } else {                   // the odd last step: the one-step companion, pi in place
+            if (lt >= m.ntiles1) continue;
+            const bool me = ENERGY && (m.energy_it < 0 || m.energy_it == s0);
+            T *pi_io = (T *)m.pi[ppi];
+            if (m.k.D == 3) {
+                if (me) tile_step<T, 3, VW, ITEMS, 256, false, ENERGY, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
+                else tile_step<T, 3, VW, ITEMS, 256, false, false, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
+            } else {
+                if (me) tile_step<T, 2, VW, ITEMS, 256, false, ENERGY, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
+                else tile_step<T, 2, VW, ITEMS, 256, false, false, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_batch_finalize2(const BatchMember *__restrict__ table, int pit)
+{
+    const BatchMember &m = table[blockIdx.x];
+    const int s0 = 2 * pit;
+    if (s0 >= m.nsteps) return;
+    const bool two = s0 + 1 < m.nsteps;
+    if (m.energy_it >= 0 && m.energy_it != (two ? s0 + 1 : s0)) return;
+    finalize_member(m.k, m.partials, two ? m.ntiles : m.ntiles1, m.sc, 1);
+}
+
+template <typename T, bool ENERGY>
+static void batch2_launch_i(int items, const BatchMember *table, int nmem, int total, int grid, size_t shm, int pit,
+                            hipStream_t st)
+{
+    if (items == 1) k_batch_step2<T, ENERGY, 1><<<grid, 256, shm, st>>>(table, nmem, total, pit);
+    else k_batch_step2<T, ENERGY, 2><<<grid, 256, shm, st>>>(table, nmem, total, pit);
+}
+
+hipError_t launch_batch_step2(int fp32, int energy, int items, const BatchMember *table, int nmem, int total_tiles,
+                              int grid, size_t shm, int pit, hipStream_t st)
+{
+    if (items > 2) return hipErrorInvalidValue;
+    if (fp32) {
+        if (energy) batch2_launch_i<float, true>(items, table, nmem, total_tiles, grid, shm, pit, st);
+        else batch2_launch_i<float, false>(items, table, nmem, total_tiles, grid, shm, pit, st);
+    } else {
+        if (energy) batch2_launch_i<double, true>(items, table, nmem, total_tiles, grid, shm, pit, st);
+        else batch2_launch_i<double, false>(items, table, nmem, total_tiles, grid, shm, pit, st);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_finalize2(const BatchMember *table, int nmem, int pit, hipStream_t st)
+{
+    if (nmem > 0) k_batch_finalize2<<<nmem, 256, 0, st>>>(table, pit);
     return hipGetLastError();
 }
 

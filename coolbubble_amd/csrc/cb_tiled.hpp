@@ -57,7 +57,12 @@ struct BatchMember {
     int ntiles;
     int par_phi, par_pi;
     int nsteps;         // steps this member takes in the current launch sequence
-    int energy_it;      // launch index of this member's energy step; -1: every energy launch
+    int energy_it;      // step index of this member's energy step; -1: every energy launch
+    // two-step members (launch_batch_step2): g / ntiles are the two-step
+    // tile's, g1 / ntiles1 the one-step companion's (odd last step)
+    int fused;
+    int ntiles1;
+    TiledGeom g1;
     void *phi[2];
     void *pi[2];
     DevScalars *sc;
@@ -79,6 +84,16 @@ hipError_t launch_batch_step(int fp32, int energy, int items, int pf, const Batc
                              int total_tiles, int grid, size_t shm, int step_it, hipStream_t st);
 // Finalise the energies of the members that measured them in launch step_it.
 hipError_t launch_batch_finalize(const BatchMember *table, int nmem, int step_it, hipStream_t st);
+// Batched two-step launch number pit of a sequence (static members with a
+// two-step variant of 256 threads and at most `items` vectors per thread):
+// it covers steps 2 pit and 2 pit + 1.  A member with both steps left runs
+// its two-step tile (phi and pi flip once per launch); one with only step
+// 2 pit left runs its one-step companion tile (the odd last step, as a lone
+// run takes it); the others are skipped.  Energies as launch_batch_step, at
+// the member's energy step.  A member spans max(ntiles, ntiles1) work items.
+hipError_t launch_batch_step2(int fp32, int energy, int items, const BatchMember *table, int nmem, int total_tiles,
+                              int grid, size_t shm, int pit, hipStream_t st);
+hipError_t launch_batch_finalize2(const BatchMember *table, int nmem, int pit, hipStream_t st);
 size_t tiled_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
 TiledGeom tiled_geom(const SimConst &k, const TiledCfg &c);
 

@@ -220,6 +220,11 @@ cb_status cb_server_release(cb_server *s, unsigned long long id);   /* drop stor
 cb_status cb_server_stats(cb_server *s, int *queued, int *active, int *done, long long *bytes);
 /* block until no request is queued or active */
 cb_status cb_server_drain(cb_server *s);
+/* The kernel variant the server gives a request with the parameters of s
+ * (a two-step variant for a static box where one fits the batched launch,
+ * else a one-step one; -1 = the general kernel).  Selecting it on a lone
+ * simulation reproduces the server's results for that request bit for bit. */
+cb_status cb_server_variant(const cb_sim *s, int *variant);
 cb_status cb_server_destroy(cb_server *s);
 
 /* Measurement helper (no reference counterpart): achieved bandwidth of a
