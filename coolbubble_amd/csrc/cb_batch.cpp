@@ -204,7 +204,7 @@ cb_status enqueue_steps(cb_batch *b, int n)
     for (int it = 0; it < n; ++it) {
         const bool energy = b->any_lagged || it == n - 1;
         if (!b->tiled.empty()) {
-            HIP_TRY(cb::launch_batch_step(b->fp32, energy, b->items, b->pf, b->d_tab, (int)b->tiled.size(),
+            HIP_TRY(cb::launch_batch_step(b->fp32, energy, 0, b->items, b->pf, b->d_tab, (int)b->tiled.size(),
                                           b->total_tiles, b->grid, b->shm, it, b->st));
             if (energy) HIP_TRY(cb::launch_batch_finalize(b->d_tab, (int)b->tiled.size(), it, b->st));
         }

@@ -280,17 +280,26 @@ cb_status tick(cb_server *sv)
     }
     hipError_t e = hipSuccess;
     if (ntab) e = hipMemcpyAsync(sv->d_tab, sv->h_tab, sizeof(cb::BatchMember) * ntab, hipMemcpyHostToDevice, sv->st);
+    // members taking part in each launch without measuring energies (they
+    // get a plain launch of their own beside the energy launch)
+    std::vector<char> plain1((size_t)L, 0), plain2((size_t)(L + 1) / 2, 0);
+    for (size_t t = 0; t < single.size(); ++t) {
+        if (sv->h_tab[t].lagged) continue;
+        for (int it = 0; it < sv->h_tab[t].nsteps - 1; ++it) plain1[(size_t)it] = 1;
+    }
+    for (size_t t = 0; t < pair.size(); ++t)
+        for (int p = 0; 2 * p + 2 < tab2[t].nsteps; ++p) plain2[(size_t)p] = 1;
     const int cap = sv->cus * 8;
     for (int it = 0; it < L1 && e == hipSuccess; ++it) {
         const bool energy = energy1[(size_t)it] != 0;
-        e = cb::launch_batch_step(fp32, energy, cb::kBatchItems, 1, sv->d_tab, (int)single.size(), tiles1,
-                                  std::min(tiles1, cap), shm1, it, sv->st);
+        e = cb::launch_batch_step(fp32, energy, plain1[(size_t)it], cb::kBatchItems, 1, sv->d_tab, (int)single.size(),
+                                  tiles1, std::min(tiles1, cap), shm1, it, sv->st);
         if (e == hipSuccess && energy) e = cb::launch_batch_finalize(sv->d_tab, (int)single.size(), it, sv->st);
     }
     const cb::BatchMember *d_tab2 = sv->d_tab + single.size();
     for (int p = 0; 2 * p < L2 && e == hipSuccess; ++p) {
         const bool energy = energy2[(size_t)p] != 0;
-        e = cb::launch_batch_step2(fp32, energy, cb::kBatchItems, d_tab2, (int)pair.size(), tiles2,
+        e = cb::launch_batch_step2(fp32, energy, plain2[(size_t)p], cb::kBatchItems, d_tab2, (int)pair.size(), tiles2,
                                    std::min(tiles2, cap), shm2, p, sv->st);
         if (e == hipSuccess && energy) e = cb::launch_batch_finalize2(d_tab2, (int)pair.size(), p, sv->st);
     }

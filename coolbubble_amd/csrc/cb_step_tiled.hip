@@ -226,7 +226,7 @@ k_step_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restric
 // with the same config, so a member steps bit-identically alone or batched.
 template <typename T, bool ENERGY, int ITEMS, int PF>
 __global__ void __launch_bounds__(256)
-k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int step_it)
+k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int step_it, int sel)
 {
     constexpr int VW = sizeof(T) == 4 ? 4 : 2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -242,6 +242,9 @@ k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int ste
         }
         const BatchMember &m = table[lo];
         if (step_it >= m.nsteps) continue;     // this member stopped at an event earlier in the sequence
+        // members measure energies every step when expanding, else at their energy step
+        const bool want = m.lagged || m.energy_it < 0 || m.energy_it == step_it;
+        if (ENERGY ? !want : (sel == 1 && want)) continue;
         const int lt = w - m.tile_begin;
         const int pphi = (m.par_phi + step_it) & 1;
         const int ppi = m.lagged ? ((m.par_pi + step_it) & 1) : m.par_pi;
@@ -249,25 +252,16 @@ k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int ste
         T *phi_out = (T *)m.phi[pphi ^ 1];
         const T *pi_in = (const T *)m.pi[ppi];
         T *pi_out = m.lagged ? (T *)m.pi[ppi ^ 1] : (T *)m.pi[ppi];
-        // static members measure energies only at their own energy step
-        const bool me = ENERGY && (m.energy_it < 0 || m.energy_it == step_it);
         if (m.k.D == 3) {
             if (m.lagged)
                 tile_step<T, 3, VW, ITEMS, 256, true, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
                                                              m.partials, lds);
-            else if (me)
+            else
                 tile_step<T, 3, VW, ITEMS, 256, false, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
                                                               m.partials, lds);
-            else
-                tile_step<T, 3, VW, ITEMS, 256, false, false, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
-                                                             m.partials, lds);
         } else {
-            if (me)
-                tile_step<T, 2, VW, ITEMS, 256, false, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
-                                                              m.partials, lds);
-            else
-                tile_step<T, 2, VW, ITEMS, 256, false, false, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
-                                                             m.partials, lds);
+            tile_step<T, 2, VW, ITEMS, 256, false, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
+                                                          m.partials, lds);
         }
     }
 }
@@ -282,9 +276,9 @@ __global__ void __launch_bounds__(256) k_batch_finalize(const BatchMember *__res
 
 template <typename T, bool ENERGY>
 static void batch_launch_ip(int items, int pf, const BatchMember *table, int nmem, int total, int grid, size_t shm,
-                            int step_it, hipStream_t st)
+                            int step_it, int sel, hipStream_t st)
 {
-#define CB_BATCH_L(I, P) k_batch_step<T, ENERGY, I, P><<<grid, 256, shm, st>>>(table, nmem, total, step_it)
+#define CB_BATCH_L(I, P) k_batch_step<T, ENERGY, I, P><<<grid, 256, shm, st>>>(table, nmem, total, step_it, sel)
     if (pf == 2) {
         if (items == 1) CB_BATCH_L(1, 2);
         else if (items == 2) CB_BATCH_L(2, 2);
@@ -297,15 +291,19 @@ static void batch_launch_ip(int items, int pf, const BatchMember *table, int nme
 #undef CB_BATCH_L
 }
 
-hipError_t launch_batch_step(int fp32, int energy, int items, int pf, const BatchMember *table, int nmem,
+hipError_t launch_batch_step(int fp32, int energy, int split, int items, int pf, const BatchMember *table, int nmem,
                              int total_tiles, int grid, size_t shm, int step_it, hipStream_t st)
 {
+    // sel 0: every member, plain; 1: members not measuring energies, plain;
+    // 2 (energy kernel): members measuring energies
     if (fp32) {
-        if (energy) batch_launch_ip<float, true>(items, pf, table, nmem, total_tiles, grid, shm, step_it, st);
-        else batch_launch_ip<float, false>(items, pf, table, nmem, total_tiles, grid, shm, step_it, st);
+        if (!energy || split)
+            batch_launch_ip<float, false>(items, pf, table, nmem, total_tiles, grid, shm, step_it, energy ? 1 : 0, st);
+        if (energy) batch_launch_ip<float, true>(items, pf, table, nmem, total_tiles, grid, shm, step_it, 2, st);
     } else {
-        if (energy) batch_launch_ip<double, true>(items, pf, table, nmem, total_tiles, grid, shm, step_it, st);
-        else batch_launch_ip<double, false>(items, pf, table, nmem, total_tiles, grid, shm, step_it, st);
+        if (!energy || split)
+            batch_launch_ip<double, false>(items, pf, table, nmem, total_tiles, grid, shm, step_it, energy ? 1 : 0, st);
+        if (energy) batch_launch_ip<double, true>(items, pf, table, nmem, total_tiles, grid, shm, step_it, 2, st);
     }
     return hipGetLastError();
 }
@@ -319,7 +317,7 @@ hipError_t launch_batch_finalize(const BatchMember *table, int nmem, int step_it
 // Batched two-step launch: every tile of every static two-step member.
 template <typename T, bool ENERGY, int ITEMS>
 __global__ void __launch_bounds__(256)
-k_batch_step2(const BatchMember *__restrict__ table, int nmem, int total, int pit)
+k_batch_step2(const BatchMember *__restrict__ table, int nmem, int total, int pit, int sel)
 {
     constexpr int VW = sizeof(T) == 4 ? 4 : 2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -336,34 +334,25 @@ k_batch_step2(const BatchMember *__restrict__ table, int nmem, int total, int pi
         }
         const BatchMember &m = table[lo];
         if (s0 >= m.nsteps) continue;
+        const bool two = s0 + 1 < m.nsteps;
+        const bool want = m.energy_it < 0 || m.energy_it == (two ? s0 + 1 : s0);
+        if (ENERGY ? !want : (sel == 1 && want)) continue;
         const int lt = w - m.tile_begin;
         const int pphi = (m.par_phi + pit) & 1;
         const int ppi = (m.par_pi + pit) & 1;
         const T *phi_in = (const T *)m.phi[pphi];
         T *phi_out = (T *)m.phi[pphi ^ 1];
-        if (s0 + 1 < m.nsteps) {   // both steps: the two-step tile
+        if (two) {                 // both steps: the two-step tile
             if (lt >= m.ntiles) continue;
-            const bool me = ENERGY && (m.energy_it < 0 || m.energy_it == s0 + 1);
             const T *pi_in = (const T *)m.pi[ppi];
             T *pi_out = (T *)m.pi[ppi ^ 1];
-            if (m.k.D == 3) {
-                if (me) tile_step2<T, 3, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
-                else tile_step2<T, 3, ITEMS, 256, false>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
-            } else {
-                if (me) tile_step2<T, 2, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
-                else tile_step2<T, 2, ITEMS, 256, false>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
-            }
+            if (m.k.D == 3) tile_step2<T, 3, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
+            else tile_step2<T, 2, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
         } else {                   // the odd last step: the one-step companion, pi in place
             if (lt >= m.ntiles1) continue;
-            const bool me = ENERGY && (m.energy_it < 0 || m.energy_it == s0);
             T *pi_io = (T *)m.pi[ppi];
-            if (m.k.D == 3) {
-                if (me) tile_step<T, 3, VW, ITEMS, 256, false, ENERGY, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
-                else tile_step<T, 3, VW, ITEMS, 256, false, false, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
-            } else {
-                if (me) tile_step<T, 2, VW, ITEMS, 256, false, ENERGY, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
-                else tile_step<T, 2, VW, ITEMS, 256, false, false, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
-            }
+            if (m.k.D == 3) tile_step<T, 3, VW, ITEMS, 256, false, ENERGY, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
+            else tile_step<T, 2, VW, ITEMS, 256, false, ENERGY, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
         }
     }
 }
@@ -380,22 +369,22 @@ __global__ void __launch_bounds__(256) k_batch_finalize2(const BatchMember *__re
 
 template <typename T, bool ENERGY>
 static void batch2_launch_i(int items, const BatchMember *table, int nmem, int total, int grid, size_t shm, int pit,
-                            hipStream_t st)
+                            int sel, hipStream_t st)
 {
-    if (items == 1) k_batch_step2<T, ENERGY, 1><<<grid, 256, shm, st>>>(table, nmem, total, pit);
-    else k_batch_step2<T, ENERGY, 2><<<grid, 256, shm, st>>>(table, nmem, total, pit);
+    if (items == 1) k_batch_step2<T, ENERGY, 1><<<grid, 256, shm, st>>>(table, nmem, total, pit, sel);
+    else k_batch_step2<T, ENERGY, 2><<<grid, 256, shm, st>>>(table, nmem, total, pit, sel);
 }
 
-hipError_t launch_batch_step2(int fp32, int energy, int items, const BatchMember *table, int nmem, int total_tiles,
-                              int grid, size_t shm, int pit, hipStream_t st)
+hipError_t launch_batch_step2(int fp32, int energy, int split, int items, const BatchMember *table, int nmem,
+                              int total_tiles, int grid, size_t shm, int pit, hipStream_t st)
 {
     if (items > 2) return hipErrorInvalidValue;
     if (fp32) {
-        if (energy) batch2_launch_i<float, true>(items, table, nmem, total_tiles, grid, shm, pit, st);
-        else batch2_launch_i<float, false>(items, table, nmem, total_tiles, grid, shm, pit, st);
+        if (!energy || split) batch2_launch_i<float, false>(items, table, nmem, total_tiles, grid, shm, pit, energy ? 1 : 0, st);
+        if (energy) batch2_launch_i<float, true>(items, table, nmem, total_tiles, grid, shm, pit, 2, st);
     } else {
-        if (energy) batch2_launch_i<double, true>(items, table, nmem, total_tiles, grid, shm, pit, st);
-        else batch2_launch_i<double, false>(items, table, nmem, total_tiles, grid, shm, pit, st);
+        if (!energy || split) batch2_launch_i<double, false>(items, table, nmem, total_tiles, grid, shm, pit, energy ? 1 : 0, st);
+        if (energy) batch2_launch_i<double, true>(items, table, nmem, total_tiles, grid, shm, pit, 2, st);
     }
     return hipGetLastError();
 }

@@ -72,15 +72,17 @@ struct BatchMember {
 // Batched persistent step: every tile of every member, one launch.  All
 // members share the precision; their tiled configs must have 256 threads and
 // at most `items` vectors per thread; `pf` is the launch's prefetch depth.
-// step_parity flips the buffer roles of every member relative to the table's
-// parities.  A member's results depend only on its (TR, NI, threads), not on
-// items, pf or the non-temporal flag, so it steps bit-identically alone with
-// a variant of the same (TR, NI, threads).
+// A member's results depend only on its (TR, NI, threads), not on items, pf
+// or the non-temporal flag, so it steps bit-identically alone with a variant
+// of the same (TR, NI, threads).
 // step_it = index of this launch in the sequence: a member takes part while
-// step_it < its nsteps (its buffers flip with step_it & 1).  In a launch with
-// `energy` set, a member measures energies if it expands, if its energy_it
-// is -1, or if its energy_it equals step_it; the others take the plain step.
-hipError_t launch_batch_step(int fp32, int energy, int items, int pf, const BatchMember *table, int nmem,
+// step_it < its nsteps (its buffers flip with step_it & 1).  With `energy`
+// set, a member measures energies if it expands, if its energy_it is -1, or
+// if its energy_it equals step_it; with `split` also set, the other members
+// take the plain step in a launch of their own first (an energy launch runs
+// only the members that measure energies, so the plain ones keep the lighter
+// kernel).
+hipError_t launch_batch_step(int fp32, int energy, int split, int items, int pf, const BatchMember *table, int nmem,
                              int total_tiles, int grid, size_t shm, int step_it, hipStream_t st);
 // Finalise the energies of the members that measured them in launch step_it.
 hipError_t launch_batch_finalize(const BatchMember *table, int nmem, int step_it, hipStream_t st);
@@ -91,8 +93,8 @@ hipError_t launch_batch_finalize(const BatchMember *table, int nmem, int step_it
 // 2 pit left runs its one-step companion tile (the odd last step, as a lone
 // run takes it); the others are skipped.  Energies as launch_batch_step, at
 // the member's energy step.  A member spans max(ntiles, ntiles1) work items.
-hipError_t launch_batch_step2(int fp32, int energy, int items, const BatchMember *table, int nmem, int total_tiles,
-                              int grid, size_t shm, int pit, hipStream_t st);
+hipError_t launch_batch_step2(int fp32, int energy, int split, int items, const BatchMember *table, int nmem,
+                              int total_tiles, int grid, size_t shm, int pit, hipStream_t st);
 hipError_t launch_batch_finalize2(const BatchMember *table, int nmem, int pit, hipStream_t st);
 size_t tiled_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
 TiledGeom tiled_geom(const SimConst &k, const TiledCfg &c);
