@@ -39,6 +39,7 @@ struct DevScalars {
     double fac_lap;         // a^(D-2+alpha) for the next force evaluation
     double fac_pot;         // a^(D+alpha)
     double adx;             // a * dx
+    double inv_adx;         // 1 / (a dx), for the energies' gradients
     double pk2, pk1;        // a_kin^(-2D), a_kin^(-D)
     double Etot, Ekintot, Epottot, Egradtot, avphi, avphiP;
     double EtotVar, EkintotVar, EpottotVar, EgradtotVar, avphiVar, avphiPVar;

@@ -121,23 +121,34 @@ __device__ inline void neighbours(long long s, int N, long long (&nb)[6])
 
 // Energy terms of one site (measureEnergies, src/bubble.cpp:790-827), in
 // double.  gP = {forward neighbour along axis 1, 0, 2} per the reference's
-// gx, gy, gz naming (1D: gx only).
-template <int D>
+// gx, gy, gz naming (1D: gx only).  EXACT: the FP64 path.
+template <int D, bool EXACT>
 __device__ inline void energy_terms(double ph, const double (&fwd)[3], double pn, double po,
                                     const SimConst &k, const DevScalars *sc, double (&acc)[NQ])
 {
-    const double adx = sc->adx, pk2 = sc->pk2, pk1 = sc->pk1, dxLD = k.dxLD;
+    // The gradient divides by a dx through its reciprocal (within 1 ulp of the
+    // reference's quotient; its terms are positive, so the sums keep that).
+    // The potential is a difference of O(c) terms that cancel near phi = 0
+    // and phi = 1, so the FP64 path keeps the reference's correctly rounded
+    // powers; the FP32 path, whose fields carry 1e-7 errors, uses products.
+    const double iadx = sc->inv_adx, pk2 = sc->pk2, pk1 = sc->pk1, dxLD = k.dxLD;
     double gx = 0., gy = 0., gz = 0.;
     if constexpr (D == 1) {
-        gx = (fwd[0] - ph) / adx;
+        gx = (fwd[0] - ph) * iadx;
     } else {
-        gx = (fwd[1] - ph) / adx;
-        gy = (fwd[0] - ph) / adx;
-        if constexpr (D == 3) gz = (fwd[2] - ph) / adx;
+        gx = (fwd[1] - ph) * iadx;
+        gy = (fwd[0] - ph) * iadx;
+        if constexpr (D == 3) gz = (fwd[2] - ph) * iadx;
     }
     double gx2 = gx * gx, gy2 = gy * gy, gz2 = gz * gz;
     double Eg = 0.5 * (gx2 + gy2 + gz2);
-    double Ep = vpot(ph, k);
+    double Ep;
+    if constexpr (EXACT) {
+        Ep = vpot(ph, k);
+    } else {
+        const double ph2 = ph * ph;
+        Ep = ((k.c * ph2 - k.c5 * (ph2 * ph)) + k.c6 * (ph2 * ph2)) + k.DeltaV;
+    }
     double Ek = 0.5 * pk2 * pn * pn;
     double Et = Ek + Eg + Ep;
     double pp = pn * pk1;
@@ -198,6 +209,7 @@ __device__ inline void finalize_member(const SimConst &k, const double *partials
     s.fac_lap = ipow_rn(s.a, k.D - 2 + k.alpha);
     s.fac_pot = ipow_rn(s.a, k.D + k.alpha);
     s.adx = s.a * k.dx;
+    s.inv_adx = 1.0 / s.adx;
     const double a_kin = s.a + 0.5 * k.dt * s.ap;
     s.pk2 = ipow_rn(a_kin, -2 * k.D);
     s.pk1 = ipow_rn(a_kin, -k.D);

@@ -67,7 +67,7 @@ k_step_simple(SimConst k, const T *__restrict__ phi_in, T *__restrict__ phi_out,
         else phi_out[s] = (T)(ph + dt * pn);
         if constexpr (ENERGY) {
             double fwd[3] = {(double)P[0], D >= 2 ? (double)P[1] : 0.0, D == 3 ? (double)P[2] : 0.0};
-            energy_terms<D>((double)ph, fwd, (double)pn, (double)po, k, sc, acc);
+            energy_terms<D, sizeof(T) == 8>((double)ph, fwd, (double)pn, (double)po, k, sc, acc);
         }
     }
     if constexpr (ENERGY) block_partials<NQ>(acc, partials + (long long)blockIdx.x * NQ);

@@ -138,6 +138,7 @@ void host_factors(const cb_sim *s, cb::DevScalars &h)
     h.fac_lap = std::pow(h.a, D - 2 + al);
     h.fac_pot = std::pow(h.a, D + al);
     h.adx = h.a * s->k.dx;
+    h.inv_adx = 1.0 / h.adx;
     double a_kin = h.a + 0.5 * s->p.dt * h.ap;
     h.pk2 = std::pow(a_kin, -2 * D);
     h.pk1 = std::pow(a_kin, -D);

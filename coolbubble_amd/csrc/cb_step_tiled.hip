@@ -184,7 +184,7 @@ __device__ inline void tile_step(const SimConst &k, const TiledGeom &g, int lt, 
                     fwd[0] = (double)mP;
                     if constexpr (D == 3) { fwd[1] = (double)rp.v[e]; fwd[2] = (double)fP; }
                     else { fwd[1] = (double)fP; fwd[2] = 0.0; }
-                    energy_terms<D>((double)c, fwd, (double)pnew, (double)po.v[e], k, sc, acc);
+                    energy_terms<D, sizeof(T) == 8>((double)c, fwd, (double)pnew, (double)po.v[e], k, sc, acc);
                 }
             }
             store_vec<T, VW>(pi_out + base + off[u], pn, nt);
