@@ -133,7 +133,7 @@ def run_ensemble(cb, hip, nt, log):
     log("ensemble: %d members, %d sites, created in %.1f s" % (len(batch.members), batch.sites,
                                                                 time.time() - t0))
     shape, tms = batch.autotune(steps=10)
-    log("ensemble: autotuned launch shape (items, prefetch, nt, NI) = %s, %.3f ms/step" % (shape, tms))
+    log("ensemble: autotuned launch shape (items, prefetch, nt, NI, two-step) = %s, %.3f ms/step" % (shape, tms))
     batch.step(40)   # graph capture + warm
     stream = batch.stream()
     a, b = hip.event(), hip.event()
@@ -243,6 +243,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--only", choices=["ensemble", "serving"], default=None,
+                    help="run one leg alone and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice time-stepper runs on a single MI355X "
@@ -259,6 +261,12 @@ def main():
     sites = N ** 3
     K, W = args.steps, args.warmup
 
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": run_ensemble(cb, hip, args.ensemble_steps, log)}), flush=True)
+        return 0
+    if args.only == "serving":
+        print(json.dumps({"serving": run_serving(cb, args.serve_requests, args.serve_rate, log)}), flush=True)
+        return 0
     copy_gbps = cb.bandwidth_probe(1 << 31, 20)
     log("device-to-device copy: %.1f GB/s (read + write)" % copy_gbps)
     if args.precision != "both":
@@ -283,12 +291,18 @@ def main():
 
     su32 = sites * K / (ms32 * 1e-3)
     traffic32, steps_per_launch32 = None, info32.get("fuse", 1) if info32 else 1
+    prof32 = {}
     try:
-        rec = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                                          "hbm_traffic.json")))["fp32"]
-        if info32 and rec["variant"] == sel32:
-            traffic32 = rec["traffic_bytes_per_launch"]
-    except (OSError, KeyError, ValueError):
+        recs = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                           "hbm_traffic.json")))["stepper"]
+        for rec in recs:
+            if info32 and rec["precision"] == "fp32" and rec["variant_config"] == info32:
+                traffic32 = rec["traffic_bytes_per_launch"]
+                prof32 = {"kernel": rec["kernel"], "kernel_avg_launch_us_profiled": rec["avg_launch_us"],
+                          "delivered_GBps_profiled": traffic32 / (rec["avg_launch_us"] * 1e-6) / 1e9,
+                          "algorithmic_GBps_profiled": 16.0 * sites * rec["steps_per_launch"]
+                          / (rec["avg_launch_us"] * 1e-6) / 1e9}
+    except (OSError, KeyError, ValueError, TypeError):
         pass
     su64 = sites * K / (ms64 * 1e-3)
     t_step32 = ms32 * 1e-3 / K
@@ -326,13 +340,17 @@ def main():
         "config": {"workload": "lattice time-stepper d=3 N=512^3 L=64 c=4 mode 1 nBubbles=32 "
                                "dt=0.025 static, energies every 40 steps",
                    "sites": sites, "energy_every": ENERGY_EVERY},
-        "roofline": {"bound": "hbm", "achieved": ach32 / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": ach32 / HBM_PEAK, "traffic": traffic32,
-                     "steps_per_launch": steps_per_launch32,
-                     "note": "achieved = 16 B per site-update x site-updates/s (one step per pass over "
-                             "phi and pi); variants that fuse two steps per pass read and write each "
-                             "byte once per two steps, so this can exceed 1. traffic = measured HBM "
-                             "bytes per launch of the profiled variant (profiles/hbm_traffic.json)"},
+        "roofline": dict({"bound": "hbm", "achieved": ach32 / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                          "frac": ach32 / HBM_PEAK, "traffic": traffic32,
+                          "steps_per_launch": steps_per_launch32,
+                          "note": "achieved = 16 B per site-update (read phi, pi; write pi, phi) x site-updates/s "
+                                  "over whole steps (energy launch and finalisation included), HIP events on the "
+                                  "simulation's stream; a two-step variant reads and writes each byte once per "
+                                  "two site-updates, so frac can exceed 1 while HBM delivers less than its peak. "
+                                  "traffic = measured HBM bytes per launch of the selected variant "
+                                  "(profiles/hbm_traffic.json, rocprofv3 FETCH_SIZE + WRITE_SIZE); "
+                                  "delivered_GBps_profiled = traffic / the kernel's profiled average launch"},
+                         **prof32),
         "cpu_baseline": cpu,
         "fp64": {"value": su64, "ms_per_step": ms64 / K, "achieved_GBps": ach64 / 1e9,
                  "frac": ach64 / HBM_PEAK},

@@ -112,6 +112,8 @@ def lib():
             getattr(L, name).restype = ctypes.c_int
         L.cb_batch_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp]
         L.cb_batch_autotune.restype = ctypes.c_int
+        L.cb_batch_set_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.cb_batch_set_shape.restype = ctypes.c_int
         ull = ctypes.c_ulonglong
         L.cb_server_create.argtypes = [ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]
         L.cb_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ull)]
@@ -309,11 +311,16 @@ class Batch:
         check(lib().cb_batch_step(self._h, int(n)))
 
     def autotune(self, steps=10):
-        """Pick the batched launch shape; returns ((items, pf, nt, NI), ms per step)."""
-        shape = (ctypes.c_int * 4)()
+        """Pick the batched launch shape; returns ((items, pf, nt, NI, two_step), ms per step)."""
+        shape = (ctypes.c_int * 5)()
         ms = ctypes.c_double()
         check(lib().cb_batch_autotune(self._h, int(steps), shape, ctypes.byref(ms)))
         return tuple(shape[:]), ms.value
+
+    def set_shape(self, shape):
+        """Set the batched launch shape (items, pf, nt, NI, two_step)."""
+        arr = (ctypes.c_int * 5)(*[int(x) for x in shape])
+        check(lib().cb_batch_set_shape(self._h, arr))
 
     def stream(self):
         s = ctypes.c_void_p()

@@ -145,7 +145,8 @@ void advance_host2(cb_sim *s, int n)
 
 struct cb_batch {
     std::vector<cb_sim *> mem;
-    std::vector<int> tiled;        // member indices on the batched kernel, table order
+    std::vector<int> tiled;        // member indices on the batched one-step kernel, table order
+    std::vector<int> paired;       // member indices on the batched two-step kernel, after them
     std::vector<int> general;      // member indices on their own general kernel
     hipStream_t st = nullptr;
     int fp32 = 0;
@@ -154,6 +155,9 @@ struct cb_batch {
     cb::BatchMember *d_tab = nullptr;
     int total_tiles = 0, grid = 0;
     size_t shm = 0;
+    int total_tiles2 = 0, grid2 = 0;    // the two-step part of the table
+    size_t shm2 = 0;
+    int fused = 0;                      // static members take two steps per launch
     long long sites = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -183,16 +187,18 @@ void batch_free(cb_batch *b)
 }
 
 // Refresh the parities of the table from the members' current buffers.
-cb_status upload_table(cb_batch *b)
+// nsteps: length of the launch sequence that follows (the two-step members
+// take their odd last step on the companion tile).
+cb_status upload_table(cb_batch *b, int nsteps)
 {
-    for (size_t t = 0; t < b->tiled.size(); ++t) {
-        const cb_sim *s = b->mem[(size_t)b->tiled[t]];
+    const size_t n1 = b->tiled.size(), n = n1 + b->paired.size();
+    for (size_t t = 0; t < n; ++t) {
+        const cb_sim *s = b->mem[(size_t)(t < n1 ? b->tiled[t] : b->paired[t - n1])];
         b->h_tab[t].par_phi = s->iphi;
         b->h_tab[t].par_pi = s->ipi;
+        if (t >= n1) b->h_tab[t].nsteps = nsteps;
     }
-    if (!b->tiled.empty())
-        HIP_TRY(hipMemcpyAsync(b->d_tab, b->h_tab, sizeof(cb::BatchMember) * b->tiled.size(), hipMemcpyHostToDevice,
-                               b->st));
+    if (n) HIP_TRY(hipMemcpyAsync(b->d_tab, b->h_tab, sizeof(cb::BatchMember) * n, hipMemcpyHostToDevice, b->st));
     return CB_OK;
 }
 
@@ -209,11 +215,23 @@ cb_status enqueue_steps(cb_batch *b, int n)
             if (energy) HIP_TRY(cb::launch_batch_finalize(b->d_tab, (int)b->tiled.size(), it, b->st));
         }
     }
+    // two-step members: launch p covers steps 2p and 2p + 1 (the last one,
+    // for odd n, only step n - 1 on the one-step companion)
+    if (!b->paired.empty()) {
+        const cb::BatchMember *d2 = b->d_tab + b->tiled.size();
+        for (int p = 0; 2 * p < n; ++p) {
+            const bool energy = p == (n - 1) / 2;
+            HIP_TRY(cb::launch_batch_step2(b->fp32, energy, 0, cb::kBatchItems, d2, (int)b->paired.size(),
+                                           b->total_tiles2, b->grid2, b->shm2, p, b->st));
+            if (energy) HIP_TRY(cb::launch_batch_finalize2(d2, (int)b->paired.size(), p, b->st));
+        }
+    }
     for (int i : b->general) {
         cb_status r = cb::step_n(b->mem[(size_t)i], n);
         if (r != CB_OK) return r;
     }
     for (int i : b->tiled) cb::advance_host(b->mem[(size_t)i], n);
+    for (int i : b->paired) cb::advance_host2(b->mem[(size_t)i], n);
     return CB_OK;
 }
 
@@ -232,37 +250,50 @@ cb_status build_table(cb_batch *b)
     if (b->st) HIP_TRY(hipStreamSynchronize(b->st));
     drop_graph(b);
     b->tiled.clear();
+    b->paired.clear();
     b->general.clear();
     for (size_t i = 0; i < b->mem.size(); ++i) {
         cb_sim *s = b->mem[i];
+        const int pick2 = b->fused ? cb::pick_variant2(s) : -1;
+        if (pick2 >= 0 && cb::select_variant(s, pick2) == CB_OK) {
+            b->paired.push_back((int)i);
+            continue;
+        }
         const int pick = cb::pick_variant(s, b->items, b->pf, b->nt, b->ni);
         cb::select_variant(s, pick);
         (pick >= 0 ? b->tiled : b->general).push_back((int)i);
     }
-    const size_t nt = b->tiled.size();
+    const size_t n1 = b->tiled.size(), n2 = b->paired.size();
     if (b->d_tab) { hipFree(b->d_tab); b->d_tab = nullptr; }
     if (b->h_tab) { hipHostFree(b->h_tab); b->h_tab = nullptr; }
-    b->total_tiles = 0;
-    b->shm = 0;
-    b->grid = 0;
-    if (!nt) return CB_OK;
-    if (hipHostMalloc((void **)&b->h_tab, sizeof(cb::BatchMember) * nt, hipHostMallocDefault) != hipSuccess ||
-        hipMalloc((void **)&b->d_tab, sizeof(cb::BatchMember) * nt) != hipSuccess) {
+    b->total_tiles = b->total_tiles2 = 0;
+    b->shm = b->shm2 = 0;
+    b->grid = b->grid2 = 0;
+    if (!n1 && !n2) return CB_OK;
+    if (hipHostMalloc((void **)&b->h_tab, sizeof(cb::BatchMember) * (n1 + n2), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void **)&b->d_tab, sizeof(cb::BatchMember) * (n1 + n2)) != hipSuccess) {
         set_error("batch table allocation failed");
         return CB_OUT_OF_MEMORY;
     }
     int tiles = 0;
-    for (size_t t = 0; t < nt; ++t) {
+    for (size_t t = 0; t < n1; ++t) {
         cb::fill_member(b->h_tab[t], b->mem[(size_t)b->tiled[t]], tiles, b->fp32, &b->shm);
-        tiles += b->h_tab[t].ntiles;
+        tiles += cb::member_work(b->h_tab[t]);
     }
     b->total_tiles = tiles;
+    tiles = 0;
+    for (size_t t = 0; t < n2; ++t) {
+        cb::fill_member(b->h_tab[n1 + t], b->mem[(size_t)b->paired[t]], tiles, b->fp32, &b->shm2);
+        tiles += cb::member_work(b->h_tab[n1 + t]);
+    }
+    b->total_tiles2 = tiles;
     int dev = 0, cus = 256;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const int cap = cus * 8;
-    b->grid = tiles < cap ? tiles : cap;
-    return upload_table(b);
+    b->grid = b->total_tiles < cap ? b->total_tiles : cap;
+    b->grid2 = b->total_tiles2 < cap ? b->total_tiles2 : cap;
+    return upload_table(b, kSegment);
 }
 
 cb_status capture_segment(cb_batch *b)
@@ -344,7 +375,7 @@ cb_status cb_batch_create(const cb_params *members, int n, cb_batch **out)
 cb_status cb_batch_step(cb_batch *b, int nsteps)
 {
     if (!b || nsteps < 0) { set_error("invalid argument"); return CB_INVALID_ARG; }
-    cb_status r = upload_table(b);
+    cb_status r = upload_table(b, kSegment);
     if (r != CB_OK) return r;
     int left = nsteps;
     if (b->exec) {   // general members moved to other buffers since the capture: capture again
@@ -370,7 +401,11 @@ cb_status cb_batch_step(cb_batch *b, int nsteps)
         }
         left -= kSegment;
     }
-    if (left > 0) return enqueue_steps(b, left);
+    if (left > 0) {
+        r = upload_table(b, left);
+        if (r != CB_OK) return r;
+        return enqueue_steps(b, left);
+    }
     return CB_OK;
 }
 
@@ -396,7 +431,7 @@ cb_status cb_batch_stream(cb_batch *b, void **stream)
     return CB_OK;
 }
 
-cb_status cb_batch_autotune(cb_batch *b, int steps, int shape[4], double *best_ms)
+cb_status cb_batch_autotune(cb_batch *b, int steps, int shape[5], double *best_ms)
 {
     if (!b || steps < 1) { set_error("invalid argument"); return CB_INVALID_ARG; }
     HIP_TRY(hipStreamSynchronize(b->st));
@@ -433,8 +468,10 @@ cb_status cb_batch_autotune(cb_batch *b, int steps, int shape[4], double *best_m
     if (r == CB_OK && (hipStreamSynchronize(b->st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
                        hipEventCreate(&e1) != hipSuccess))
         r = CB_HIP_ERROR;
-    const int shapes[][4] = {{4, 1, 1, 32}, {4, 2, 1, 32}, {4, 1, 0, 32}, {2, 1, 1, 32}, {1, 1, 1, 32},
-                             {4, 1, 1, 16}, {4, 1, 1, 64}, {1, 2, 1, 32}, {2, 2, 1, 16}, {4, 2, 0, 16}};
+    // (vectors per thread, prefetch, non-temporal stores, chunk planes, two steps per launch)
+    const int shapes[][5] = {{4, 1, 1, 32, 0}, {4, 2, 1, 32, 0}, {4, 1, 0, 32, 0}, {2, 1, 1, 32, 0},
+                             {1, 1, 1, 32, 0}, {4, 1, 1, 16, 0}, {4, 1, 1, 64, 0}, {1, 2, 1, 32, 0},
+                             {2, 2, 1, 16, 0}, {4, 2, 0, 16, 0}, {1, 1, 1, 32, 1}, {2, 1, 1, 32, 1}};
     double best = 1e300;
     int best_i = 0;
     for (int c = 0; c < (int)(sizeof shapes / sizeof shapes[0]) && r == CB_OK; ++c) {
@@ -442,12 +479,13 @@ cb_status cb_batch_autotune(cb_batch *b, int steps, int shape[4], double *best_m
         b->pf = shapes[c][1];
         b->nt = shapes[c][2];
         b->ni = shapes[c][3];
+        b->fused = shapes[c][4];
         r = build_table(b);
         if (r != CB_OK) break;
         double per = 1e300;
         for (int rep = 0; rep < 2 && r == CB_OK; ++rep) {
             if (restore() != hipSuccess) { r = CB_HIP_ERROR; break; }
-            r = upload_table(b);
+            r = upload_table(b, steps);
             if (r != CB_OK) break;
             hipEventRecord(e0, b->st);
             r = enqueue_steps(b, steps);
@@ -470,12 +508,30 @@ cb_status cb_batch_autotune(cb_batch *b, int steps, int shape[4], double *best_m
     b->pf = shapes[best_i][1];
     b->nt = shapes[best_i][2];
     b->ni = shapes[best_i][3];
+    b->fused = shapes[best_i][4];
     cb_status r2 = build_table(b);
     if (r == CB_OK) r = r2;
     if (r != CB_OK) return r;
-    if (shape) for (int i = 0; i < 4; ++i) shape[i] = shapes[best_i][i];
+    if (shape) for (int i = 0; i < 5; ++i) shape[i] = shapes[best_i][i];
     if (best_ms) *best_ms = best;
     return CB_OK;
+}
+
+cb_status cb_batch_set_shape(cb_batch *b, const int shape[5])
+{
+    if (!b || !shape) { set_error("invalid argument"); return CB_INVALID_ARG; }
+    const int items = shape[0], pf = shape[1], nt = shape[2], ni = shape[3], fused = shape[4];
+    if ((items != 1 && items != 2 && items != 4) || (pf != 1 && pf != 2) || (nt != 0 && nt != 1) || ni < 1 ||
+        (fused != 0 && fused != 1)) {
+        set_error("batch shape out of range");
+        return CB_INVALID_ARG;
+    }
+    b->items = items;
+    b->pf = pf;
+    b->nt = nt;
+    b->ni = ni;
+    b->fused = fused;
+    return build_table(b);
 }
 
 cb_status cb_batch_destroy(cb_batch *b)

@@ -25,11 +25,19 @@ def members(precision):
     return base
 
 
+@pytest.mark.parametrize("two_step", [0, 1])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_members_bit_identical_alone_and_batched(precision):
+def test_members_bit_identical_alone_and_batched(precision, two_step):
+    """two_step: static members take two leapfrog steps per batched launch
+    (odd remainders on their one-step companion), as a lone run with the
+    same two-step variant does."""
     ms = members(precision)
-    print("batch", precision, len(ms), flush=True)
+    print("batch", precision, len(ms), "two_step", two_step, flush=True)
     b = cb.Batch(ms)
+    b.set_shape((2, 1, 1, 32, two_step))
+    if two_step:
+        fused = [v.variant_info(v.kernel_variants()[1])["fuse"] for v in b.members if v.kernel_variants()[1] >= 0]
+        assert 2 in fused
     alone = []
     for m, view in zip(ms, b.members):
         s = cb.Bubble(**m)
