@@ -131,6 +131,11 @@ def lib():
                      "cb_server_rows", "cb_server_spectrum", "cb_server_release", "cb_server_stats",
                      "cb_server_drain", "cb_server_destroy", "cb_server_variant"):
             getattr(L, name).restype = ctypes.c_int
+        L.cb_default_run_opts.argtypes = [vp]
+        L.cb_run_folder_name.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+        L.cb_run.argtypes = [vp]
+        for name in ("cb_default_run_opts", "cb_run_folder_name", "cb_run"):
+            getattr(L, name).restype = ctypes.c_int
         L.cb_nsites.argtypes = [vp]
         L.cb_nsites.restype = ctypes.c_longlong
         L.cb_destroy.argtypes = [vp]
@@ -164,6 +169,51 @@ def default_params(**kw):
             val = CB_FP32 if val.lower() == "fp32" else CB_FP64
         setattr(p, key, val)
     return p
+
+
+class RunOpts(ctypes.Structure):
+    """cb_run_opts (include/coolbubble.h)."""
+    _fields_ = [("params", Params), ("nt", ctypes.c_int), ("snapshots", ctypes.c_int),
+                ("snapshot_spacing", ctypes.c_int), ("energy_every", ctypes.c_int), ("fft", ctypes.c_int),
+                ("full1d_every", ctypes.c_int), ("out_dir", ctypes.c_char_p),
+                ("xns", ctypes.POINTER(ctypes.c_double)), ("n_xns", ctypes.c_int),
+                ("folder", ctypes.c_char * 1024), ("elapsed_s", ctypes.c_double)]
+
+
+def _run_opts(out_dir, nt, snapshots, snapshot_spacing, energy_every, fft, full1d_every, params):
+    o = RunOpts()
+    check(lib().cb_default_run_opts(ctypes.byref(o)))
+    o.params = params
+    for key, val in (("nt", nt), ("snapshots", snapshots), ("snapshot_spacing", snapshot_spacing),
+                     ("energy_every", energy_every), ("fft", fft), ("full1d_every", full1d_every)):
+        if val is not None:
+            setattr(o, key, int(val))
+    o.out_dir = os.fsencode(out_dir)
+    return o
+
+
+def run_folder_name(nt=None, snapshots=None, fft=None, full1d_every=None, **kw):
+    """The reference's run-folder name for these options (src/main.cpp:243-247)."""
+    o = _run_opts(".", nt, snapshots, None, None, fft, full1d_every, default_params(**kw))
+    buf = ctypes.create_string_buffer(1024)
+    check(lib().cb_run_folder_name(ctypes.byref(o), buf, 1024))
+    return buf.value.decode()
+
+
+def run(out_dir, nt=None, snapshots=None, snapshot_spacing=None, energy_every=None, fft=None,
+        full1d_every=None, xns=None, **kw):
+    """The reference's whole run (src/main.cpp:229-310): creates the run folder
+    under out_dir, writes parameters.dat, EnergyDensities.dat, __xns.dat,
+    _phi_k_t=NNNN.dat and (1D) FullField.dat.  Returns (folder, elapsed s)."""
+    o = _run_opts(out_dir, nt, snapshots, snapshot_spacing, energy_every, fft, full1d_every,
+                  default_params(**kw))
+    keep = None
+    if xns is not None:
+        keep = np.ascontiguousarray(np.asarray(xns, dtype=np.float64).reshape(-1, 3))
+        o.xns = _dp(keep)
+        o.n_xns = keep.shape[0]
+    check(lib().cb_run(ctypes.byref(o)))
+    return o.folder.decode(), o.elapsed_s
 
 
 def bandwidth_probe(nbytes=1 << 30, reps=20, device=0):

@@ -1,0 +1,308 @@
+// cb_run.cpp -- the reference's run driver over the C ABI: the time loop of
+// src/main.cpp:229-310 with the reference's text outputs in a run folder.
+//
+// Files (same names, layouts and printf formats as the reference, so its
+// readers and plotting scripts take them unchanged):
+//   parameters.dat       OutParameters, src/bubble.cpp:1018-1032, plus the
+//                        "Elapsed time" line of src/main.cpp:303-308
+//   EnergyDensities.dat  InitOutputFiles header (src/bubble.cpp:958-966) and
+//                        one outEP row (:977-989) at t = 0 and every
+//                        OutEnergyEvery steps
+//   __xns.dat            mode 1: the nucleation points as GenerateNucleationPoints
+//                        writes them (:397-417); mode 0 reads it (:420-453)
+//   _phi_k_t=NNNN.dat    FourierTransform + ComputePowerSpectrum (:642-709) at
+//                        the initial state and every snapshot
+//   FullField.dat        1D: outFullField (:993-1006) at t = 0 and every
+//                        Full1DFieldEvery steps
+// Not written: the HDF5 field snapshots, plots, videos, wall tracking.  The
+// library prints nothing (the reference's progress lines go to stdout).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <string>
+#include <vector>
+
+#include "cb_sim.hpp"
+
+using cb::set_error;
+
+namespace {
+
+// std::to_string(num) cut to `prec` decimals (to_precistr, src/main.cpp:63-66)
+std::string to_precistr(double num, int prec)
+{
+    const std::string s = std::to_string(num);
+    return s.substr(0, s.find('.') + prec + 1);
+}
+
+std::string folder_name(const cb_run_opts &o)
+{
+    const cb_params &p = o.params;
+    std::string n = "Sim" + std::to_string(p.D) + "_D_" + "nt_" + std::to_string(o.nt) + "_dt_" + to_precistr(p.dt, 5) +
+                    "_nGrids_" + std::to_string(p.nGrids) + "_L_" + to_precistr(p.L, 1) + "_Mode_" +
+                    std::to_string(p.readxns) + "_nB_" + std::to_string(p.nBubbles) + "_c_" + to_precistr(p.c, 2) +
+                    "_FFT_" + std::to_string(o.fft);
+    if (p.D == 1) n += "_Full1DFieldEvery_" + std::to_string(o.full1d_every);
+    n += "_Mpl_" + std::to_string((int)p.Mpl);
+    return n;
+}
+
+// SnapshotSchedule, src/main.cpp:201-223
+std::vector<int> snapshot_schedule(const cb_run_opts &o)
+{
+    const double dt = o.params.dt, L = o.params.L;
+    const int n = o.snapshots, nt = o.nt;
+    const double T = nt * dt;
+    std::vector<double> times;
+    if (o.snapshot_spacing == 1 && T > L && n >= 2) {
+        const int nEarly = (n + 1) / 2, nLate = n - nEarly;
+        for (int k = 1; k <= nEarly; k++) times.push_back(L * k / nEarly);
+        for (int k = 1; k <= nLate; k++) times.push_back(L + (T - L) * k / nLate);
+    } else {
+        for (int k = 1; k <= n; k++) times.push_back(T * k / n);
+    }
+    std::vector<int> steps;
+    for (double time : times) {
+        int st = (int)std::lround(time / dt);
+        st = st < 1 ? 1 : st;
+        st = st > nt ? nt : st;
+        if (steps.empty() || st > steps.back()) steps.push_back(st);
+    }
+    return steps;
+}
+
+struct File {
+    FILE *f = nullptr;
+    File(const std::string &path, const char *mode) : f(std::fopen(path.c_str(), mode)) {}
+    ~File()
+    {
+        if (f) std::fclose(f);
+    }
+};
+
+cb_status open_fail(const std::string &path)
+{
+    set_error("could not open " + path + " for writing");
+    return CB_INVALID_ARG;
+}
+
+cb_status write_energy_header(const std::string &folder, double c)
+{
+    const std::string path = folder + "/EnergyDensities.dat";
+    File fp(path, "w");
+    if (!fp.f) return open_fail(path);
+    std::fprintf(fp.f, "# c and DeltaV are: %.2f %.4f \n", c, 1.0);
+    std::fprintf(fp.f, "# columns: 1 t, 2 a, 3 Etot, 4 Ekin, 5 Egrad, 6 Epot, 7-10 spread of Etot/Ekin/Egrad/Epot,\n");
+    std::fprintf(fp.f, "#          11 <phi>, 12 spread of <phi>, 13 <phi'>, 14 spread of <phi'>, 15 H, 16 Hubble "
+                       "constraint C = H^2 - rho/(3 Mpl^2)\n");
+    return CB_OK;
+}
+
+cb_status append_energy_row(const std::string &folder, cb_sim *s)
+{
+    cb_energies e;
+    cb_status r = cb_energies_row(s, &e);
+    if (r != CB_OK) return r;
+    const std::string path = folder + "/EnergyDensities.dat";
+    File fp(path, "a");
+    if (!fp.f) return open_fail(path);
+    const double *v = e.col;
+    std::fprintf(fp.f, "%.6e %.10e %.12e %.12e %.12e %.12e %.6e %.6e %.6e %.6e %.6e %.6e %.6e %.6e %.8e %.8e\n", v[0],
+                 v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9], v[10], v[11], v[12], v[13], v[14], v[15]);
+    return CB_OK;
+}
+
+cb_status write_parameters(const std::string &folder, const cb_run_opts &o, const cb_sim *s)
+{
+    const cb_params &p = o.params;
+    const std::string path = folder + "/parameters.dat";
+    File fp(path, "w");
+    if (!fp.f) return open_fail(path);
+    const double dx = (double)p.L / p.nGrids;
+    const double kUV = std::sqrt(double(p.D)) * M_PI / dx;                  // src/bubble.hpp:54
+    const double kIR = 2 * M_PI / double(p.L);                              // :55
+    const double kRStar = std::sqrt(double(p.D)) * 4 * M_PI / (double(p.L - 2 * s->Rn));   // src/bubble.cpp:166
+    const double kMeff = s->mT;                                             // :170
+    std::fprintf(fp.f, "# The data in this folder correspond to the following:\n");
+    std::fprintf(fp.f, " D = %d , L = %.10g,  nGrids = %d , TIME_STEP = %.10g  \n", p.D, p.L, p.nGrids, p.dt);
+    std::fprintf(fp.f, " c = %.10g , nt = %d,  readxns = %d , expansion = %d  \n", p.c, o.nt, p.readxns, p.expansion);
+    std::fprintf(fp.f, " nb = %d , FFT = %d,  ALPHAVAL = %d , WallTracking = %d  \n", p.nBubbles, o.fft, p.alpha, 0);
+    std::fprintf(fp.f, "Mpl = %.10g, memorysave = %d, TEST = %d  \n", p.Mpl, 0, p.test_wave);
+    std::fprintf(fp.f, " Full1DFieldEvery = %d , Trad = %.10g , fixedseed = %d \n", o.full1d_every, p.Trad, 1);
+    std::fprintf(fp.f, " kUV = %.10g , kIR = %.10g , kRStar = %.10g , kMeff = %.10g \n", kUV, kIR, kRStar, kMeff);
+    return CB_OK;
+}
+
+cb_status write_spectrum(const std::string &folder, cb_sim *s, int n)
+{
+    const int cap = (int)(std::sqrt((double)s->p.D) * s->p.nGrids / 2) + 2;
+    std::vector<double> k((size_t)cap), kp((size_t)cap), d((size_t)cap);
+    int nb = 0;
+    cb_status r = cb_power_spectrum(s, k.data(), kp.data(), d.data(), cap, &nb);
+    if (r != CB_OK) return r;
+    cb_state st;
+    r = cb_get_state(s, &st);
+    if (r != CB_OK) return r;
+    char name[64];
+    std::snprintf(name, sizeof name, "/_phi_k_t=%04d.dat", n);
+    const std::string path = folder + name;
+    File fp(path, "w");
+    if (!fp.f) return open_fail(path);
+    // std::ostream's default floating format is "%g" (precision 6)
+    std::fprintf(fp.f, "# t    a        (then one line per |k| bin: k_comoving  k_physical  Delta(k))\n");
+    std::fprintf(fp.f, "%g %g\n", st.t, st.a);
+    for (int b = 0; b < nb && b < cap; ++b) std::fprintf(fp.f, "%g   %g   %g\n", k[(size_t)b], kp[(size_t)b], d[(size_t)b]);
+    return CB_OK;
+}
+
+cb_status append_full_field(const std::string &folder, cb_sim *s)
+{
+    std::vector<double> phi((size_t)s->nsites), pi((size_t)s->nsites);
+    cb_status r = cb_get_fields(s, phi.data(), pi.data());
+    if (r != CB_OK) return r;
+    const std::string path = folder + "/FullField.dat";
+    File fp(path, "a");
+    if (!fp.f) return open_fail(path);
+    for (double v : phi) std::fprintf(fp.f, "%.5f ", v);
+    std::fprintf(fp.f, "\n");
+    return CB_OK;
+}
+
+// Mode 0 centres from <folder>/__xns.dat: fscanf("%lf %lf %lf") until the end
+// or a malformed line (NucleateBubblesAtGivenPoints, src/bubble.cpp:420-453).
+cb_status read_xns(const std::string &folder, std::vector<double> &xyz)
+{
+    const std::string path = folder + "/__xns.dat";
+    File fp(path, "r");
+    if (!fp.f) {
+        set_error("mode 0 needs the bubble positions in " + path + " (one \"x y z\" per line)");
+        return CB_INVALID_ARG;
+    }
+    double v[3];
+    while (std::fscanf(fp.f, "%lf %lf %lf", &v[0], &v[1], &v[2]) == 3) xyz.insert(xyz.end(), v, v + 3);
+    return CB_OK;
+}
+
+struct SimGuard {
+    cb_sim *s = nullptr;
+    ~SimGuard()
+    {
+        if (s) cb_destroy(s);
+    }
+};
+
+} // namespace
+
+extern "C" {
+
+cb_status cb_default_run_opts(cb_run_opts *o)
+{
+    if (!o) { set_error("invalid argument"); return CB_INVALID_ARG; }
+    std::memset(o, 0, sizeof *o);
+    cb_default_params(&o->params);
+    o->nt = 1000;            // nt_default, src/def.h:32
+    o->snapshots = 20;       // nSnapshots, :66
+    o->snapshot_spacing = 0; // SnapshotSpacing, :67
+    o->energy_every = 40;    // OutEnergyEvery, :65
+    o->fft = 1;              // FFT, :70
+    o->full1d_every = 100;   // Full1DFieldEvery, :71
+    o->out_dir = "./";       // Path_default, :63
+    return CB_OK;
+}
+
+cb_status cb_run_folder_name(const cb_run_opts *o, char *buf, int cap)
+{
+    if (!o || !buf || cap < 1) { set_error("invalid argument"); return CB_INVALID_ARG; }
+    const std::string n = folder_name(*o);
+    if ((int)n.size() >= cap) { set_error("buffer too small for the run folder name"); return CB_INVALID_ARG; }
+    std::memcpy(buf, n.c_str(), n.size() + 1);
+    return CB_OK;
+}
+
+cb_status cb_run(cb_run_opts *o)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!o || !o->out_dir) { set_error("invalid argument"); return CB_INVALID_ARG; }
+    if (o->nt < 1 || o->snapshots < 1 || o->energy_every < 1 || o->full1d_every < 1 ||
+        (o->fft != 0 && o->fft != 1) || (o->snapshot_spacing != 0 && o->snapshot_spacing != 1)) {
+        set_error("nt, snapshots, energy_every and full1d_every must be >= 1; fft and snapshot_spacing 0 or 1");
+        return CB_INVALID_ARG;
+    }
+    std::string why;
+    if (!cb::params_ok(&o->params, why)) { set_error(why); return CB_INVALID_ARG; }
+    cb_status r = cb_check_stability(&o->params);
+    if (r != CB_OK) return r;
+    const std::string folder = (std::filesystem::path(o->out_dir) / folder_name(*o)).string();
+    if (folder.size() >= sizeof o->folder) { set_error("run folder path too long"); return CB_INVALID_ARG; }
+    std::error_code ec;
+    std::filesystem::create_directories(folder, ec);
+    if (ec) { set_error("could not create " + folder + ": " + ec.message()); return CB_INVALID_ARG; }
+    std::memcpy(o->folder, folder.c_str(), folder.size() + 1);
+
+    std::vector<double> xyz;
+    const double *xns = nullptr;
+    int n_xns = 0;
+    if (o->params.readxns == 0 && o->params.D >= 2) {
+        if (o->xns) {
+            xns = o->xns;
+            n_xns = o->n_xns;
+        } else {
+            r = read_xns(folder, xyz);
+            if (r != CB_OK) return r;
+            xns = xyz.data();
+            n_xns = (int)(xyz.size() / 3);
+        }
+    }
+    // Bubble s(folder): the ctor writes __xns.dat (mode 1) and the file headers
+    SimGuard g;
+    r = cb_create(&o->params, xns, n_xns, &g.s);
+    if (r != CB_OK) return r;
+    cb_sim *s = g.s;
+    if (o->params.readxns == 1 && o->params.D >= 2 && !o->params.test_wave) {
+        const std::string path = folder + "/__xns.dat";
+        File fp(path, "w");
+        if (!fp.f) return open_fail(path);
+        std::fputs(s->xns_text.c_str(), fp.f);
+    }
+    r = write_energy_header(folder, o->params.c);
+    if (r == CB_OK) r = write_parameters(folder, *o, s);
+    if (r == CB_OK) r = append_energy_row(folder, s);
+    if (r == CB_OK && o->fft) r = write_spectrum(folder, s, 0);
+    if (r == CB_OK && o->params.D == 1) r = append_full_field(folder, s);
+    if (r != CB_OK) return r;
+
+    const std::vector<int> snaps = snapshot_schedule(*o);
+    size_t next = 0;
+    int it = 0;
+    while (it < o->nt) {
+        // next event: an energy row, a snapshot, a 1D field line, the end
+        int ev = o->nt;
+        const int e40 = (it / o->energy_every + 1) * o->energy_every;
+        if (e40 < ev) ev = e40;
+        if (next < snaps.size() && snaps[next] < ev) ev = snaps[next];
+        if (o->params.D == 1) {
+            const int ef = (it / o->full1d_every + 1) * o->full1d_every;
+            if (ef < ev) ev = ef;
+        }
+        r = cb_step(s, ev - it);
+        if (r != CB_OK) return r;
+        it = ev;
+        if (o->params.D == 1 && it % o->full1d_every == 0) r = append_full_field(folder, s);
+        if (r == CB_OK && it % o->energy_every == 0) r = append_energy_row(folder, s);
+        if (r == CB_OK && next < snaps.size() && it == snaps[next]) {
+            ++next;
+            if (o->fft) r = write_spectrum(folder, s, (int)next);
+        }
+        if (r != CB_OK) return r;
+    }
+    o->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const std::string path = folder + "/parameters.dat";
+    File fp(path, "a");
+    if (!fp.f) return open_fail(path);
+    std::fprintf(fp.f, "Elapsed time %f seconds\n", o->elapsed_s);
+    return CB_OK;
+}
+
+} // extern "C"

@@ -38,8 +38,9 @@ namespace {
 // src/bubble.cpp:397-417, src/gridfuncs.cpp:93-119): glibc's rand() stream
 // after srand(seed) (drawn from a private random_r state, so the caller's
 // rand() state is untouched), written as text with "%.5e" / "%.6e" and read
-// back with strtod exactly as the reference's fscanf("%lf") does.
-std::vector<double> generate_xns(int D, double L, double Rcut, int nBubbles, unsigned seed)
+// back with strtod exactly as the reference's fscanf("%lf") does.  The text
+// (the __xns.dat file) goes to *text_out when given.
+std::vector<double> generate_xns(int D, double L, double Rcut, int nBubbles, unsigned seed, std::string *text_out)
 {
     struct random_data rd;
     char state[128];
@@ -88,6 +89,7 @@ std::vector<double> generate_xns(int D, double L, double Rcut, int nBubbles, uns
         if (!ok) break;
         out.insert(out.end(), v, v + 3);
     }
+    if (text_out) *text_out = text;
     return out;
 }
 
@@ -245,7 +247,7 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
         add(3. * L / 4., 3. * L / 4, L / 2., -p.DeltaR, -p.DeltaR, 0.0);
     } else if (p.readxns == 1 || p.readxns == 0) {
         if (p.readxns == 1) {
-            s->xns = generate_xns(D, L, s->Rn + 5 * s->lw, p.nBubbles, p.rand_seed);
+            s->xns = generate_xns(D, L, s->Rn + 5 * s->lw, p.nBubbles, p.rand_seed, &s->xns_text);
         } else {
             if (n_xns < 0 || (n_xns > 0 && !xns)) { set_error("mode 0 needs the bubble centres"); return CB_INVALID_ARG; }
             s->xns.assign(xns, xns + 3 * (size_t)n_xns);

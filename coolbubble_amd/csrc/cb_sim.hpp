@@ -43,6 +43,7 @@ struct cb_sim {
     double t = 0.0;
     long long step = 0;
     std::vector<double> xns;    // centres (the __xns.dat rows)
+    std::string xns_text;       // mode 1: the __xns.dat text as the reference writes it
     cb::FftPlan *fft = nullptr;
     // ctor-time energy row (spreads are zero there, src/bubble.cpp:44)
     bool row_valid = false;

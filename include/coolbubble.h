@@ -232,6 +232,37 @@ cb_status cb_server_drain(cb_server *s);
 cb_status cb_server_variant(const cb_sim *s, int *variant);
 cb_status cb_server_destroy(cb_server *s);
 
+/* ---- Run driver (the time loop of src/main.cpp:229-310) ----
+ * cb_run runs a whole simulation as the reference's main does and writes
+ * its text outputs into <out_dir>/<run folder> (the folder name of
+ * src/main.cpp:243-247): parameters.dat (OutParameters, src/bubble.cpp:
+ * 1018-1032, plus the elapsed-time line), EnergyDensities.dat (header and
+ * outEP rows at t = 0 and every energy_every steps, :958-989), __xns.dat
+ * (mode 1; mode 0 reads it from the run folder unless xns is given,
+ * :397-453), _phi_k_t=NNNN.dat at the initial state and every snapshot
+ * (:642-709), FullField.dat in 1D (:993-1006).  HDF5 field snapshots,
+ * plots, videos and wall tracking are not produced.  Nothing is printed. */
+typedef struct cb_run_opts {
+    cb_params params;
+    int nt;                      /* time steps (nt_default 1000, src/def.h:32) */
+    int snapshots;               /* snapshots after the initial one (nSnapshots 20) */
+    int snapshot_spacing;        /* SnapshotSpacing 0 | 1 (src/def.h:67) */
+    int energy_every;            /* OutEnergyEvery 40 */
+    int fft;                     /* FFT 1: a spectrum file per snapshot, 0: none */
+    int full1d_every;            /* Full1DFieldEvery 100 (1D) */
+    const char *out_dir;         /* folder in which the run folder is created (Path_default "./") */
+    const double *xns;           /* mode 0: n_xns centres (x, y, z); NULL: read <run folder>/__xns.dat */
+    int n_xns;
+    char folder[1024];           /* out: the run folder */
+    double elapsed_s;            /* out: wall time of cb_run */
+} cb_run_opts;
+/* src/def.h defaults (and cb_default_params for params). */
+cb_status cb_default_run_opts(cb_run_opts *o);
+/* The run folder name (no directory part), e.g.
+ * "Sim2_D_nt_1000_dt_0.01250_nGrids_512_L_32.0_Mode_1_nB_5_c_4.00_FFT_1_Mpl_200". */
+cb_status cb_run_folder_name(const cb_run_opts *o, char *buf, int cap);
+cb_status cb_run(cb_run_opts *o);
+
 /* Measurement helper (no reference counterpart): achieved bandwidth of a
  * plain 16-byte-vector device-to-device copy of `bytes` bytes, repeated
  * `reps` times, counting read + write bytes, in GB/s. */

@@ -70,3 +70,28 @@ def test_cfl_violation_rejected_by_create():
 def test_status_strings():
     for code in range(10):
         assert cb.lib().cb_status_string(code)
+
+
+@pytest.mark.parametrize("kw,name", [
+    (dict(D=2, nGrids=512, L=32.0, dt=0.0125, c=4.0, readxns=1, nBubbles=5, nt=1000),
+     "Sim2_D_nt_1000_dt_0.01250_nGrids_512_L_32.0_Mode_1_nB_5_c_4.00_FFT_1_Mpl_200"),
+    (dict(D=3, nGrids=384, L=48.0, dt=0.025, c=2.0, readxns=-3, nt=3000, Mpl=10.0, expansion=1),
+     "Sim3_D_nt_3000_dt_0.02500_nGrids_384_L_48.0_Mode_-3_nB_5_c_2.00_FFT_1_Mpl_10"),
+    (dict(D=1, nGrids=65536, L=4096.0, dt=0.0125, c=7.25, nt=500, fft=0, full1d_every=50),
+     "Sim1_D_nt_500_dt_0.01250_nGrids_65536_L_4096.0_Mode_1_nB_5_c_7.25_FFT_0_Full1DFieldEvery_50_Mpl_200"),
+    (dict(), "Sim2_D_nt_1000_dt_0.00400_nGrids_512_L_32.0_Mode_1_nB_5_c_4.00_FFT_1_Mpl_200"),
+])
+def test_run_folder_name_is_the_references(kw, name):
+    """Run-folder names as src/main.cpp:243-247 builds them (to_precistr cuts
+    std::to_string's six decimals; Mpl is printed as an int)."""
+    assert cb.run_folder_name(**kw) == name
+
+
+def test_run_rejects_bad_options_and_cfl_without_device(tmp_path):
+    with pytest.raises(cb.CBError) as e:
+        cb.run(str(tmp_path), nt=0, D=2, nGrids=64, L=8.0, dt=0.025)
+    assert e.value.status == 1
+    with pytest.raises(cb.CBError) as e:
+        cb.run(str(tmp_path), nt=10, D=2, nGrids=64, L=4.0, dt=0.1)
+    assert e.value.status == 2
+    assert not os.listdir(tmp_path)       # nothing created before the stability check passes

@@ -1,0 +1,164 @@
+"""The run driver (cb_run, src/main.cpp:229-310) writes the reference's run
+folder: file names, line layouts and printf formats as the reference's
+writers (src/bubble.cpp:397-417, :642-709, :958-1032), values as the oracle's
+run of the same configuration (FP64, to the printed precision)."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import coolbubble_amd as cb
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+# printf formats of one outEP row (src/bubble.cpp:980-987): digits after the point
+ROW_DIGITS = [6, 10, 12, 12, 12, 12, 6, 6, 6, 6, 6, 6, 6, 6, 8, 8]
+
+
+def sci(d):
+    return r"-?\d\.\d{%d}e[+-]\d{2}" % d
+
+
+ROW_RE = re.compile("^" + " ".join(sci(d) for d in ROW_DIGITS) + "$")
+
+
+def schedule(nt, dt, n):
+    T = nt * dt
+    out = []
+    for k in range(1, n + 1):
+        st = min(nt, max(1, int(math.floor(T * k / n / dt + 0.5))))
+        if not out or st > out[-1]:
+            out.append(st)
+    return out
+
+
+def oracle_run(kw, nt, snaps, xns=None):
+    """Rows at t = 0 and every 40 steps, spectra (t, a, k, kphys, delta) at
+    the initial state and each snapshot, from the oracle."""
+    o = oracle.Sim(xns=xns, **kw)
+    rows, spectra = [o.energy_row()], [(0.0, 1.0) + o.spectrum()]
+    sched = schedule(nt, kw["dt"], snaps)
+    events = sorted(set(range(40, nt + 1, 40)) | set(sched))
+    done = 0
+    for ev in events:
+        o.step(ev - done)
+        done = ev
+        if ev % 40 == 0:
+            rows.append(o.energy_row())
+        if ev in sched:
+            sc = o.scalars()
+            spectra.append((sc[4], sc[0]) + o.spectrum())
+    return o, np.array(rows), spectra
+
+
+def check_rows(path, want, c):
+    lines = open(path).read().splitlines()
+    assert lines[0] == "# c and DeltaV are: %.2f %.4f " % (c, 1.0)
+    assert lines[1].startswith("# columns: 1 t, 2 a, 3 Etot")
+    assert lines[2].startswith("#          11 <phi>")
+    body = lines[3:]
+    assert len(body) == len(want)
+    got = np.array([[float(x) for x in ln.split()] for ln in body])
+    for ln in body:
+        assert ROW_RE.match(ln), ln
+    for col, d in enumerate(ROW_DIGITS):
+        # "%.de" keeps d + 1 significant digits: half a unit of the last one is
+        # at most 5e-(d+1) x 10 of the value; plus the 1e-12 parity of the values
+        tol = (5.1 * 10.0 ** (-d) + 2e-12) * np.abs(want[:, col])
+        assert np.all(np.abs(got[:, col] - want[:, col]) <= tol), (col, got[:, col], want[:, col])
+
+
+def check_spectrum(path, want):
+    t, a, k, kp, d = want
+    lines = open(path).read().splitlines()
+    assert lines[0] == "# t    a        (then one line per |k| bin: k_comoving  k_physical  Delta(k))"
+    assert lines[1] == "%g %g" % (t, a)
+    body = [ln.split("   ") for ln in lines[2:]]
+    assert len(body) == len(k)
+    assert all(len(row) == 3 for row in body)       # "k   k/a   Delta", std::ostream "%g" numbers
+    got = np.array([[float(x) for x in row] for row in body])
+    np.testing.assert_allclose(got[:, 0], k, rtol=1e-5)
+    np.testing.assert_allclose(got[:, 1], kp, rtol=1e-5)
+    np.testing.assert_allclose(got[:, 2], d, rtol=1e-5, atol=1e-5 * np.max(d))
+
+
+def test_2d_mode1_run_folder(tmp_path):
+    kw = dict(D=2, nGrids=64, L=8.0, c=4.0, readxns=1, nBubbles=5, dt=0.025)
+    nt, snaps = 200, 4
+    folder, elapsed = cb.run(str(tmp_path), nt=nt, snapshots=snaps, precision="fp64", **kw)
+    assert os.path.basename(folder) == "Sim2_D_nt_200_dt_0.02500_nGrids_64_L_8.0_Mode_1_nB_5_c_4.00_FFT_1_Mpl_200"
+    assert elapsed > 0
+    o, rows, spectra = oracle_run(kw, nt, snaps)
+    names = sorted(os.listdir(folder))
+    assert names == sorted(["parameters.dat", "EnergyDensities.dat", "__xns.dat"]
+                           + ["_phi_k_t=%04d.dat" % n for n in range(snaps + 1)])
+    check_rows(os.path.join(folder, "EnergyDensities.dat"), rows, kw["c"])
+    for n in range(snaps + 1):
+        check_spectrum(os.path.join(folder, "_phi_k_t=%04d.dat" % n), spectra[n])
+    # __xns.dat: centres "%.5e", periodic images "%.6e", read back as the oracle's
+    xl = open(os.path.join(folder, "__xns.dat")).read().splitlines()
+    for ln in xl:
+        assert re.match(r"^(%s %s %s|%s %s %s)$" % ((sci(5),) * 3 + (sci(6),) * 3), ln), ln
+    np.testing.assert_array_equal(np.loadtxt(os.path.join(folder, "__xns.dat")), o.xns())
+    # parameters.dat: OutParameters lines, then the elapsed time
+    pl = open(os.path.join(folder, "parameters.dat")).read().splitlines()
+    assert pl[0] == "# The data in this folder correspond to the following:"
+    assert pl[1] == " D = 2 , L = 8,  nGrids = 64 , TIME_STEP = 0.025  "
+    assert pl[2] == " c = 4 , nt = 200,  readxns = 1 , expansion = 0  "
+    assert pl[3] == " nb = 5 , FFT = 1,  ALPHAVAL = 0 , WallTracking = 0  "
+    assert pl[4] == "Mpl = 200, memorysave = 0, TEST = 0  "
+    assert pl[5] == " Full1DFieldEvery = 100 , Trad = 0.2 , fixedseed = 1 "
+    kv = dict(re.findall(r"(\w+) = ([-+0-9.eE]+)", pl[6]))
+    dx = 8.0 / 64
+    fit = oracle.bubble_fit(2, 4.0)
+    assert float(kv["kUV"]) == pytest.approx(math.sqrt(2) * math.pi / dx, rel=1e-9)
+    assert float(kv["kIR"]) == pytest.approx(2 * math.pi / 8.0, rel=1e-9)
+    assert float(kv["kRStar"]) == pytest.approx(math.sqrt(2) * 4 * math.pi / (8.0 - 2 * fit[1]), rel=1e-9)
+    assert float(kv["kMeff"]) == pytest.approx(math.sqrt(2 * (4.0 + 6)), rel=1e-9)
+    assert re.match(r"^Elapsed time \d+\.\d{6} seconds$", pl[7])
+    assert len(pl) == 8
+
+
+def test_mode0_reads_xns_from_the_run_folder(tmp_path):
+    kw = dict(D=3, nGrids=32, L=4.0, c=2.0, readxns=0, nBubbles=5, dt=0.025)
+    xyz = np.array([[1.0, 1.5, 2.0], [3.2, 0.4, 1.1], [2.0, 2.0, 3.9]])
+    nt, snaps = 80, 2
+    name = cb.run_folder_name(nt=nt, snapshots=snaps, **kw)
+    os.makedirs(tmp_path / name)
+    with open(tmp_path / name / "__xns.dat", "w") as f:
+        for x in xyz:
+            f.write("%.5e %.5e %.5e\n" % tuple(x))
+    folder, _ = cb.run(str(tmp_path), nt=nt, snapshots=snaps, precision="fp64", **kw)
+    assert folder == str(tmp_path / name)
+    _, rows, spectra = oracle_run(kw, nt, snaps, xns=np.loadtxt(tmp_path / name / "__xns.dat"))
+    check_rows(os.path.join(folder, "EnergyDensities.dat"), rows, kw["c"])
+    check_spectrum(os.path.join(folder, "_phi_k_t=0002.dat"), spectra[2])
+    # without the file, mode 0 is an error (the reference exits, src/bubble.cpp:424-428)
+    with pytest.raises(cb.CBError):
+        cb.run(str(tmp_path / "empty"), nt=nt, snapshots=snaps, precision="fp64", **kw)
+
+
+def test_1d_full_field_lines(tmp_path):
+    kw = dict(D=1, nGrids=256, L=16.0, c=4.0, dt=0.0125)
+    nt = 250
+    folder, _ = cb.run(str(tmp_path), nt=nt, snapshots=2, precision="fp64", **kw)
+    assert "_Full1DFieldEvery_100_" in os.path.basename(folder)
+    lines = open(os.path.join(folder, "FullField.dat")).read().splitlines()
+    assert len(lines) == 1 + nt // 100          # t = 0, then steps 100 and 200
+    o = oracle.Sim(**kw)
+    want = [o.phi().copy()]
+    o.step(100)
+    want.append(o.phi().copy())
+    o.step(100)
+    want.append(o.phi().copy())
+    for ln, w in zip(lines, want):
+        vals = ln.split(" ")
+        assert vals[-1] == ""                   # every value is followed by a space
+        assert all(re.match(r"^-?\d+\.\d{5}$", v) for v in vals[:-1])
+        got = np.array([float(v) for v in vals[:-1]])
+        assert got.shape == (256,)
+        assert np.all(np.abs(got - w) <= 5.1e-6)
+    assert not os.path.exists(os.path.join(folder, "__xns.dat"))
