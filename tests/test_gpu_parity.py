@@ -110,6 +110,10 @@ THERMAL = {
     "3d_static": params(D=3, readxns=-3, nGrids=32, L=4.0, c=2.0, Trad=0.2),
     "3d_expanding": params(D=3, readxns=-3, nGrids=32, L=4.0, c=2.0, Trad=0.2, expansion=1, Mpl=10.0),
     "3d_expanding_n24": params(D=3, readxns=-3, nGrids=24, L=3.0, c=2.0, Trad=0.3, expansion=1, Mpl=10.0),
+    # conformal time, ALPHAVAL = 1 (src/def.h:57, src/bubble.cpp:741-755)
+    "3d_expanding_conformal": params(D=3, readxns=-3, nGrids=32, L=4.0, c=2.0, Trad=0.2, expansion=1, Mpl=10.0,
+                                     alpha=1),
+    "3d_bubble_expanding_conformal": params(D=3, readxns=-2, nGrids=32, L=4.0, expansion=1, Mpl=10.0, alpha=1),
 }
 
 

@@ -162,3 +162,18 @@ def test_1d_full_field_lines(tmp_path):
         assert got.shape == (256,)
         assert np.all(np.abs(got - w) <= 5.1e-6)
     assert not os.path.exists(os.path.join(folder, "__xns.dat"))
+
+
+def test_snapshot_spacing_1_front_loads_the_snapshots(tmp_path):
+    """SnapshotSpacing = 1 (src/main.cpp:205-210): half the snapshots up to
+    t = L, the rest spread to the end (here T = 10 > L = 8: t = 4, 8, 9, 10)."""
+    kw = dict(D=2, nGrids=64, L=8.0, c=4.0, readxns=-2, dt=0.025)
+    folder, _ = cb.run(str(tmp_path), nt=400, snapshots=4, snapshot_spacing=1, precision="fp32", **kw)
+    ts = []
+    for n in range(1, 5):
+        with open(os.path.join(folder, "_phi_k_t=%04d.dat" % n)) as f:
+            f.readline()
+            ts.append(float(f.readline().split()[0]))
+    np.testing.assert_allclose(ts, [4.0, 8.0, 9.0, 10.0], rtol=1e-5)
+    rows = np.loadtxt(os.path.join(folder, "EnergyDensities.dat"))
+    assert rows.shape == (11, 16)                # t = 0 and every 40 steps

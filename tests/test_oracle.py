@@ -243,6 +243,24 @@ def test_de_sitter_scale_factor_closed_form():
     assert row[3] == 0.0 and row[4] == 0.0 and abs(row[5] - 1.0) < 1e-14
 
 
+def test_conformal_time_de_sitter_converges_to_the_closed_form():
+    """Conformal time (ALPHAVAL = 1, src/bubble.cpp:751-755): false vacuum at
+    rest gives a'' = 2 a^3 V / (3 Mpl^2), whose solution from a(0) = 1,
+    a'(0) = H0 is a(tau) = 1 / (1 - H0 tau).  The leapfrog of a approaches it
+    at second order in dt (a wrong power of a would miss it)."""
+    Mpl, tau = 10.0, 10.0
+    H0 = math.sqrt(1.0 / (3 * Mpl ** 2))
+    exact = 1.0 / (1.0 - H0 * tau)
+    errs = []
+    for dt in (0.05, 0.025):
+        s = oracle.Sim(readxns=0, xns=np.zeros((0, 3)), D=3, nGrids=4, L=1.0, dt=dt, c=2.0,
+                       expansion=1, Mpl=Mpl, alpha=1)
+        s.step(int(round(tau / dt)))
+        errs.append(abs(s.energy_row()[1] - exact) / exact)
+    assert errs[0] < 1e-3
+    assert 3.5 < errs[0] / errs[1] < 4.5, errs
+
+
 def test_gradient_energy_of_a_plane_wave():
     """Forward-difference gradient energy (src/bubble.cpp:790-808) of
     A cos(2 pi m i / N) along one axis: A^2 sin^2(pi m / N) / dx^2."""
