@@ -134,7 +134,12 @@ def lib():
         L.cb_default_run_opts.argtypes = [vp]
         L.cb_run_folder_name.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
         L.cb_run.argtypes = [vp]
-        for name in ("cb_default_run_opts", "cb_run_folder_name", "cb_run"):
+        L.cb_hdf5_available.argtypes = []
+        L.cb_write_snapshot_h5.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.POINTER(ctypes.c_float)), ctypes.c_int]
+        for name in ("cb_default_run_opts", "cb_run_folder_name", "cb_run", "cb_hdf5_available",
+                     "cb_write_snapshot_h5"):
             getattr(L, name).restype = ctypes.c_int
         L.cb_nsites.argtypes = [vp]
         L.cb_nsites.restype = ctypes.c_longlong
@@ -175,17 +180,17 @@ class RunOpts(ctypes.Structure):
     """cb_run_opts (include/coolbubble.h)."""
     _fields_ = [("params", Params), ("nt", ctypes.c_int), ("snapshots", ctypes.c_int),
                 ("snapshot_spacing", ctypes.c_int), ("energy_every", ctypes.c_int), ("fft", ctypes.c_int),
-                ("full1d_every", ctypes.c_int), ("out_dir", ctypes.c_char_p),
+                ("full1d_every", ctypes.c_int), ("field_snapshots", ctypes.c_int), ("memorysave", ctypes.c_int),
+                ("deflate", ctypes.c_int), ("out_dir", ctypes.c_char_p),
                 ("xns", ctypes.POINTER(ctypes.c_double)), ("n_xns", ctypes.c_int),
                 ("folder", ctypes.c_char * 1024), ("elapsed_s", ctypes.c_double)]
 
 
-def _run_opts(out_dir, nt, snapshots, snapshot_spacing, energy_every, fft, full1d_every, params):
+def _run_opts(out_dir, params, **opts):
     o = RunOpts()
     check(lib().cb_default_run_opts(ctypes.byref(o)))
     o.params = params
-    for key, val in (("nt", nt), ("snapshots", snapshots), ("snapshot_spacing", snapshot_spacing),
-                     ("energy_every", energy_every), ("fft", fft), ("full1d_every", full1d_every)):
+    for key, val in opts.items():
         if val is not None:
             setattr(o, key, int(val))
     o.out_dir = os.fsencode(out_dir)
@@ -194,19 +199,21 @@ def _run_opts(out_dir, nt, snapshots, snapshot_spacing, energy_every, fft, full1
 
 def run_folder_name(nt=None, snapshots=None, fft=None, full1d_every=None, **kw):
     """The reference's run-folder name for these options (src/main.cpp:243-247)."""
-    o = _run_opts(".", nt, snapshots, None, None, fft, full1d_every, default_params(**kw))
+    o = _run_opts(".", default_params(**kw), nt=nt, snapshots=snapshots, fft=fft, full1d_every=full1d_every)
     buf = ctypes.create_string_buffer(1024)
     check(lib().cb_run_folder_name(ctypes.byref(o), buf, 1024))
     return buf.value.decode()
 
 
 def run(out_dir, nt=None, snapshots=None, snapshot_spacing=None, energy_every=None, fft=None,
-        full1d_every=None, xns=None, **kw):
+        full1d_every=None, field_snapshots=None, memorysave=None, deflate=None, xns=None, **kw):
     """The reference's whole run (src/main.cpp:229-310): creates the run folder
     under out_dir, writes parameters.dat, EnergyDensities.dat, __xns.dat,
-    _phi_k_t=NNNN.dat and (1D) FullField.dat.  Returns (folder, elapsed s)."""
-    o = _run_opts(out_dir, nt, snapshots, snapshot_spacing, energy_every, fft, full1d_every,
-                  default_params(**kw))
+    _phi_k_t=NNNN.dat, _phi_t=NNNN.h5 (field_snapshots) and (1D)
+    FullField.dat.  Returns (folder, elapsed s)."""
+    o = _run_opts(out_dir, default_params(**kw), nt=nt, snapshots=snapshots, snapshot_spacing=snapshot_spacing,
+                  energy_every=energy_every, fft=fft, full1d_every=full1d_every,
+                  field_snapshots=field_snapshots, memorysave=memorysave, deflate=deflate)
     keep = None
     if xns is not None:
         keep = np.ascontiguousarray(np.asarray(xns, dtype=np.float64).reshape(-1, 3))
@@ -214,6 +221,38 @@ def run(out_dir, nt=None, snapshots=None, snapshot_spacing=None, energy_every=No
         o.n_xns = keep.shape[0]
     check(lib().cb_run(ctypes.byref(o)))
     return o.folder.decode(), o.elapsed_s
+
+
+def hdf5_available():
+    """True when the HDF5 C library loads (the field snapshots need it)."""
+    return lib().cb_hdf5_available() == 0
+
+
+def snapshot_points(D, nGrids):
+    """(reduc, n): the thinning step and saved points per axis of a snapshot
+    (src/bubble.cpp:891-894)."""
+    reduc = 1
+    if D == 2 and nGrids > 2048:
+        reduc = nGrids // 2048
+    if D == 3 and nGrids > 512:
+        reduc = nGrids // 512
+    return reduc, (nGrids + reduc - 1) // reduc
+
+
+def write_snapshot_h5(path, fields, D, nGrids, L, c, t, a, deflate=9):
+    """Bubble::OutField's file (src/bubble.cpp:887-954) from host arrays:
+    fields = [phi, pi] or [phi, pi, Ekin, gradphi, potEnergy], each n^D values
+    of the saved points in lattice order."""
+    if len(fields) not in (2, 5):
+        raise ValueError("fields must hold 2 or 5 arrays")
+    _, n = snapshot_points(D, nGrids)
+    keep = [np.ascontiguousarray(np.asarray(f, dtype=np.float32).reshape(-1)) for f in fields]
+    for f in keep:
+        if f.size != n ** D:
+            raise ValueError("each field must hold %d values" % n ** D)
+    fp = ctypes.POINTER(ctypes.c_float)
+    arr = (fp * len(keep))(*[f.ctypes.data_as(fp) for f in keep])
+    check(lib().cb_write_snapshot_h5(os.fsencode(path), D, nGrids, L, c, t, a, len(keep), arr, deflate))
 
 
 def bandwidth_probe(nbytes=1 << 30, reps=20, device=0):

@@ -14,17 +14,24 @@
 //                        the initial state and every snapshot
 //   FullField.dat        1D: outFullField (:993-1006) at t = 0 and every
 //                        Full1DFieldEvery steps
-// Not written: the HDF5 field snapshots, plots, videos, wall tracking.  The
-// library prints nothing (the reference's progress lines go to stdout).
+//   _phi_t=NNNN.h5       OutField (:883-954) with every spectrum: the device
+//                        packs the saved points into shuffled chunk images, a
+//                        side stream copies them to pinned host memory and a
+//                        host thread deflates and writes the file while the
+//                        time steps go on
+// Not written: plots, videos, wall tracking.  The library prints nothing (the
+// reference's progress lines go to stdout).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <filesystem>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "cb_sim.hpp"
+#include "cb_snapshot.hpp"
 
 using cb::set_error;
 
@@ -129,7 +136,7 @@ cb_status write_parameters(const std::string &folder, const cb_run_opts &o, cons
     std::fprintf(fp.f, " D = %d , L = %.10g,  nGrids = %d , TIME_STEP = %.10g  \n", p.D, p.L, p.nGrids, p.dt);
     std::fprintf(fp.f, " c = %.10g , nt = %d,  readxns = %d , expansion = %d  \n", p.c, o.nt, p.readxns, p.expansion);
     std::fprintf(fp.f, " nb = %d , FFT = %d,  ALPHAVAL = %d , WallTracking = %d  \n", p.nBubbles, o.fft, p.alpha, 0);
-    std::fprintf(fp.f, "Mpl = %.10g, memorysave = %d, TEST = %d  \n", p.Mpl, 0, p.test_wave);
+    std::fprintf(fp.f, "Mpl = %.10g, memorysave = %d, TEST = %d  \n", p.Mpl, o.memorysave, p.test_wave);
     std::fprintf(fp.f, " Full1DFieldEvery = %d , Trad = %.10g , fixedseed = %d \n", o.full1d_every, p.Trad, 1);
     std::fprintf(fp.f, " kUV = %.10g , kIR = %.10g , kRStar = %.10g , kMeff = %.10g \n", kUV, kIR, kRStar, kMeff);
     return CB_OK;
@@ -185,6 +192,137 @@ cb_status read_xns(const std::string &folder, std::vector<double> &xyz)
     return CB_OK;
 }
 
+#define RUN_HIP(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                \
+            return CB_HIP_ERROR;                                                         \
+        }                                                                                \
+    } while (0)
+
+// The HDF5 field snapshots of one run.  The energy maps of a snapshot after
+// step n are those of the last energy measurement, which saw phi at step
+// n - 1 and pi after that step's kick (src/bubble.cpp:761-858), so the run
+// keeps phi and the device scalars of step n - 1 (save_map_state) before its
+// last step to a snapshot; the initial snapshot's maps are the ctor's
+// (src/bubble.cpp:239-265): phi at t = 0, pi before the back-step, a = 1.
+struct Snapshots {
+    cb_sim *s = nullptr;
+    cb::SnapGeom g{};
+    int nfields = 2, deflate = 9;
+    std::string folder;
+    unsigned char *dimg = nullptr;      // device chunk images
+    unsigned char *himg = nullptr;      // pinned host copy
+    size_t bytes = 0;
+    void *phim = nullptr;               // phi of the last energy measurement
+    cb::DevScalars *msc = nullptr;      // its factors (a dx, a_kin^(-2D))
+    hipStream_t cst = nullptr;          // copy stream
+    hipEvent_t packed = nullptr, copied = nullptr;
+    std::thread writer;
+    cb_status wst = CB_OK;
+    std::string werr;
+
+    bool maps() const { return nfields > 2; }
+
+    cb_status init(cb_sim *sim, const cb_run_opts &o, const std::string &dir)
+    {
+        s = sim;
+        folder = dir;
+        g = cb::snap_geom(o.params.D, o.params.nGrids);
+        nfields = o.memorysave == 0 ? cb::kSnapFieldsMax : 2;
+        deflate = o.deflate;
+        bytes = (size_t)nfields * g.dataset_bytes();
+        RUN_HIP(hipMalloc((void **)&dimg, bytes));
+        RUN_HIP(hipHostMalloc((void **)&himg, bytes, hipHostMallocDefault));
+        if (maps()) {
+            RUN_HIP(hipMalloc(&phim, s->esz * (size_t)s->nsites));
+            RUN_HIP(hipMalloc((void **)&msc, sizeof(cb::DevScalars)));
+        }
+        RUN_HIP(hipStreamCreateWithFlags(&cst, hipStreamNonBlocking));
+        RUN_HIP(hipEventCreateWithFlags(&packed, hipEventDisableTiming));
+        RUN_HIP(hipEventCreateWithFlags(&copied, hipEventDisableTiming));
+        return CB_OK;
+    }
+
+    // Called with the run at step n - 1 before its last step to snapshot n.
+    cb_status save_map_state()
+    {
+        if (!maps()) return CB_OK;
+        cb_status r = cb::materialise_drift(s);
+        if (r != CB_OK) return r;
+        RUN_HIP(hipMemcpyAsync(phim, s->phi[s->iphi], s->esz * (size_t)s->nsites, hipMemcpyDeviceToDevice, s->st));
+        RUN_HIP(hipMemcpyAsync(msc, s->dsc, sizeof(cb::DevScalars), hipMemcpyDeviceToDevice, s->st));
+        return CB_OK;
+    }
+
+    // Waits for the file being written; its status.
+    cb_status wait()
+    {
+        if (writer.joinable()) writer.join();
+        if (wst != CB_OK) set_error(werr);
+        return wst;
+    }
+
+    // Snapshot n of the current state.
+    cb_status write(int n)
+    {
+        cb_status r = wait();   // the images and the pinned buffer are free again
+        if (r != CB_OK) return r;
+        r = cb::materialise_drift(s);
+        if (r != CB_OK) return r;
+        const void *phi = s->phi[s->iphi], *pi = s->pi[s->ipi];
+        const void *phi_m = phim, *pi_m = pi;
+        cb::DevScalars h0{};
+        if (maps() && n == 0) {
+            if (!s->pi0) { set_error("initial pi was not kept"); return CB_INVALID_ARG; }
+            h0.adx = 1.0 * s->k.dx;     // a = 1 in the ctor
+            h0.pk2 = 1.0;
+            RUN_HIP(hipMemcpyAsync(msc, &h0, sizeof h0, hipMemcpyHostToDevice, s->st));
+            phi_m = phi;
+            pi_m = s->pi0;
+        }
+        RUN_HIP(cb::launch_snapshot_pack(s->k, s->fp32, g, nfields, phi, pi, phi_m, pi_m, msc, dimg, s->st));
+        RUN_HIP(hipEventRecord(packed, s->st));
+        RUN_HIP(hipStreamWaitEvent(cst, packed, 0));
+        RUN_HIP(hipMemcpyAsync(himg, dimg, bytes, hipMemcpyDeviceToHost, cst));
+        RUN_HIP(hipEventRecord(copied, cst));
+        cb::DevScalars h;
+        r = cb::sim_scalars(s, h);
+        if (r != CB_OK) return r;
+        if (n == 0 && s->pi0) {
+            hipFree(s->pi0);
+            s->pi0 = nullptr;
+        }
+        char name[32];
+        std::snprintf(name, sizeof name, "/_phi_t=%04d.h5", n);
+        cb::SnapMeta m{folder + name, s->p.D, s->p.nGrids, s->p.L, s->p.c, s->t, h.a, nfields, deflate};
+        writer = std::thread([this, m]() {
+            const hipError_t e = hipEventSynchronize(copied);
+            if (e != hipSuccess) {
+                werr = std::string("snapshot copy: ") + hipGetErrorString(e);
+                wst = CB_HIP_ERROR;
+                return;
+            }
+            wst = cb::h5_write_images(m, g, himg, werr);
+        });
+        return CB_OK;
+    }
+
+    ~Snapshots()
+    {
+        if (writer.joinable()) writer.join();
+        if (cst) hipStreamSynchronize(cst);
+        if (copied) hipEventDestroy(copied);
+        if (packed) hipEventDestroy(packed);
+        if (cst) hipStreamDestroy(cst);
+        if (dimg) hipFree(dimg);
+        if (himg) hipHostFree(himg);
+        if (phim) hipFree(phim);
+        if (msc) hipFree(msc);
+    }
+};
+
 struct SimGuard {
     cb_sim *s = nullptr;
     ~SimGuard()
@@ -208,6 +346,9 @@ cb_status cb_default_run_opts(cb_run_opts *o)
     o->energy_every = 40;    // OutEnergyEvery, :65
     o->fft = 1;              // FFT, :70
     o->full1d_every = 100;   // Full1DFieldEvery, :71
+    o->field_snapshots = 1;  // OutField at every snapshot (src/main.cpp:192-198)
+    o->memorysave = 0;       // memorysave, :64
+    o->deflate = 9;          // setDeflate(9), src/bubble.cpp:928
     o->out_dir = "./";       // Path_default, :63
     return CB_OK;
 }
@@ -226,9 +367,19 @@ cb_status cb_run(cb_run_opts *o)
     const auto t0 = std::chrono::steady_clock::now();
     if (!o || !o->out_dir) { set_error("invalid argument"); return CB_INVALID_ARG; }
     if (o->nt < 1 || o->snapshots < 1 || o->energy_every < 1 || o->full1d_every < 1 ||
-        (o->fft != 0 && o->fft != 1) || (o->snapshot_spacing != 0 && o->snapshot_spacing != 1)) {
-        set_error("nt, snapshots, energy_every and full1d_every must be >= 1; fft and snapshot_spacing 0 or 1");
+        (o->fft != 0 && o->fft != 1) || (o->snapshot_spacing != 0 && o->snapshot_spacing != 1) ||
+        (o->field_snapshots != 0 && o->field_snapshots != 1) || (o->memorysave != 0 && o->memorysave != 1) ||
+        o->deflate < 0 || o->deflate > 9) {
+        set_error("nt, snapshots, energy_every and full1d_every must be >= 1; fft, snapshot_spacing, "
+                  "field_snapshots and memorysave 0 or 1; deflate 0..9");
         return CB_INVALID_ARG;
+    }
+    if (o->field_snapshots) {
+        std::string why;
+        if (!cb::h5_available(why)) {
+            set_error("field snapshots: " + why);
+            return CB_UNSUPPORTED;
+        }
     }
     std::string why;
     if (!cb::params_ok(&o->params, why)) { set_error(why); return CB_INVALID_ARG; }
@@ -257,9 +408,15 @@ cb_status cb_run(cb_run_opts *o)
     }
     // Bubble s(folder): the ctor writes __xns.dat (mode 1) and the file headers
     SimGuard g;
-    r = cb_create(&o->params, xns, n_xns, &g.s);
+    const bool maps = o->field_snapshots && o->memorysave == 0;
+    r = cb::create_sim(&o->params, xns, n_xns, maps, &g.s);
     if (r != CB_OK) return r;
     cb_sim *s = g.s;
+    Snapshots snap;   // destroyed before the simulation: its writer reads nothing of it
+    if (o->field_snapshots) {
+        r = snap.init(s, *o, folder);
+        if (r != CB_OK) return r;
+    }
     if (o->params.readxns == 1 && o->params.D >= 2 && !o->params.test_wave) {
         const std::string path = folder + "/__xns.dat";
         File fp(path, "w");
@@ -269,6 +426,7 @@ cb_status cb_run(cb_run_opts *o)
     r = write_energy_header(folder, o->params.c);
     if (r == CB_OK) r = write_parameters(folder, *o, s);
     if (r == CB_OK) r = append_energy_row(folder, s);
+    if (r == CB_OK && o->field_snapshots) r = snap.write(0);
     if (r == CB_OK && o->fft) r = write_spectrum(folder, s, 0);
     if (r == CB_OK && o->params.D == 1) r = append_full_field(folder, s);
     if (r != CB_OK) return r;
@@ -286,15 +444,26 @@ cb_status cb_run(cb_run_opts *o)
             const int ef = (it / o->full1d_every + 1) * o->full1d_every;
             if (ef < ev) ev = ef;
         }
-        r = cb_step(s, ev - it);
+        if (o->field_snapshots && next < snaps.size() && ev == snaps[next]) {
+            r = cb_step(s, ev - 1 - it);
+            if (r == CB_OK) r = snap.save_map_state();
+            if (r == CB_OK) r = cb_step(s, 1);
+        } else {
+            r = cb_step(s, ev - it);
+        }
         if (r != CB_OK) return r;
         it = ev;
         if (o->params.D == 1 && it % o->full1d_every == 0) r = append_full_field(folder, s);
         if (r == CB_OK && it % o->energy_every == 0) r = append_energy_row(folder, s);
         if (r == CB_OK && next < snaps.size() && it == snaps[next]) {
             ++next;
-            if (o->fft) r = write_spectrum(folder, s, (int)next);
+            if (o->field_snapshots) r = snap.write((int)next);
+            if (r == CB_OK && o->fft) r = write_spectrum(folder, s, (int)next);
         }
+        if (r != CB_OK) return r;
+    }
+    if (o->field_snapshots) {
+        r = snap.wait();
         if (r != CB_OK) return r;
     }
     o->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

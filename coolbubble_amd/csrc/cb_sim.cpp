@@ -126,6 +126,7 @@ void release(cb_sim *s)
         if (s->phi[i]) hipFree(s->phi[i]);
         if (s->pi[i]) hipFree(s->pi[i]);
     }
+    if (s->pi0) hipFree(s->pi0);
     if (s->partials) hipFree(s->partials);
     if (s->dtot) hipFree(s->dtot);
     if (s->dsc) hipFree(s->dsc);
@@ -277,6 +278,13 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
         if (e != hipSuccess) { set_error(std::string("seeding: ") + hipGetErrorString(e)); return CB_HIP_ERROR; }
     }
 
+    // pi at t = 0, before the back-step below: the kinetic-energy map of the
+    // initial snapshot reads it (src/bubble.cpp:253-261)
+    if (s->keep_pi0) {
+        HIP_TRY(hipMalloc(&s->pi0, s->esz * (size_t)s->nsites));
+        HIP_TRY(hipMemcpyAsync(s->pi0, s->pi[0], s->esz * (size_t)s->nsites, hipMemcpyDeviceToDevice, s->st));
+    }
+
     // initial energies (src/bubble.cpp:239-269)
     double tot[cb::NQ];
     HIP_TRY(cb::launch_ctor_energy(k, s->fp32, s->phi[0], s->pi[0], s->partials, s->cfg, s->st));
@@ -336,6 +344,29 @@ cb_status materialise(cb_sim *s)
 } // namespace
 
 namespace cb {
+cb_status create_sim(const cb_params *p, const double *xns, int n_xns, bool keep_pi0, cb_sim **out)
+{
+    if (!out) { set_error("out is NULL"); return CB_INVALID_ARG; }
+    *out = nullptr;
+    std::string why;
+    if (!valid_params(p, why)) { set_error(why); return CB_INVALID_ARG; }
+    cb_status r = cb_check_stability(p);
+    if (r != CB_OK) return r;
+    cb_sim *s = new (std::nothrow) cb_sim();
+    if (!s) { set_error("host allocation failed"); return CB_OUT_OF_MEMORY; }
+    s->p = *p;
+    s->keep_pi0 = keep_pi0;
+    r = construct(s, xns, n_xns);
+    if (r != CB_OK) {
+        if (r == CB_HIP_ERROR && hipGetLastError() == hipErrorOutOfMemory) r = CB_OUT_OF_MEMORY;
+        release(s);
+        return r;
+    }
+    *out = s;
+    return CB_OK;
+}
+cb_status materialise_drift(cb_sim *s) { return materialise(s); }
+cb_status sim_scalars(cb_sim *s, DevScalars &h) { return read_scalars(s, h); }
 cb_status construct_sim(cb_sim *s, const double *xns, int n_xns) { return construct(s, xns, n_xns); }
 void release_sim(cb_sim *s) { release(s); }
 bool params_ok(const cb_params *p, std::string &why) { return valid_params(p, why); }
@@ -380,23 +411,7 @@ cb_status cb_check_stability(const cb_params *p)
 
 cb_status cb_create(const cb_params *p, const double *xns, int n_xns, cb_sim **out)
 {
-    if (!out) { set_error("out is NULL"); return CB_INVALID_ARG; }
-    *out = nullptr;
-    std::string why;
-    if (!valid_params(p, why)) { set_error(why); return CB_INVALID_ARG; }
-    cb_status r = cb_check_stability(p);
-    if (r != CB_OK) return r;
-    cb_sim *s = new (std::nothrow) cb_sim();
-    if (!s) { set_error("host allocation failed"); return CB_OUT_OF_MEMORY; }
-    s->p = *p;
-    r = construct(s, xns, n_xns);
-    if (r != CB_OK) {
-        if (r == CB_HIP_ERROR && hipGetLastError() == hipErrorOutOfMemory) r = CB_OUT_OF_MEMORY;
-        release(s);
-        return r;
-    }
-    *out = s;
-    return CB_OK;
+    return cb::create_sim(p, xns, n_xns, false, out);
 }
 
 cb_status cb_step(cb_sim *s, int nsteps)

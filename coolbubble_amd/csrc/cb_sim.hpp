@@ -45,6 +45,8 @@ struct cb_sim {
     std::vector<double> xns;    // centres (the __xns.dat rows)
     std::string xns_text;       // mode 1: the __xns.dat text as the reference writes it
     cb::FftPlan *fft = nullptr;
+    bool keep_pi0 = false;      // keep pi at t = 0 (before the back-step) in pi0
+    void *pi0 = nullptr;
     // ctor-time energy row (spreads are zero there, src/bubble.cpp:44)
     bool row_valid = false;
 };
@@ -53,6 +55,12 @@ namespace cb {
 // spectrum of the current phi (cb_fft.hip); bins filled with the reference's
 // non-empty-shell convention
 cb_status step_n(cb_sim *s, int nsteps);
+// cb_create, optionally keeping pi at t = 0 (the initial snapshot's kinetic map)
+cb_status create_sim(const cb_params *p, const double *xns, int n_xns, bool keep_pi0, cb_sim **out);
+// apply a lagged drift still pending on phi (expanding runs)
+cb_status materialise_drift(cb_sim *s);
+// the device scalars of s (synchronises its stream)
+cb_status sim_scalars(cb_sim *s, DevScalars &h);
 cb_status construct_sim(cb_sim *s, const double *xns, int n_xns);
 void release_sim(cb_sim *s);
 bool params_ok(const cb_params *p, std::string &why);

@@ -240,8 +240,11 @@ cb_status cb_server_destroy(cb_server *s);
  * outEP rows at t = 0 and every energy_every steps, :958-989), __xns.dat
  * (mode 1; mode 0 reads it from the run folder unless xns is given,
  * :397-453), _phi_k_t=NNNN.dat at the initial state and every snapshot
- * (:642-709), FullField.dat in 1D (:993-1006).  HDF5 field snapshots,
- * plots, videos and wall tracking are not produced.  Nothing is printed. */
+ * (:642-709), FullField.dat in 1D (:993-1006), and (field_snapshots = 1)
+ * the HDF5 field snapshots _phi_t=NNNN.h5 at the same times (OutField,
+ * :883-954; see cb_write_snapshot_h5).  The snapshot files are written by a
+ * host thread while the time steps go on.  Plots, videos and wall tracking
+ * are not produced.  Nothing is printed. */
 typedef struct cb_run_opts {
     cb_params params;
     int nt;                      /* time steps (nt_default 1000, src/def.h:32) */
@@ -250,6 +253,10 @@ typedef struct cb_run_opts {
     int energy_every;            /* OutEnergyEvery 40 */
     int fft;                     /* FFT 1: a spectrum file per snapshot, 0: none */
     int full1d_every;            /* Full1DFieldEvery 100 (1D) */
+    int field_snapshots;         /* 1: _phi_t=NNNN.h5 with every spectrum (the reference always writes them) */
+    int memorysave;              /* memorysave (src/def.h:64) 0: the .h5 files hold phi, pi, Ekin, gradphi,
+                                    potEnergy; 1: phi, pi */
+    int deflate;                 /* gzip level of the .h5 datasets, 0..9 (the reference: 9) */
     const char *out_dir;         /* folder in which the run folder is created (Path_default "./") */
     const double *xns;           /* mode 0: n_xns centres (x, y, z); NULL: read <run folder>/__xns.dat */
     int n_xns;
@@ -262,6 +269,23 @@ cb_status cb_default_run_opts(cb_run_opts *o);
  * "Sim2_D_nt_1000_dt_0.01250_nGrids_512_L_32.0_Mode_1_nB_5_c_4.00_FFT_1_Mpl_200". */
 cb_status cb_run_folder_name(const cb_run_opts *o, char *buf, int cap);
 cb_status cb_run(cb_run_opts *o);
+
+/* ---- HDF5 field snapshots (Bubble::OutField, src/bubble.cpp:883-954) ----
+ * The HDF5 C library (1.10.3 or later) is loaded at run time: the path in
+ * $CB_HDF5_LIBRARY, else libhdf5.so on the loader's path, else the usual
+ * install locations.  CB_OK when it loads, else CB_UNSUPPORTED (reason in
+ * cb_last_error). */
+cb_status cb_hdf5_available(void);
+/* Writes one snapshot file as OutField does: nfields = 2 (phi, pi) or 5 (phi,
+ * pi, Ekin, gradphi, potEnergy) float datasets of shape (n), (n, n) or
+ * (n, n, n), fields[f] holding n^D values in lattice order, where n is the
+ * number of saved points per axis: nGrids, or for 2D above 2048 and 3D above
+ * 512 points ceil(nGrids / reduc) with reduc = nGrids / 2048 (2D) or
+ * nGrids / 512 (3D).  Chunked by whole rows (1D the whole field, 2D
+ * min(n, max(1, 65536 / n)) rows, 3D one plane), shuffle + deflate (level
+ * 0..9, the reference: 9); double attributes D, L, nGrids, reduc, c, t, a. */
+cb_status cb_write_snapshot_h5(const char *path, int D, int nGrids, double L, double c, double t, double a,
+                               int nfields, const float *const *fields, int deflate);
 
 /* Measurement helper (no reference counterpart): achieved bandwidth of a
  * plain 16-byte-vector device-to-device copy of `bytes` bytes, repeated

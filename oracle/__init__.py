@@ -73,6 +73,8 @@ def lib():
         L.cbo_seed_sites.argtypes = [ctypes.POINTER(Params), dp, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_longlong), ctypes.c_int, dp]
         L.cbo_forces.argtypes = [ctypes.c_void_p, dp]
+        L.cbo_energy_maps.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                      dp, dp, dp]
         L.cbo_V.restype = ctypes.c_double
         L.cbo_V.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.cbo_Vpr.restype = ctypes.c_double
@@ -165,6 +167,15 @@ class Sim:
         out = np.zeros(self.n)
         lib().cbo_forces(self._h, _dp(out))
         return out.reshape(self.shape())
+
+    def energy_maps(self, phi, pi, a=1.0, a_kin=1.0, ctor=False):
+        """(Ekin, gradphi, potEnergy) per site of the snapshot maps for phi,
+        pi: the ctor's formulas (ctor=True) or measureEnergies' with a, a_kin."""
+        ph = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
+        pp = np.ascontiguousarray(pi, dtype=np.float64).reshape(-1)
+        out = [np.zeros(self.n) for _ in range(3)]
+        lib().cbo_energy_maps(self._h, _dp(ph), _dp(pp), a, a_kin, int(ctor), *(_dp(o) for o in out))
+        return tuple(o.reshape(self.shape()) for o in out)
 
     def xns(self):
         n = lib().cbo_xns(self._h, None, 0)

@@ -1003,3 +1003,36 @@ void cbo_set_scalars(cbo_sim *s, const double in[6])
 {
     s->a = in[0]; s->ap = in[1]; s->H = in[2]; s->C = in[3]; s->t = in[4]; s->EkinHalfPrev = in[5];
 }
+
+void cbo_energy_maps(const cbo_sim *s, const double *phi, const double *pi, double a, double a_kin, int ctor,
+                     double *ekin, double *egrad, double *epot)
+{
+    const int D = s->p.D, n = s->p.nGrids, nth = s->nthreads;
+    const double dx = s->dx, c = s->p.c, DeltaV = s->DeltaV;
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (long long i = 0; i < s->N; ++i) {
+        int ig = (int)i;
+        if (ctor) {                                              /* src/bubble.cpp:239-265 */
+            double tmp = 0;
+            if (D == 1) {
+                tmp += pow((phi[cbo_iGridM(D, n, ig, 1)] - phi[ig]) / dx, 2);
+            } else {
+                for (int sft = 0; sft < D; ++sft) tmp += pow((phi[cbo_iGridP(D, n, ig, sft)] - phi[ig]) / dx, 2);
+            }
+            egrad[i] = 0.5 * tmp;
+            epot[i] = cbo_V(phi[ig], c, DeltaV);
+            ekin[i] = 0.5 * pi[ig] * pi[ig];
+        } else {                                                 /* :790-833 */
+            double gx = 0., gy = 0., gz = 0.;
+            if (D == 1) gx = (phi[cbo_iGridM(D, n, ig, 1)] - phi[ig]) / (a * dx);
+            if (D == 2 || D == 3) {
+                gx = (phi[cbo_iGridP(D, n, ig, 1)] - phi[ig]) / (a * dx);
+                gy = (phi[cbo_iGridP(D, n, ig, 0)] - phi[ig]) / (a * dx);
+            }
+            if (D == 3) gz = (phi[cbo_iGridP(D, n, ig, 2)] - phi[ig]) / (a * dx);
+            egrad[i] = 0.5 * (gx * gx + gy * gy + gz * gz);
+            epot[i] = cbo_V(phi[ig], c, DeltaV);
+            ekin[i] = 0.5 * pow(a_kin, -2 * D) * pi[ig] * pi[ig];
+        }
+    }
+}

@@ -90,6 +90,11 @@ void cbo_thermal_modes(int D, int N, double L, double T, double m, int velocity,
 void cbo_thermal_field(int D, int N, double L, double T, double m, int velocity,
                        uint64_t base, double *field);
 void cbo_forces(cbo_sim *s, double *dpi_out);
+/* Local energy densities per site for the HDF5 snapshots: ctor = 1 the ctor's
+ * (src/bubble.cpp:239-265, a = 1, pi at t = 0), ctor = 0 measureEnergies'
+ * (:790-833) with the scale factors a and a_kin.  phi, pi: N^D values. */
+void cbo_energy_maps(const cbo_sim *s, const double *phi, const double *pi, double a, double a_kin, int ctor,
+                     double *ekin, double *egrad, double *epot);
 
 #ifdef __cplusplus
 }

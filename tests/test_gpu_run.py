@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import coolbubble_amd as cb
+import h5read
 import oracle
 
 pytestmark = pytest.mark.gpu
@@ -54,6 +55,65 @@ def oracle_run(kw, nt, snaps, xns=None):
     return o, np.array(rows), spectra
 
 
+def oracle_snapshots(kw, nt, snaps, xns=None):
+    """The oracle's snapshot contents (src/bubble.cpp:887-954) at t = 0 and
+    each snapshot step: (t, a, [phi, pi, Ekin, gradphi, potEnergy]).  The
+    maps of a snapshot after step n are those of step n's energy
+    measurement: phi at n - 1, pi after the kick, a and a_kin before the
+    step; at t = 0 the ctor's, with pi before the back-step (0 for bubbles)."""
+    o = oracle.Sim(xns=xns, **kw)
+    phi = o.phi().copy()
+    pi_raw = np.zeros_like(phi) if kw.get("readxns", 1) != -3 else None
+    maps0 = o.energy_maps(phi, pi_raw if pi_raw is not None else np.zeros_like(phi), ctor=True)
+    if pi_raw is None:
+        maps0 = (None,) + maps0[1:]                  # thermal: raw pi not kept by the oracle
+    out = [(0.0, 1.0, [phi, o.pi().copy()] + list(maps0))]
+    done = 0
+    for st in schedule(nt, kw["dt"], snaps):
+        o.step(st - 1 - done)
+        phi_prev = o.phi().copy()
+        a, ap = o.scalars()[:2]
+        o.step(1)
+        done = st
+        sc = o.scalars()
+        maps = o.energy_maps(phi_prev, o.pi(), a=a, a_kin=a + 0.5 * kw["dt"] * ap)
+        out.append((sc[4], sc[0], [o.phi().copy(), o.pi().copy()] + list(maps)))
+    return out
+
+
+H5_NAMES = ["phi", "pi", "Ekin", "gradphi", "potEnergy"]
+
+
+def check_snapshot(path, want, kw, nfields=5, fp32=False):
+    """One _phi_t=NNNN.h5 against the oracle's contents (float32 of the FP64
+    values: within 2 float ulp; the FP32 path: 1e-4 of the field's scale)."""
+    t, a, fields = want
+    D, N = kw["D"], kw["nGrids"]
+    reduc, n = cb.snapshot_points(D, N)
+    with h5read.File(path) as f:
+        assert sorted(f.names()) == sorted(H5_NAMES[:nfields])
+        at = f.attrs()
+        assert at["D"] == D and at["nGrids"] == N and at["reduc"] == reduc
+        assert at["L"] == kw["L"] and at["c"] == kw["c"]
+        assert at["t"] == pytest.approx(t, rel=1e-12, abs=1e-12)
+        assert at["a"] == pytest.approx(a, rel=1e-6 if fp32 else 1e-12)   # FP32: a follows FP32 energies
+        for name, w in zip(H5_NAMES[:nfields], fields):
+            got, _, chunk, filters = f.dataset(name)
+            assert got.shape == (n,) * D
+            assert filters == [(2, (4,)), (1, (9,))]
+            if w is None:
+                continue
+            w = w[(slice(None, None, reduc),) * D]
+            w32 = w.astype(np.float32)
+            scale = float(np.max(np.abs(w))) or 1.0
+            if fp32:
+                tol = 1e-4 * scale + 1e-4 * np.abs(w)
+            else:
+                tol = 2 * np.spacing(np.abs(w32)).astype(np.float64) + 1e-9 * scale
+            err = np.abs(got.astype(np.float64) - w32.astype(np.float64))
+            assert np.all(err <= tol), (path, name, float(err.max()), float(np.max(err - tol)))
+
+
 def check_rows(path, want, c):
     lines = open(path).read().splitlines()
     assert lines[0] == "# c and DeltaV are: %.2f %.4f " % (c, 1.0)
@@ -94,10 +154,13 @@ def test_2d_mode1_run_folder(tmp_path):
     o, rows, spectra = oracle_run(kw, nt, snaps)
     names = sorted(os.listdir(folder))
     assert names == sorted(["parameters.dat", "EnergyDensities.dat", "__xns.dat"]
-                           + ["_phi_k_t=%04d.dat" % n for n in range(snaps + 1)])
+                           + ["_phi_k_t=%04d.dat" % n for n in range(snaps + 1)]
+                           + ["_phi_t=%04d.h5" % n for n in range(snaps + 1)])
     check_rows(os.path.join(folder, "EnergyDensities.dat"), rows, kw["c"])
     for n in range(snaps + 1):
         check_spectrum(os.path.join(folder, "_phi_k_t=%04d.dat" % n), spectra[n])
+    for n, want in enumerate(oracle_snapshots(kw, nt, snaps)):
+        check_snapshot(os.path.join(folder, "_phi_t=%04d.h5" % n), want, kw)
     # __xns.dat: centres "%.5e", periodic images "%.6e", read back as the oracle's
     xl = open(os.path.join(folder, "__xns.dat")).read().splitlines()
     for ln in xl:
@@ -162,6 +225,8 @@ def test_1d_full_field_lines(tmp_path):
         assert got.shape == (256,)
         assert np.all(np.abs(got - w) <= 5.1e-6)
     assert not os.path.exists(os.path.join(folder, "__xns.dat"))
+    for n, want in enumerate(oracle_snapshots(kw, nt, 2)):
+        check_snapshot(os.path.join(folder, "_phi_t=%04d.h5" % n), want, kw)
 
 
 def test_snapshot_spacing_1_front_loads_the_snapshots(tmp_path):
@@ -177,3 +242,46 @@ def test_snapshot_spacing_1_front_loads_the_snapshots(tmp_path):
     np.testing.assert_allclose(ts, [4.0, 8.0, 9.0, 10.0], rtol=1e-5)
     rows = np.loadtxt(os.path.join(folder, "EnergyDensities.dat"))
     assert rows.shape == (11, 16)                # t = 0 and every 40 steps
+
+
+def test_fp32_expanding_thermal_snapshots(tmp_path):
+    """FP32 storage, expanding background (the lagged-drift stepper), thermal
+    start: the snapshot maps see the right time level and scale factors; the
+    initial kinetic map (pi before the back-step) sums to the initial Ekin."""
+    kw = dict(D=3, nGrids=32, L=4.0, c=2.0, readxns=-3, Trad=0.2, expansion=1, Mpl=10.0, dt=0.025)
+    nt, snaps = 60, 3
+    folder, _ = cb.run(str(tmp_path), nt=nt, snapshots=snaps, precision="fp32", fft=0, **kw)
+    want = oracle_snapshots(kw, nt, snaps)
+    for n in range(snaps + 1):
+        check_snapshot(os.path.join(folder, "_phi_t=%04d.h5" % n), want[n], kw, fp32=True)
+    rows = np.loadtxt(os.path.join(folder, "EnergyDensities.dat"))
+    with h5read.File(os.path.join(folder, "_phi_t=0000.h5")) as f:
+        ekin0 = f.dataset("Ekin")[0].astype(np.float64)
+    assert ekin0.sum() * (1.0 / 32) ** 3 == pytest.approx(rows[0, 3], rel=1e-5)
+    assert rows[0, 3] > 0
+
+
+def test_memorysave_keeps_phi_and_pi_only(tmp_path):
+    kw = dict(D=2, nGrids=64, L=8.0, c=4.0, readxns=-2, dt=0.025)
+    folder, _ = cb.run(str(tmp_path), nt=40, snapshots=1, memorysave=1, precision="fp64", **kw)
+    want = oracle_snapshots(kw, 40, 1)
+    for n in range(2):
+        check_snapshot(os.path.join(folder, "_phi_t=%04d.h5" % n), want[n], kw, nfields=2)
+    pl = open(os.path.join(folder, "parameters.dat")).read().splitlines()
+    assert pl[4] == "Mpl = 200, memorysave = 1, TEST = 0  "
+    folder2, _ = cb.run(str(tmp_path / "off"), nt=40, snapshots=1, field_snapshots=0, precision="fp64", **kw)
+    assert not [x for x in os.listdir(folder2) if x.endswith(".h5")]
+
+
+def test_thinned_2d_snapshots_with_partial_chunks(tmp_path):
+    """2D above 2048^2 keeps every reduc-th point: N = 4100 gives reduc 2,
+    n = 2050 and chunks of 31 rows, the last one partial."""
+    kw = dict(D=2, nGrids=4100, L=1025.0, c=4.0, readxns=-2, dt=0.025)
+    nt, snaps = 2, 1
+    folder, _ = cb.run(str(tmp_path), nt=nt, snapshots=snaps, precision="fp64", fft=0, **kw)
+    want = oracle_snapshots(kw, nt, snaps)
+    for n in range(snaps + 1):
+        path = os.path.join(folder, "_phi_t=%04d.h5" % n)
+        check_snapshot(path, want[n], kw)
+        with h5read.File(path) as f:
+            assert f.dataset("phi")[2] == (31, 2050)

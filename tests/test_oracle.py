@@ -450,3 +450,54 @@ def test_seed_sites_and_ctor_pieces_reproduce_the_whole_ctor(kw):
     o.step(3)
     b.step(3)
     np.testing.assert_array_equal(b.phi(), o.phi())
+
+
+@pytest.mark.parametrize("D,N", [(1, 64), (2, 24), (3, 10)])
+def test_ctor_energy_maps_sum_to_the_initial_energies(D, N):
+    """The ctor's local energy densities (src/bubble.cpp:239-265) times the
+    cell volume sum to the initial energy row, with pi before the back-step."""
+    kw = dict(D=D, nGrids=N, L=4.0, c=3.0, dt=0.01)
+    s = oracle.Sim(blank=True, **kw)
+    rng = np.random.default_rng(7 + D)
+    phi = 0.4 + 0.3 * rng.standard_normal((N,) * D)
+    pi0 = 0.2 * rng.standard_normal((N,) * D)
+    s.phi()[...] = phi
+    s.pi()[...] = pi0
+    s.ctor()
+    row = s.energy_row()
+    ekin, egrad, epot = s.energy_maps(phi, pi0, ctor=True)
+    dxLD = (1.0 / N) ** D
+    np.testing.assert_allclose([ekin.sum() * dxLD, egrad.sum() * dxLD, epot.sum() * dxLD], row[3:6], rtol=1e-13)
+    # a plane wave has the closed-form gradient energy of forward differences
+    m = (2,) + (1,) * (D - 1)
+    idx = np.indices((N,) * D)
+    wave = np.cos(sum(2 * np.pi * m[a] * idx[a] / N for a in range(D)))
+    _, g, _ = s.energy_maps(wave, np.zeros_like(wave), ctor=True)
+    dx = 4.0 / N
+    fwd = [np.roll(wave, -1, axis=a) - wave for a in range(D)]
+    np.testing.assert_allclose(g, 0.5 * sum(f * f for f in fwd) / dx ** 2, rtol=1e-12, atol=1e-14)
+    # the maps of the ctor and of measureEnergies at a = a_kin = 1 agree
+    for x, y in zip(s.energy_maps(phi, pi0, ctor=True), s.energy_maps(phi, pi0, a=1.0, a_kin=1.0)):
+        np.testing.assert_allclose(x, y, rtol=2e-16, atol=0)
+
+
+@pytest.mark.parametrize("D,N", [(1, 64), (2, 24), (3, 10)])
+def test_step_energy_maps_sum_to_the_measured_energies(D, N):
+    """The maps of measureEnergies (src/bubble.cpp:790-833) see phi before the
+    step's drift, pi after its kick and the scale factors before the step:
+    their sums are the step's gradient and potential energies and its raw
+    half-step kinetic energy (EkinHalfPrev after the step)."""
+    kw = dict(D=D, nGrids=N, L=4.0, c=3.0, dt=0.01, readxns=-3, Trad=0.3, expansion=1, Mpl=2.0)
+    s = oracle.Sim(**kw)
+    s.step(5)
+    phi_prev = s.phi().copy()
+    a, ap = s.scalars()[:2]
+    s.step(1)
+    row, sc = s.energy_row(), s.scalars()
+    assert (a != 1.0) == (D == 3)        # the background expands in 3D only (src/bubble.hpp:77)
+    ekin, egrad, epot = s.energy_maps(phi_prev, s.pi(), a=a, a_kin=a + 0.5 * 0.01 * ap)
+    dxLD = (1.0 / N) ** D
+    np.testing.assert_allclose([egrad.sum() * dxLD, epot.sum() * dxLD], row[4:6], rtol=1e-12)
+    np.testing.assert_allclose(ekin.sum() * dxLD, sc[5], rtol=1e-12)
+    # a slip of the phi or pi time level is seen
+    assert abs(s.energy_maps(s.phi(), s.pi(), a=a, a_kin=a + 0.005 * ap)[1].sum() * dxLD - row[4]) > 1e-9 * row[4]
