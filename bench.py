@@ -201,6 +201,36 @@ def run_serving(cb, n, rate, log):
                       "mode in {-2,-1,1,-3}; nt ~ U[500,5000]; 5 snapshots with spectra"}
 
 
+def run_driver(cb, log):
+    """Whole runs of the 3D N=512 32-bubble workload through cb_run: the
+    reference's time loop with energy rows every 40 steps, 20 spectra and the
+    text files (no HDF5), then the HDF5 field snapshots alone (two files of
+    5 x 512^3 floats at deflate 9 and 1; one more run with five files at
+    deflate 1 shows the writer keeping up with the stepping)."""
+    import shutil
+    import tempfile
+    out = tempfile.mkdtemp(prefix="cb_run_")
+    kw = dict(CONFIG)
+    legs = [("text_fp32", dict(field_snapshots=0, snapshots=20, precision="fp32")),
+            ("text_fp64", dict(field_snapshots=0, snapshots=20, precision="fp64")),
+            ("h5_deflate9_fp32", dict(field_snapshots=1, snapshots=1, deflate=9, precision="fp32")),
+            ("h5_deflate1_fp32", dict(field_snapshots=1, snapshots=1, deflate=1, precision="fp32")),
+            ("h5x5_deflate1_fp32", dict(field_snapshots=1, snapshots=4, deflate=1, precision="fp32"))]
+    res = {}
+    try:
+        for name, o in legs:
+            d = os.path.join(out, name)
+            folder, el = cb.run(d, nt=2000, **o, **kw)
+            h5 = [os.path.getsize(os.path.join(folder, f)) for f in os.listdir(folder) if f.endswith(".h5")]
+            res[name] = {"elapsed_s": el, "h5_files": len(h5), "h5_bytes": sum(h5)}
+            log("run %s: %.2f s, %d .h5 files, %.2f GB" % (name, el, len(h5), sum(h5) / 1e9))
+            shutil.rmtree(d, ignore_errors=True)
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+    res["config"] = "3D N=512 L=64 c=4 mode 1 with 32 bubbles, dt=0.2 dx, nt=2000"
+    return res
+
+
 def cpu_baseline(sim64, nsteps, log):
     """Oracle (FP64 C/OpenMP restatement, 4 passes per step with energies
     every step, like the reference) stepping the same lattice from the GPU's
@@ -243,7 +273,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--only", choices=["ensemble", "serving"], default=None,
+    ap.add_argument("--only", choices=["ensemble", "serving", "run"], default=None,
                     help="run one leg alone and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -266,6 +296,9 @@ def main():
         return 0
     if args.only == "serving":
         print(json.dumps({"serving": run_serving(cb, args.serve_requests, args.serve_rate, log)}), flush=True)
+        return 0
+    if args.only == "run":
+        print(json.dumps({"run": run_driver(cb, log)}), flush=True)
         return 0
     copy_gbps = cb.bandwidth_probe(1 << 31, 20)
     log("device-to-device copy: %.1f GB/s (read + write)" % copy_gbps)
