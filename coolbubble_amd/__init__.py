@@ -181,7 +181,8 @@ class RunOpts(ctypes.Structure):
     _fields_ = [("params", Params), ("nt", ctypes.c_int), ("snapshots", ctypes.c_int),
                 ("snapshot_spacing", ctypes.c_int), ("energy_every", ctypes.c_int), ("fft", ctypes.c_int),
                 ("full1d_every", ctypes.c_int), ("field_snapshots", ctypes.c_int), ("memorysave", ctypes.c_int),
-                ("deflate", ctypes.c_int), ("out_dir", ctypes.c_char_p),
+                ("deflate", ctypes.c_int), ("wall_tracking", ctypes.c_int), ("wall_track_every", ctypes.c_int),
+                ("out_dir", ctypes.c_char_p),
                 ("xns", ctypes.POINTER(ctypes.c_double)), ("n_xns", ctypes.c_int),
                 ("folder", ctypes.c_char * 1024), ("elapsed_s", ctypes.c_double)]
 
@@ -206,14 +207,16 @@ def run_folder_name(nt=None, snapshots=None, fft=None, full1d_every=None, **kw):
 
 
 def run(out_dir, nt=None, snapshots=None, snapshot_spacing=None, energy_every=None, fft=None,
-        full1d_every=None, field_snapshots=None, memorysave=None, deflate=None, xns=None, **kw):
+        full1d_every=None, field_snapshots=None, memorysave=None, deflate=None, wall_tracking=None,
+        wall_track_every=None, xns=None, **kw):
     """The reference's whole run (src/main.cpp:229-310): creates the run folder
     under out_dir, writes parameters.dat, EnergyDensities.dat, __xns.dat,
-    _phi_k_t=NNNN.dat, _phi_t=NNNN.h5 (field_snapshots) and (1D)
-    FullField.dat.  Returns (folder, elapsed s)."""
+    _phi_k_t=NNNN.dat, _phi_t=NNNN.h5 (field_snapshots), gamma.dat
+    (wall_tracking) and (1D) FullField.dat.  Returns (folder, elapsed s)."""
     o = _run_opts(out_dir, default_params(**kw), nt=nt, snapshots=snapshots, snapshot_spacing=snapshot_spacing,
                   energy_every=energy_every, fft=fft, full1d_every=full1d_every,
-                  field_snapshots=field_snapshots, memorysave=memorysave, deflate=deflate)
+                  field_snapshots=field_snapshots, memorysave=memorysave, deflate=deflate,
+                  wall_tracking=wall_tracking, wall_track_every=wall_track_every)
     keep = None
     if xns is not None:
         keep = np.ascontiguousarray(np.asarray(xns, dtype=np.float64).reshape(-1, 3))

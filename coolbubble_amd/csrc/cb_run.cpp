@@ -19,9 +19,14 @@
 //                        side stream copies them to pinned host memory and a
 //                        host thread deflates and writes the file while the
 //                        time steps go on
-// Not written: plots, videos, wall tracking.  The library prints nothing (the
-// reference's progress lines go to stdout).
+//   gamma.dat            wall tracking (TrackWall, :535-559; outGamma,
+//                        :1009-1014): a device gather of one lattice line of
+//                        phi, scanned on the host as WallFinder does (:576-621)
+// Not written: plots, videos.  The library prints nothing (the reference's
+// progress lines go to stdout).
+#include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -135,7 +140,8 @@ cb_status write_parameters(const std::string &folder, const cb_run_opts &o, cons
     std::fprintf(fp.f, "# The data in this folder correspond to the following:\n");
     std::fprintf(fp.f, " D = %d , L = %.10g,  nGrids = %d , TIME_STEP = %.10g  \n", p.D, p.L, p.nGrids, p.dt);
     std::fprintf(fp.f, " c = %.10g , nt = %d,  readxns = %d , expansion = %d  \n", p.c, o.nt, p.readxns, p.expansion);
-    std::fprintf(fp.f, " nb = %d , FFT = %d,  ALPHAVAL = %d , WallTracking = %d  \n", p.nBubbles, o.fft, p.alpha, 0);
+    std::fprintf(fp.f, " nb = %d , FFT = %d,  ALPHAVAL = %d , WallTracking = %d  \n", p.nBubbles, o.fft, p.alpha,
+                 o.wall_tracking);
     std::fprintf(fp.f, "Mpl = %.10g, memorysave = %d, TEST = %d  \n", p.Mpl, o.memorysave, p.test_wave);
     std::fprintf(fp.f, " Full1DFieldEvery = %d , Trad = %.10g , fixedseed = %d \n", o.full1d_every, p.Trad, 1);
     std::fprintf(fp.f, " kUV = %.10g , kIR = %.10g , kRStar = %.10g , kMeff = %.10g \n", kUV, kIR, kRStar, kMeff);
@@ -323,6 +329,121 @@ struct Snapshots {
     }
 };
 
+// Wall tracking (src/bubble.cpp:525-621).  The field is read along the first
+// lattice axis from the tracked bubble's centre; Rmin is the outermost radius
+// where it is still above its exit value phiZero, Rmax the first where it is
+// at or below the barrier top phiMax; the Lorentz factor is the initial wall
+// width over the current one.
+struct Wall {
+    bool on = false;
+    int D = 1, N = 0, it_start = 0, every = 400;
+    int centre[3] = {0, 0, 0};
+    double phi_zero = 0, phi_max = 0, dx = 0, L = 0, lw = 0, Rn = 0, max_radius = 0;
+    double dini = 0, width = 0, rmin = 0, rmax = 0;
+    std::vector<double> line;
+    double *dline = nullptr;
+    std::string path;
+
+    // CanTrackWall (:533-536), the centre (:310-313), WallTrackStart (:562-565)
+    cb_status init(const cb_sim *s, const cb_run_opts &o, const std::string &folder)
+    {
+        const cb_params &p = o.params;
+        on = p.D == 1 || p.readxns == -1 || p.readxns == -2;
+        if (!on) return CB_OK;
+        D = p.D;
+        N = p.nGrids;
+        every = o.wall_track_every;
+        const int mid = D == 3 ? N / 2 : 0;
+        if (p.readxns == -2 && D >= 2) centre[0] = centre[1] = N / 2, centre[2] = mid;
+        else if (p.readxns == -1 && D >= 2) centre[0] = centre[1] = N / 4, centre[2] = mid;
+        const double c = p.c;
+        phi_max = c / (6 + 2 * c);
+        phi_zero = (2 + c - std::sqrt(4 + c)) / (3 + c);
+        dx = s->k.dx;
+        L = p.L;
+        lw = s->lw;
+        Rn = s->Rn;
+        max_radius = (p.readxns == -1 && D >= 2) ? L / 4.0 - p.DeltaR : L / 2.0;
+        it_start = (int)(M_PI / s->mT / p.dt);
+        path = folder + "/gamma.dat";
+        line.assign((size_t)N, 0.);
+        RUN_HIP(hipMalloc((void **)&dline, sizeof(double) * (size_t)N));
+        return CB_OK;
+    }
+
+    double gamma(double d) const { return d == 0 ? 0 : dini / d; }
+
+    // WallFinder (:576-621)
+    cb_status find(cb_sim *s)
+    {
+        if (rmax >= 0.98 * max_radius || rmin >= 0.98 * max_radius) return CB_OK;   // left the box
+        cb_status r = cb::materialise_drift(s);
+        if (r != CB_OK) return r;
+        RUN_HIP(cb::launch_gather_line(s->k, s->fp32, s->phi[s->iphi], centre[0], centre[1], centre[2], dline,
+                                       s->st));
+        RUN_HIP(hipMemcpyAsync(line.data(), dline, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, s->st));
+        RUN_HIP(hipStreamSynchronize(s->st));
+        // centres lie on cell corners, field values at cell centres
+        const auto radius = [&](int i) { return dx * std::sqrt((i + 0.5) * (i + 0.5) + 0.25 * (D - 1)); };
+        const int istart = std::max((int)((rmin - lw) / dx), (int)(Rn / (2 * dx)));
+        rmin = 0;
+        rmax = 0;
+        for (int i = istart; (rmax == 0 || rmin == 0) && radius(i) < max_radius; i++) {
+            const double v = line[(size_t)(i % N)];
+            if (v > phi_zero) rmin = radius(i);
+            if (v <= phi_max && rmax == 0) rmax = radius(i);
+        }
+        if (rmax < rmin) return CB_OK;                     // no wall found: the width stays
+        if (rmax * rmin == 0 && rmin + rmax != 0) rmax = rmin;
+        width = rmax - rmin;
+        return CB_OK;
+    }
+
+    cb_status out(double t) const
+    {
+        File fp(path, "a");
+        if (!fp.f) return open_fail(path);
+        std::fprintf(fp.f, "%.3f %.4f %.3f %.3f %.3f\n ", t, gamma(width), width, rmin, rmax);
+        return CB_OK;
+    }
+
+    // TrackWall (:543-559), after step it
+    cb_status track(cb_sim *s, int it)
+    {
+        if (!on) return CB_OK;
+        cb_status r = CB_OK;
+        if (it == it_start) {
+            r = find(s);
+            dini = width;
+            if (r == CB_OK) r = out(s->t);
+        } else if (it > it_start && it % every == 0 && s->t < L / 2) {
+            r = find(s);
+            if (r == CB_OK) r = out(s->t);
+        }
+        return r;
+    }
+
+    // UpdateWall (:569-573), before a snapshot
+    cb_status update(cb_sim *s, int it)
+    {
+        if (on && it > it_start && s->t < L / 2) return find(s);
+        return CB_OK;
+    }
+
+    // next step after it at which track() acts, or INT_MAX
+    int next_event(int it) const
+    {
+        if (!on) return INT_MAX;
+        if (it < it_start) return it_start;
+        return (it / every + 1) * every;
+    }
+
+    ~Wall()
+    {
+        if (dline) hipFree(dline);
+    }
+};
+
 struct SimGuard {
     cb_sim *s = nullptr;
     ~SimGuard()
@@ -349,6 +470,8 @@ cb_status cb_default_run_opts(cb_run_opts *o)
     o->field_snapshots = 1;  // OutField at every snapshot (src/main.cpp:192-198)
     o->memorysave = 0;       // memorysave, :64
     o->deflate = 9;          // setDeflate(9), src/bubble.cpp:928
+    o->wall_tracking = 0;    // WallTracking, :72
+    o->wall_track_every = 400; // WallTrackEvery, :73
     o->out_dir = "./";       // Path_default, :63
     return CB_OK;
 }
@@ -369,9 +492,10 @@ cb_status cb_run(cb_run_opts *o)
     if (o->nt < 1 || o->snapshots < 1 || o->energy_every < 1 || o->full1d_every < 1 ||
         (o->fft != 0 && o->fft != 1) || (o->snapshot_spacing != 0 && o->snapshot_spacing != 1) ||
         (o->field_snapshots != 0 && o->field_snapshots != 1) || (o->memorysave != 0 && o->memorysave != 1) ||
-        o->deflate < 0 || o->deflate > 9) {
-        set_error("nt, snapshots, energy_every and full1d_every must be >= 1; fft, snapshot_spacing, "
-                  "field_snapshots and memorysave 0 or 1; deflate 0..9");
+        o->deflate < 0 || o->deflate > 9 || (o->wall_tracking != 0 && o->wall_tracking != 1) ||
+        o->wall_track_every < 1) {
+        set_error("nt, snapshots, energy_every, full1d_every and wall_track_every must be >= 1; fft, "
+                  "snapshot_spacing, field_snapshots, memorysave and wall_tracking 0 or 1; deflate 0..9");
         return CB_INVALID_ARG;
     }
     if (o->field_snapshots) {
@@ -417,6 +541,16 @@ cb_status cb_run(cb_run_opts *o)
         r = snap.init(s, *o, folder);
         if (r != CB_OK) return r;
     }
+    Wall wall;
+    if (o->wall_tracking) {
+        // InitOutputFiles (:967-973): the header, also when the mode cannot be tracked
+        const std::string path = folder + "/gamma.dat";
+        File fp(path, "w");
+        if (!fp.f) return open_fail(path);
+        std::fputs("#t, gamma factor, wall width, rmin, rmax\n", fp.f);
+        r = wall.init(s, *o, folder);
+        if (r != CB_OK) return r;
+    }
     if (o->params.readxns == 1 && o->params.D >= 2 && !o->params.test_wave) {
         const std::string path = folder + "/__xns.dat";
         File fp(path, "w");
@@ -440,6 +574,7 @@ cb_status cb_run(cb_run_opts *o)
         const int e40 = (it / o->energy_every + 1) * o->energy_every;
         if (e40 < ev) ev = e40;
         if (next < snaps.size() && snaps[next] < ev) ev = snaps[next];
+        ev = std::min(ev, wall.next_event(it));
         if (o->params.D == 1) {
             const int ef = (it / o->full1d_every + 1) * o->full1d_every;
             if (ef < ev) ev = ef;
@@ -453,11 +588,13 @@ cb_status cb_run(cb_run_opts *o)
         }
         if (r != CB_OK) return r;
         it = ev;
-        if (o->params.D == 1 && it % o->full1d_every == 0) r = append_full_field(folder, s);
+        r = wall.track(s, it);
+        if (r == CB_OK && o->params.D == 1 && it % o->full1d_every == 0) r = append_full_field(folder, s);
         if (r == CB_OK && it % o->energy_every == 0) r = append_energy_row(folder, s);
         if (r == CB_OK && next < snaps.size() && it == snaps[next]) {
             ++next;
-            if (o->field_snapshots) r = snap.write((int)next);
+            r = wall.update(s, it);
+            if (r == CB_OK && o->field_snapshots) r = snap.write((int)next);
             if (r == CB_OK && o->fft) r = write_spectrum(folder, s, (int)next);
         }
         if (r != CB_OK) return r;

@@ -1,6 +1,7 @@
-// cb_snapshot.hip -- device side of the HDF5 field snapshots: thinning,
-// conversion to float, the local energy densities and HDF5's byte shuffle,
-// written straight into the chunk images the host deflates.
+// cb_snapshot.hip -- device side of the run driver's field read-outs: the
+// HDF5 snapshots (thinning, conversion to float, the local energy densities
+// and HDF5's byte shuffle, written straight into the chunk images the host
+// deflates) and wall tracking's line of the field.
 //
 // One thread per saved point, in lattice order, so that a wavefront's 64
 // points are consecutive elements of one chunk row and each of the four
@@ -146,6 +147,36 @@ hipError_t pack_t(const SimConst &k, const SnapGeom &g, int nfields, const void 
 }
 
 } // namespace
+
+namespace {
+
+template <typename T>
+__global__ void k_gather_line(SimConst k, const T *__restrict__ phi, int c0, int c1, int c2, double *out)
+{
+    const long long N = k.N;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+        long long site;
+        if (k.D == 1) {
+            site = i;
+        } else {
+            const long long i0 = (c0 + i) % N;
+            site = k.D == 2 ? i0 * N + c1 : (i0 * N + c1) * N + c2;
+        }
+        out[i] = (double)phi[site];
+    }
+}
+
+} // namespace
+
+hipError_t launch_gather_line(const SimConst &k, int fp32, const void *phi, int c0, int c1, int c2, double *out,
+                              hipStream_t st)
+{
+    const int block = 256;
+    const unsigned grid = (unsigned)((k.N + block - 1) / block);
+    if (fp32) k_gather_line<float><<<grid, block, 0, st>>>(k, (const float *)phi, c0, c1, c2, out);
+    else k_gather_line<double><<<grid, block, 0, st>>>(k, (const double *)phi, c0, c1, c2, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_snapshot_pack(const SimConst &k, int fp32, const SnapGeom &g, int nfields, const void *phi,
                                 const void *pi, const void *phi_m, const void *pi_m, const DevScalars *msc,

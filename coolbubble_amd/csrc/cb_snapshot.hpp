@@ -61,4 +61,9 @@ hipError_t launch_snapshot_pack(const SimConst &k, int fp32, const SnapGeom &g, 
                                 const void *pi, const void *phi_m, const void *pi_m, const DevScalars *msc,
                                 unsigned char *out, hipStream_t st);
 
+// Wall tracking's line read-out (src/bubble.cpp:587-591): out[i] = phi at
+// lattice point (c0 + i mod N, c1, c2), i in [0, N) (1D: phi[i]), as double.
+hipError_t launch_gather_line(const SimConst &k, int fp32, const void *phi, int c0, int c1, int c2, double *out,
+                              hipStream_t st);
+
 } // namespace cb

@@ -242,9 +242,10 @@ cb_status cb_server_destroy(cb_server *s);
  * :397-453), _phi_k_t=NNNN.dat at the initial state and every snapshot
  * (:642-709), FullField.dat in 1D (:993-1006), and (field_snapshots = 1)
  * the HDF5 field snapshots _phi_t=NNNN.h5 at the same times (OutField,
- * :883-954; see cb_write_snapshot_h5).  The snapshot files are written by a
- * host thread while the time steps go on.  Plots, videos and wall tracking
- * are not produced.  Nothing is printed. */
+ * :883-954; see cb_write_snapshot_h5), and (wall_tracking = 1) gamma.dat
+ * (TrackWall / WallFinder / outGamma, :535-621, :1009-1014).  The snapshot
+ * files are written by a host thread while the time steps go on.  Plots and
+ * videos are not produced.  Nothing is printed. */
 typedef struct cb_run_opts {
     cb_params params;
     int nt;                      /* time steps (nt_default 1000, src/def.h:32) */
@@ -257,6 +258,8 @@ typedef struct cb_run_opts {
     int memorysave;              /* memorysave (src/def.h:64) 0: the .h5 files hold phi, pi, Ekin, gradphi,
                                     potEnergy; 1: phi, pi */
     int deflate;                 /* gzip level of the .h5 datasets, 0..9 (the reference: 9) */
+    int wall_tracking;           /* WallTracking (src/def.h:72) 1: gamma.dat (1D, modes -1 and -2) */
+    int wall_track_every;        /* WallTrackEvery 400 */
     const char *out_dir;         /* folder in which the run folder is created (Path_default "./") */
     const double *xns;           /* mode 0: n_xns centres (x, y, z); NULL: read <run folder>/__xns.dat */
     int n_xns;

@@ -8,6 +8,7 @@ The oracle restates the reference's algorithm (Bubble ctor, oneStep,
 measureEnergies, ComputePowerSpectrum; src/bubble.cpp:156-876) in plain C.
 """
 import ctypes
+import math
 import os
 import subprocess
 
@@ -188,6 +189,84 @@ class Sim:
         k, kp, d = np.zeros(cap), np.zeros(cap), np.zeros(cap)
         nb = lib().cbo_spectrum(self._h, _dp(k), _dp(kp), _dp(d), cap)
         return k[:nb].copy(), kp[:nb].copy(), d[:nb].copy()
+
+
+class WallTracker:
+    """The reference's wall tracking on the oracle's fields, in plain Python:
+    CanTrackWall / TrackWall / WallTrackStart / UpdateWall / WallFinder /
+    GammaFactor (src/bubble.cpp:533-621), the centre (:310-313), the
+    thresholds phiMax, phiZero and mass mT (:167-169) and outGamma's line
+    format (:1009-1014)."""
+
+    def __init__(self, D, nGrids, L, c, readxns, dt, DeltaR=0.0, every=400):
+        self.D, self.N, self.L, self.every = D, nGrids, L, every
+        self.on = D == 1 or readxns in (-1, -2)
+        mid = nGrids // 2 if D == 3 else 0
+        if readxns == -2 and D >= 2:
+            self.centre = (nGrids // 2, nGrids // 2, mid)
+        elif readxns == -1 and D >= 2:
+            self.centre = (nGrids // 4, nGrids // 4, mid)
+        else:
+            self.centre = (0, 0, 0)
+        self.phi_max = c / (6 + 2 * c)
+        self.phi_zero = (2 + c - math.sqrt(4 + c)) / (3 + c)
+        self.dx = L / nGrids
+        _, self.Rn, self.lw = bubble_fit(D, c)
+        self.max_radius = L / 4.0 - DeltaR if (readxns == -1 and D >= 2) else L / 2.0
+        self.it_start = int(math.pi / math.sqrt(2 * (c + 6)) / dt)
+        self.dini = self.width = self.rmin = self.rmax = 0.0
+
+    def radius(self, i):
+        return self.dx * math.sqrt((i + 0.5) * (i + 0.5) + 0.25 * (self.D - 1))
+
+    def along(self, phi, i):
+        if self.D == 1:
+            return float(phi[i])
+        i0 = ((self.centre[0] + i) % self.N + self.N) % self.N
+        return float(phi[(i0,) + tuple(self.centre[1:self.D])])
+
+    def find(self, phi):
+        if self.rmax >= 0.98 * self.max_radius or self.rmin >= 0.98 * self.max_radius:
+            return
+        istart = max(int((self.rmin - self.lw) / self.dx), int(self.Rn / (2 * self.dx)))
+        self.rmin = self.rmax = 0.0
+        i = istart
+        while (self.rmax == 0 or self.rmin == 0) and self.radius(i) < self.max_radius:
+            v = self.along(phi, i)
+            if v > self.phi_zero:
+                self.rmin = self.radius(i)
+            if v <= self.phi_max and self.rmax == 0:
+                self.rmax = self.radius(i)
+            i += 1
+        if self.rmax < self.rmin:
+            return
+        if self.rmax * self.rmin == 0 and self.rmin + self.rmax != 0:
+            self.rmax = self.rmin
+        self.width = self.rmax - self.rmin
+
+    def gamma(self, d):
+        return 0.0 if d == 0 else self.dini / d
+
+    def line(self, t):
+        return "%.3f %.4f %.3f %.3f %.3f\n " % (t, self.gamma(self.width), self.width, self.rmin, self.rmax)
+
+    def track(self, it, t, phi):
+        """TrackWall after step it: the gamma.dat text it appends."""
+        if not self.on:
+            return ""
+        if it == self.it_start:
+            self.find(phi)
+            self.dini = self.width
+            return self.line(t)
+        if it > self.it_start and it % self.every == 0 and t < self.L / 2:
+            self.find(phi)
+            return self.line(t)
+        return ""
+
+    def update(self, it, t, phi):
+        """UpdateWall before snapshot output."""
+        if self.on and it > self.it_start and t < self.L / 2:
+            self.find(phi)
 
 
 def seed_sites(xns, idx, **kw):

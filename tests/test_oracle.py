@@ -501,3 +501,48 @@ def test_step_energy_maps_sum_to_the_measured_energies(D, N):
     np.testing.assert_allclose(ekin.sum() * dxLD, sc[5], rtol=1e-12)
     # a slip of the phi or pi time level is seen
     assert abs(s.energy_maps(s.phi(), s.pi(), a=a, a_kin=a + 0.005 * ap)[1].sum() * dxLD - row[4]) > 1e-9 * row[4]
+
+
+@pytest.mark.parametrize("D,readxns,N", [(1, 1, 512), (2, -2, 128), (2, -1, 128), (3, -2, 64)])
+@pytest.mark.parametrize("R", [3.1, 7.6, 12.05])
+def test_wall_finder_matches_a_brute_force_scan(D, readxns, N, R):
+    """WallFinder (src/bubble.cpp:576-621) on a monotone tanh wall at radius R
+    along the tracked line: Rmin = the outermost radius with phi > phiZero,
+    Rmax = the first with phi <= phiMax, found here over all radii at once."""
+    L, c = 32.0, 4.0
+    w = oracle.WallTracker(D, N, L, c, readxns, dt=0.05)
+    dx = L / N
+    shape = (N,) * D
+    phi = np.zeros(shape)
+    i = np.arange(N)
+    r = dx * np.sqrt((i + 0.5) ** 2 + 0.25 * (D - 1))
+    prof = 0.5 * (1 - np.tanh((r - R) / 0.7))
+    idx = (i if D == 1 else (w.centre[0] + i) % N,) + tuple(np.full(N, w.centre[d]) for d in range(1, D))
+    phi[idx] = prof
+    w.find(phi)
+    ok = (r < w.max_radius) & (np.arange(N) >= int(w.Rn / (2 * dx)))
+    out = np.nonzero(ok & (prof <= w.phi_max))[0]
+    inside = np.nonzero(ok & (prof > w.phi_zero))[0]
+    if len(out):
+        assert w.rmax == r[out[0]] and w.rmin == r[inside[inside < out[0]][-1]]
+        assert w.width == w.rmax - w.rmin > 0
+    else:   # no point at the barrier top before the scan's end: Rmax < Rmin, the width is kept
+        assert w.rmax == 0 and w.rmin == r[inside[-1]] and w.width == 0 and w.gamma(w.width) == 0
+    # the next scan starts just inside the wall and finds it again
+    before = (w.rmin, w.rmax)
+    w.find(phi)
+    assert (w.rmin, w.rmax) == before or w.rmax >= 0.98 * w.max_radius
+
+
+def test_wall_tracker_schedule_and_line_format():
+    w = oracle.WallTracker(1, 256, 16.0, 4.0, 1, dt=0.0125, every=100)
+    assert w.it_start == int(math.pi / math.sqrt(20.0) / 0.0125) == 56
+    phi = 0.5 * (1 - np.tanh((np.arange(256) * 16.0 / 256 - 3.0) / 0.5))
+    assert w.track(55, 0.6875, phi) == ""
+    first = w.track(56, 0.7, phi)
+    assert w.dini == w.width > 0
+    assert first == "0.700 1.0000 %.3f %.3f %.3f\n " % (w.width, w.rmin, w.rmax)
+    assert w.track(150, 1.875, phi) == ""                          # not a multiple of every
+    assert w.track(200, 2.5, phi).startswith("2.500 1.0000 ")
+    assert w.track(700, 8.75, phi) == ""                           # t >= L / 2
+    assert oracle.WallTracker(2, 64, 16.0, 4.0, 1, dt=0.05).track(14, 0.7, np.zeros((64, 64))) == ""
