@@ -324,7 +324,7 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
     s->t = 0.;
     s->step = 0;
     s->row_valid = true;
-    return cb::select_variant(s, s->variants.empty() ? -1 : 0);
+    return cb::select_variant(s, cb::default_variant(s));
 }
 
 cb_status read_scalars(cb_sim *s, cb::DevScalars &h)

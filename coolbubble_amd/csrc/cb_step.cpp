@@ -121,6 +121,37 @@ int companion_variant(const cb_sim *s, const TiledCfg &f)
     return best >= 0 ? best : first;
 }
 
+// The variant a simulation starts with (cb_autotune can do better): the
+// tiled shape nearest to what the autotuner picks on MI355X for lattices of
+// this kind, measured over 2D N = 256-2048, 3D N = 64-512, static and
+// expanding, FP32 and FP64.  Small lattices (3D below 2^19 sites, 2D below
+// 512^2) fill the GPU only with the one-site-per-thread kernel; large static
+// ones take two steps per pass.
+int default_variant(const cb_sim *s)
+{
+    const int D = s->p.D, N = s->p.nGrids;
+    if (s->variants.empty() || (D == 3 ? s->nsites < (1LL << 19) : N < 512)) return -1;
+    struct Want { int fuse, nthr, TR, NI, pf, nt; } w;
+    if (s->lagged) w = s->fp32 ? Want{1, 256, 4, 16, 1, 1} : Want{1, 256, 2, 32, 1, 1};
+    else if (D == 3 && N >= 512) w = {2, 512, 2, 32, 1, 1};
+    else if (D == 3 && N >= 192) w = {2, 256, 4, 16, 1, 0};
+    else if (D == 3) w = {1, 256, 8, 8, 1, 1};
+    else if (N >= 2048) w = {2, 512, 1, 8, 1, 0};
+    else if (N >= 1024) w = {2, 256, 1, 8, 1, 1};
+    else w = {1, 256, 1, 4, 2, 1};
+    int best = -1;
+    double best_score = -1e300;
+    for (size_t v = 0; v < s->variants.size(); ++v) {
+        const TiledCfg &c = s->variants[v];
+        const double score = -10.0 * (c.fuse != w.fuse) - 2.0 * std::fabs(std::log2((double)c.nthr / w.nthr)) -
+                             2.0 * std::fabs(std::log2((double)c.TR / w.TR)) -
+                             std::fabs(std::log2((double)c.NI / w.NI)) - 0.5 * (c.pf != w.pf) - 0.5 * (c.nt != w.nt) -
+                             0.1 * c.items;
+        if (score > best_score) { best_score = score; best = (int)v; }
+    }
+    return best;
+}
+
 cb_status select_variant(cb_sim *s, int variant)
 {
     if (variant < -1 || variant >= (int)s->variants.size()) {

@@ -68,11 +68,23 @@ CASES = {
 }
 
 
+def has_tiled(kw):
+    g = cb.Bubble(**kw)
+    n = g.kernel_variants()[0]
+    g.close()
+    return n > 0
+
+
+# each case on the kernel cb_create selects, and on a tiled variant (small
+# lattices start on the general kernel)
+@pytest.mark.parametrize("variant", [None, 0])
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_fp64_matches_oracle(name):
+def test_fp64_matches_oracle(name, variant):
     kw = CASES[name]
-    print("case", name, kw, flush=True)
-    o, g, ro, rg = run_pair(kw, [1, 40, 80])
+    print("case", name, kw, variant, flush=True)
+    if variant is not None and not has_tiled(kw):
+        pytest.skip("no tiled variant for this lattice")
+    o, g, ro, rg = run_pair(kw, [1, 40, 80], variant)
     err = row_errors(ro, rg)
     assert err.max() <= TOL, (name, err.max(axis=0))
     phi_g, pi_g = g.fields()
@@ -117,11 +129,12 @@ THERMAL = {
 }
 
 
+@pytest.mark.parametrize("variant", [None, 0])
 @pytest.mark.parametrize("name", sorted(THERMAL))
-def test_thermal_and_expanding_match_oracle(name):
+def test_thermal_and_expanding_match_oracle(name, variant):
     kw = THERMAL[name]
-    print("case", name, kw, flush=True)
-    o, g, ro, rg = run_pair(kw, [1, 2, 30])
+    print("case", name, kw, variant, flush=True)
+    o, g, ro, rg = run_pair(kw, [1, 2, 30], variant)
     err = row_errors(ro, rg)
     assert err.max() <= TOL, (name, err.max(axis=0))
     phi_g, pi_g = g.fields()
@@ -129,9 +142,10 @@ def test_thermal_and_expanding_match_oracle(name):
     assert field_err(o.pi(), pi_g) <= TOL
 
 
-def test_expanding_bubble_matches_oracle():
+@pytest.mark.parametrize("variant", [None, 0])
+def test_expanding_bubble_matches_oracle(variant):
     kw = params(D=3, readxns=-2, nGrids=32, L=4.0, expansion=1, Mpl=10.0)
-    o, g, ro, rg = run_pair(kw, [1, 7, 40])
+    o, g, ro, rg = run_pair(kw, [1, 7, 40], variant)
     assert row_errors(ro, rg).max() <= TOL
     phi_g, pi_g = g.fields()
     assert field_err(o.phi(), phi_g) <= TOL
@@ -269,3 +283,16 @@ def test_two_step_variants_bit_identical_with_odd_counts(name):
         np.testing.assert_array_equal(pi, pi0)
         assert np.max(row_errors(e0[None], g.energy_row()[None])) <= 1e-13
         g.close()
+
+
+def test_default_variant_follows_the_lattice():
+    """cb_create starts small lattices on the general kernel, large static
+    boxes on a two-step tile, expanding ones on a one-step tile."""
+    small = cb.Bubble(**CASES["2d_mode1"])
+    assert small.kernel_variants()[1] == -1
+    big = cb.Bubble(precision="fp32", **params(D=3, readxns=-2, nGrids=256, L=32.0))
+    n, sel = big.kernel_variants()
+    assert 0 <= sel < n and big.variant_info(sel)["fuse"] == 2
+    exp = cb.Bubble(precision="fp32", **params(D=3, readxns=-2, nGrids=128, L=16.0, expansion=1, Mpl=10.0))
+    n, sel = exp.kernel_variants()
+    assert 0 <= sel < n and exp.variant_info(sel)["fuse"] == 1
