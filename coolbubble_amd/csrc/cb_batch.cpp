@@ -45,7 +45,7 @@ int pick_variant(const cb_sim *s, int items_max, int pf, int nt, int ni_pref)
     double best_score = -1e300;
     for (size_t v = 0; v < s->variants.size(); ++v) {
         const TiledCfg &c = s->variants[v];
-        if (c.fuse == 2 || c.nthr != 256 || c.items > items_max || c.pf != pf || c.nt != nt) continue;
+        if (c.fuse != 1 || c.nthr != 256 || c.items > items_max || c.pf != pf || c.nt != nt) continue;
         const double score = 1000.0 * c.TR - std::fabs(std::log2((double)c.NI / ni_pref));
         if (score > best_score) { best_score = score; best = (int)v; }
     }

@@ -49,8 +49,8 @@ static hipError_t launch_one(cb_sim *s, bool energy)
     return e;
 }
 
-// Two steps in one launch (static box): phi and pi both ping-pong.
-static hipError_t launch_pair(cb_sim *s, bool energy)
+// Two or three steps in one launch (static box): phi and pi both ping-pong.
+static hipError_t launch_multi(cb_sim *s, int fuse, bool energy)
 {
     StepPtrs p;
     p.phi_in = s->phi[s->iphi];
@@ -59,35 +59,35 @@ static hipError_t launch_pair(cb_sim *s, bool energy)
     p.pi_out = s->pi[s->ipi ^ 1];
     int nblocks = 0;
     const TiledCfg c = tiled_get(s->tiled);
-    hipError_t e = launch_step2(s->k, s->fp32, energy, c, p, s->dsc, s->partials, s->st, &nblocks);
+    hipError_t e = fuse == 3 ? launch_step3(s->k, s->fp32, energy, c, p, s->dsc, s->partials, s->st, &nblocks)
+                             : launch_step2(s->k, s->fp32, energy, c, p, s->dsc, s->partials, s->st, &nblocks);
     if (e == hipSuccess && energy) e = launch_finalize(s->k, s->partials, nblocks, s->dsc, 1, s->st);
     return e;
 }
 
 cb_status step_n(cb_sim *s, int nsteps)
 {
-    const bool fused = s->variant >= 0 && s->variants[(size_t)s->variant].fuse == 2;
+    const int fuse = s->variant >= 0 ? s->variants[(size_t)s->variant].fuse : 1;
     int it = 0;
-    if (fused) {
-        for (; it + 2 <= nsteps; it += 2) {
-            const bool energy = it + 2 == nsteps;
-            HIP_TRY(launch_pair(s, energy));
+    if (fuse >= 2) {
+        for (; it + fuse <= nsteps; it += fuse) {
+            const bool energy = it + fuse == nsteps;
+            HIP_TRY(launch_multi(s, fuse, energy));
             s->iphi ^= 1;
             s->ipi ^= 1;
-            s->t += s->p.dt;
-            s->t += s->p.dt;
-            s->step += 2;
+            for (int q = 0; q < fuse; ++q) s->t += s->p.dt;
+            s->step += fuse;
         }
-        if (it < nsteps) {   // odd remainder on the one-step companion variant
+        // the remainder on the one-step companion variant
+        for (; it < nsteps; ++it) {
             const int keep = s->tiled;
             s->tiled = s->single_variant < 0 ? -1 : tiled_register(s->variants[(size_t)s->single_variant]);
-            hipError_t e = launch_one(s, true);
+            hipError_t e = launch_one(s, it == nsteps - 1);
             s->tiled = keep;
             HIP_TRY(e);
             s->iphi ^= 1;
             s->t += s->p.dt;
             s->step += 1;
-            ++it;
         }
         return CB_OK;
     }
@@ -112,7 +112,7 @@ int companion_variant(const cb_sim *s, const TiledCfg &f)
     double best_score = -1e300;
     for (size_t v = 0; v < s->variants.size(); ++v) {
         const TiledCfg &c = s->variants[v];
-        if (c.fuse == 2) continue;
+        if (c.fuse != 1) continue;
         if (first < 0) first = (int)v;
         if (c.nthr != f.nthr || c.pf != 1 || c.items > 2) continue;
         const double score = 1000.0 * c.TR - std::fabs(std::log2((double)c.NI / f.NI)) + (c.nt == f.nt ? 0.5 : 0.0);
@@ -126,7 +126,7 @@ int companion_variant(const cb_sim *s, const TiledCfg &f)
 // this kind, measured over 2D N = 256-2048, 3D N = 64-512, static and
 // expanding, FP32 and FP64.  Small lattices (3D below 2^19 sites, 2D below
 // 512^2) fill the GPU only with the one-site-per-thread kernel; large static
-// ones take two steps per pass.
+// ones take two steps per pass (three for 2D from 2048^2).
 int default_variant(const cb_sim *s)
 {
     const int D = s->p.D, N = s->p.nGrids;
@@ -136,7 +136,7 @@ int default_variant(const cb_sim *s)
     else if (D == 3 && N >= 512) w = {2, 512, 2, 32, 1, 1};
     else if (D == 3 && N >= 192) w = {2, 256, 4, 16, 1, 0};
     else if (D == 3) w = {1, 256, 8, 8, 1, 1};
-    else if (N >= 2048) w = {2, 512, 1, 8, 1, 0};
+    else if (N >= 2048) w = {3, 512, 1, N / 256, 1, 0};
     else if (N >= 1024) w = {2, 256, 1, 8, 1, 1};
     else w = {1, 256, 1, 4, 2, 1};
     int best = -1;
@@ -158,7 +158,7 @@ cb_status select_variant(cb_sim *s, int variant)
         set_error("kernel variant out of range");
         return CB_UNSUPPORTED;
     }
-    if (variant >= 0 && s->variants[(size_t)variant].fuse == 2) {
+    if (variant >= 0 && s->variants[(size_t)variant].fuse >= 2) {
         if (!s->pi[1]) {   // the fused kernel needs pi double-buffered
             if (hipMalloc(&s->pi[1], (size_t)s->nsites * s->esz) != hipSuccess) {
                 set_error("out of device memory for the second pi buffer");

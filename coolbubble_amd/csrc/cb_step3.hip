@@ -1,0 +1,148 @@
+// cb_step3.hip -- three leapfrog steps per HBM pass (temporal blocking) for a
+// static box, 2D and 3D, on gfx950: one block per tile of cb_tile3.hpp, the
+// launch's configurations, validity and candidates.
+#include "cb_device.hpp"
+#include "cb_tile3.hpp"
+#include "cb_tile_common.hpp"
+#include "cb_tiled.hpp"
+
+namespace cb {
+
+template <typename T, int D, int ITEMS, int NTHR, bool ENERGY>
+__global__ void __launch_bounds__(NTHR)
+k_step3_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restrict__ phi_out,
+              const T *__restrict__ pi_in, T *__restrict__ pi_out, const DevScalars *__restrict__ sc,
+              double *__restrict__ partials)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int G = gridDim.x, b = blockIdx.x;
+    const int lt = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;   // XCD-contiguous tiles
+    tile_step3<T, D, ITEMS, NTHR, ENERGY>(k, g, lt, phi_in, phi_out, pi_in, pi_out, sc, partials,
+                                          reinterpret_cast<T *>(smem_raw));
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static TiledGeom geom3(const SimConst &k, const TiledCfg &c)
+{
+    TiledGeom g;
+    g.TR = (k.D == 3) ? c.TR : 1;
+    g.NI = c.NI;
+    g.NV = k.N / c.vw;
+    g.ntr = (k.D == 3) ? k.N / c.TR : 1;
+    g.nic = (k.N + c.NI - 1) / c.NI;
+    g.nt = c.nt;
+    return g;
+}
+
+size_t step3_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c)
+{
+    const size_t esz = fp32 ? 4 : 8;
+    const int rows = (k.D == 3) ? (c.TR + 6) + (c.TR + 4) + (c.TR + 2) : 3;
+    return (size_t)rows * k.N * esz;
+}
+
+bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c)
+{
+    if (k.D < 2 || k.expanding || c.fuse != 3) return false;
+    const int want_vw = fp32 ? 4 : 2;
+    if (c.vw != want_vw || k.N % c.vw != 0 || k.N < 8) return false;
+    if (c.items != 1 && c.items != 2 && c.items != 4) return false;
+    if (c.nthr != 256 && c.nthr != 512 && c.nthr != 1024) return false;
+    if (c.NI < 1 || c.pf != 1 || (c.nt != 0 && c.nt != 1)) return false;
+    if (k.D == 3 && (c.TR < 1 || k.N % c.TR != 0)) return false;
+    const int NV = k.N / c.vw;
+    const int rowsA = (k.D == 3) ? c.TR + 4 : 1;
+    if ((long long)rowsA * NV > (long long)c.items * c.nthr) return false;
+    if (k.D == 3 && 2 * NV > c.items * c.nthr) return false;
+    if (step3_lds_bytes(k, fp32, c) > 65536) return false;
+    return true;
+}
+
+template <typename T, int D, int ITEMS, int NTHR>
+static void launch3_t(const SimConst &k, int energy, const TiledCfg &c, const StepPtrs &p, const DevScalars *sc,
+                      double *partials, hipStream_t st, size_t shm)
+{
+    TiledGeom g = geom3(k, c);
+    dim3 grid(g.ntr * g.nic), block(NTHR);
+    if (energy)
+        k_step3_tiled<T, D, ITEMS, NTHR, true><<<grid, block, shm, st>>>(
+            k, g, (const T *)p.phi_in, (T *)p.phi_out, (const T *)p.pi_in, (T *)p.pi_out, sc, partials);
+    else
+        k_step3_tiled<T, D, ITEMS, NTHR, false><<<grid, block, shm, st>>>(
+            k, g, (const T *)p.phi_in, (T *)p.phi_out, (const T *)p.pi_in, (T *)p.pi_out, sc, partials);
+}
+
+template <typename T, int D>
+static void launch3_d(const SimConst &k, int energy, const TiledCfg &c, const StepPtrs &p, const DevScalars *sc,
+                      double *partials, hipStream_t st, size_t shm)
+{
+#define CB_L3(I, NT) launch3_t<T, D, I, NT>(k, energy, c, p, sc, partials, st, shm)
+    if (c.nthr == 256) {
+        if (c.items == 1) CB_L3(1, 256);
+        else if (c.items == 2) CB_L3(2, 256);
+        else CB_L3(4, 256);
+    } else if (c.nthr == 512) {
+        if (c.items == 1) CB_L3(1, 512);
+        else if (c.items == 2) CB_L3(2, 512);
+        else CB_L3(4, 512);
+    } else {
+        if (c.items == 1) CB_L3(1, 1024);
+        else if (c.items == 2) CB_L3(2, 1024);
+        else CB_L3(4, 1024);
+    }
+#undef CB_L3
+}
+
+hipError_t launch_step3(const SimConst &k, int fp32, int energy, const TiledCfg &c, const StepPtrs &p,
+                        const DevScalars *sc, double *partials, hipStream_t st, int *nblocks)
+{
+    if (!step3_valid(k, fp32, c)) return hipErrorInvalidValue;
+    const size_t shm = step3_lds_bytes(k, fp32, c);
+    TiledGeom g = geom3(k, c);
+    if (nblocks) *nblocks = g.ntr * g.nic;
+    if (fp32) {
+        if (k.D == 3) launch3_d<float, 3>(k, energy, c, p, sc, partials, st, shm);
+        else launch3_d<float, 2>(k, energy, c, p, sc, partials, st, shm);
+    } else {
+        if (k.D == 3) launch3_d<double, 3>(k, energy, c, p, sc, partials, st, shm);
+        else launch3_d<double, 2>(k, energy, c, p, sc, partials, st, shm);
+    }
+    return hipGetLastError();
+}
+
+int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
+{
+    int n = 0;
+    const int vw = fp32 ? 4 : 2;
+    const int NV = k.N / vw;
+    auto push = [&](int nthr, int TR, int NI, int nt) {
+        const int rowsA = (k.D == 3) ? TR + 4 : 1;
+        int items = (rowsA * NV + nthr - 1) / nthr;
+        if (items == 3) items = 4;
+        if (k.D == 3) {
+            int hitems = (2 * NV + nthr - 1) / nthr;
+            if (hitems == 3) hitems = 4;
+            if (hitems > items) items = hitems;
+        }
+        if (items < 1) items = 1;
+        // (more vectors per thread than this spill registers)
+        if (items > (nthr == 256 ? 4 : nthr == 512 ? 2 : 1)) return;
+        TiledCfg c{vw, items, nthr, TR, NI, 1, nt, 3};
+        if (n < cap && step3_valid(k, fp32, c)) out[n++] = c;
+    };
+    if (k.D == 3) {
+        for (int TR : {4, 2, 6, 8, 1})
+            for (int nthr : {512, 1024, 256})
+                for (int NI : {32, 16})
+                    for (int nt : {1, 0}) push(nthr, TR, NI, nt);
+    } else {
+        for (int nthr : {256, 512})
+            for (int NI : {16, 32, 8})
+                for (int nt : {1, 0}) push(nthr, 1, NI, nt);
+    }
+    return n;
+}
+
+} // namespace cb

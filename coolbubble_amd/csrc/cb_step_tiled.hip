@@ -456,6 +456,7 @@ int tiled_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
 {
     int n = single_candidates(k, fp32, out, cap);
     if (!k.expanding && n < cap) n += step2_candidates(k, fp32, out + n, cap - n);
+    if (!k.expanding && n < cap) n += step3_candidates(k, fp32, out + n, cap - n);
     return n;
 }
 
@@ -540,7 +541,7 @@ static void launch_d(const SimConst &k, int lagged, int energy, const TiledCfg &
 hipError_t launch_step_tiled_cfg(const SimConst &k, int fp32, int lagged, int energy, const TiledCfg &c,
                                  const StepPtrs &p, const DevScalars *sc, double *partials, hipStream_t st)
 {
-    if (c.fuse == 2) return hipErrorInvalidValue;   // two-step shapes launch through launch_step2
+    if (c.fuse != 1) return hipErrorInvalidValue;   // multi-step shapes launch through launch_step2 / 3
     if (!tiled_valid(k, fp32, c)) return hipErrorInvalidValue;
     if (lagged && k.D != 3) return hipErrorInvalidValue;
     const size_t shm = lds_bytes(k, fp32, c);

@@ -1,0 +1,235 @@
+// cb_tile3.hpp -- one tile of the three-steps-per-pass leapfrog for a static
+// box, 2D and 3D (internal; included by cb_step3.hip).
+//
+// The deeper form of the two-step tile (cb_tile2.hpp): in a static box step
+// n+1 at a site needs only step n at its neighbours, so a block marching the
+// slow axis i can chain three steps plane by plane:
+//   step A (n   -> n+1) on plane p   for TR+4 rows (two extra each side),
+//   step B (n+1 -> n+2) on plane p-1 for TR+2 rows,
+//   step C (n+2 -> n+3) on plane p-2 for the TR owned rows,
+// each from the previous level's planes p-1, p, p+1 (relative to its own
+// plane) in registers and its plane in LDS for the j and k taps:
+//   L0 = phi_n     plane p,   rows r0-3 .. r0+TR+2 (TR+6 rows)
+//   L1 = phi_n+1   plane p-1, rows r0-2 .. r0+TR+1 (TR+4 rows, step A's)
+//   L2 = phi_n+2   plane p-2, rows r0-1 .. r0+TR   (TR+2 rows, step B's)
+// The pass reads phi_n, pi_{n-1/2} and writes phi_{n+3}, pi_{n+5/2}: one
+// pass of 16 B (FP32) per site for three site-updates, plus the overlap
+// (rows and planes +-3 per tile and chunk) that L2 mostly serves.  Each
+// site's arithmetic is that of three single steps (src/bubble.cpp:716-743,
+// :854-856, :868-870), so results are bit-identical to the one-step kernels.
+// Energies, when asked for, are those of step C with the kinetic energy of
+// its incoming pi_{n+3/2} (the time-centred Ekin).  pi ping-pongs.  2D: one
+// row, the three levels on rows p, p-1, p-2 of the marching axis.
+#pragma once
+
+#include "cb_device.hpp"
+#include "cb_tile_common.hpp"
+#include "cb_tiled.hpp"
+
+namespace cb {
+
+// One leapfrog step of VW adjacent sites: c the field, mM / mP its i-1 / i+1
+// values, lrow its row in LDS (j taps at lrow -+ N in 3D, k taps at the
+// vector's edges), piv the incoming pi.  Writes the kicked pi and drifted phi.
+template <typename T, int D, int VW, bool ENERGY>
+__device__ inline void leap_vec(const SimConst &k, const Vec<T, VW> &c, const Vec<T, VW> &mM, const Vec<T, VW> &mP,
+                                const T *lrow, int col, const Vec<T, VW> &piv, T fac_lap, T fac_pot, T inv_dx2,
+                                T fc2, T fc3, T fc4, T dt, Vec<T, VW> &pout, Vec<T, VW> &phout,
+                                const DevScalars *sc, double (&acc)[NQ])
+{
+    using V = Vec<T, VW>;
+    using C = T;
+    const int N = k.N;
+    V rm, rp;
+    if constexpr (D == 3) {
+        rm = *reinterpret_cast<const V *>(lrow - N + col);
+        rp = *reinterpret_cast<const V *>(lrow + N + col);
+    }
+    const C fl = lrow[col == 0 ? N - 1 : col - 1];
+    const C fr = lrow[(col + VW == N) ? 0 : col + VW];
+#pragma unroll
+    for (int e = 0; e < VW; ++e) {
+        const C x = c.v[e];
+        const C fM = (e == 0) ? fl : c.v[e - 1];
+        const C fP = (e == VW - 1) ? fr : c.v[e + 1];
+        C dd = (C)0.0;
+        dd += mP.v[e] - (C)2.0 * x + mM.v[e];
+        if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * x + rm.v[e];
+        dd += fP - (C)2.0 * x + fM;
+        C F;
+        if constexpr (sizeof(T) == 8) F = fac_lap * dd / k.dx2 - fac_pot * vprime(x, k);
+        else F = fac_lap * dd * inv_dx2 - fac_pot * vprime_f(x, fc2, fc3, fc4);
+        const C pnew = piv.v[e] + dt * F;
+        pout.v[e] = pnew;
+        phout.v[e] = x + dt * pnew;
+        if constexpr (ENERGY) {
+            double fwd[3];
+            fwd[0] = (double)mP.v[e];
+            if constexpr (D == 3) { fwd[1] = (double)rp.v[e]; fwd[2] = (double)fP; }
+            else { fwd[1] = (double)fP; fwd[2] = 0.0; }
+            energy_terms<D, sizeof(T) == 8>((double)x, fwd, (double)pnew, (double)piv.v[e], k, sc, acc);
+        }
+    }
+}
+
+template <typename T, int D, int ITEMS, int NTHR, bool ENERGY>
+__device__ inline void tile_step3(const SimConst &k, const TiledGeom &g, int lt, const T *__restrict__ phi_in,
+                                  T *__restrict__ phi_out, const T *__restrict__ pi_in, T *__restrict__ pi_out,
+                                  const DevScalars *__restrict__ sc, double *__restrict__ partials, T *lds)
+{
+    constexpr int VW = sizeof(T) == 4 ? 4 : 2;
+    constexpr int H = (D == 3) ? 1 : 0;           // extra rows per level on each side
+    using V = Vec<T, VW>;
+
+    const int N = k.N;
+    const int NV = g.NV;
+    const long long NN = (D == 3) ? (long long)N * N : (long long)N;
+    const int TRv = (D == 3) ? g.TR : 1;
+    const int rt = (D == 3) ? lt % g.ntr : 0;
+    const int ic = (D == 3) ? lt / g.ntr : lt;
+    const int r0 = rt * TRv;
+    const int i0 = ic * g.NI;
+    const int i1 = min(i0 + g.NI, N);
+    const int rowsA = TRv + 4 * H;
+    const int nvecA = rowsA * NV;
+    T *L0 = lds;                                  // phi_n   plane p,   rows r0-3H .. r0+TR-1+3H
+    T *L1 = L0 + (size_t)(TRv + 6 * H) * N;       // phi_n+1 plane p-1, rows r0-2H .. r0+TR-1+2H
+    T *L2 = L1 + (size_t)(TRv + 4 * H) * N;       // phi_n+2 plane p-2, rows r0-H  .. r0+TR-1+H
+    const bool nt = g.nt != 0;
+
+    auto wrapN = [&](int i) { return i < 0 ? i + N : (i >= N ? i - N : i); };
+
+    const T dt = (T)k.dt;
+    const T fac_lap = (T)sc->fac_lap, fac_pot = (T)sc->fac_pot;
+    const T inv_dx2 = (T)(1.0 / k.dx2);
+    const T fc2 = (T)k.c2, fc3 = (T)k.c3, fc4 = (T)k.c4;
+
+    // item u: row rowA of the step-A rows (tile rows r0-2H+rowA), vector col
+    int rowA[ITEMS], col[ITEMS];
+    bool ok[ITEMS], inB[ITEMS], inC[ITEMS];
+    long long off[ITEMS];
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+        const int q = threadIdx.x + u * NTHR;
+        ok[u] = q < nvecA;
+        const int qq = ok[u] ? q : 0;
+        rowA[u] = qq / NV;
+        col[u] = (qq - rowA[u] * NV) * VW;
+        const int grow = (D == 3) ? wrapN(r0 - 2 * H + rowA[u]) : 0;
+        off[u] = (D == 3) ? (long long)grow * N + col[u] : (long long)col[u];
+        inB[u] = ok[u] && rowA[u] >= H && rowA[u] < 3 * H + TRv;
+        inC[u] = ok[u] && rowA[u] >= 2 * H && rowA[u] < 2 * H + TRv;
+    }
+    // outer rows of L0 (3D): rows r0-3 and r0+TR+2 at L0 rows 0 and TR+5
+    bool hok[ITEMS];
+    long long hoff[ITEMS];
+    int hlds[ITEMS];
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+        const int h = threadIdx.x + u * NTHR;
+        hok[u] = (D == 3) && h < 2 * NV;
+        const int hh = hok[u] ? h : 0;
+        const int side = hh >= NV;
+        const int c0 = (hh - side * NV) * VW;
+        const int grow = side ? wrapN(r0 + TRv + 2) : wrapN(r0 - 3);
+        hoff[u] = (long long)grow * N + c0;
+        hlds[u] = (side ? TRv + 5 : 0) * N + c0;
+    }
+
+    V p0[ITEMS], c0[ITEMS], n0[ITEMS];      // phi_n   at planes p-1, p,   p+1
+    V p1[ITEMS], c1[ITEMS], n1[ITEMS];      // phi_n+1 at planes p-2, p-1, p
+    V p2[ITEMS], c2[ITEMS], n2[ITEMS];      // phi_n+2 at planes p-3, p-2, p-1
+    V q1c[ITEMS], q1n[ITEMS];               // pi_n+1/2 at planes p-1, p
+    V q2c[ITEMS], q2n[ITEMS];               // pi_n+3/2 at planes p-2, p-1
+
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    double nacc[NQ];                         // steps A and B measure nothing
+
+    {
+        const long long bm3 = (long long)wrapN(i0 - 3) * NN, bm2 = (long long)wrapN(i0 - 2) * NN;
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u)
+            if (ok[u]) {
+                p0[u] = *reinterpret_cast<const V *>(phi_in + bm3 + off[u]);
+                c0[u] = *reinterpret_cast<const V *>(phi_in + bm2 + off[u]);
+            }
+    }
+
+    for (int p = i0 - 2; p <= i1 + 1; ++p) {
+        const long long base = (long long)wrapN(p) * NN;
+        const long long baseN = (long long)wrapN(p + 1) * NN;
+        V pi0[ITEMS];
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u)
+            if (ok[u]) {
+                n0[u] = *reinterpret_cast<const V *>(phi_in + baseN + off[u]);
+                pi0[u] = *reinterpret_cast<const V *>(pi_in + base + off[u]);
+            }
+        const bool doB = p >= i0;            // step B on plane p-1 in [i0-1, i1]
+        const bool doC = p >= i0 + 2;        // step C on plane p-2 in [i0, i1-1]
+        // stage phi_n(p), phi_n+1(p-1), phi_n+2(p-2)
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u)
+            if (ok[u]) *reinterpret_cast<V *>(L0 + (rowA[u] + H) * N + col[u]) = c0[u];
+        if constexpr (D == 3) {
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u)
+                if (hok[u]) *reinterpret_cast<V *>(L0 + hlds[u]) = *reinterpret_cast<const V *>(phi_in + base + hoff[u]);
+        }
+        if (doB) {
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u)
+                if (ok[u]) *reinterpret_cast<V *>(L1 + rowA[u] * N + col[u]) = c1[u];
+        }
+        if (doC) {
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u)
+                if (inB[u]) *reinterpret_cast<V *>(L2 + (rowA[u] - H) * N + col[u]) = c2[u];
+        }
+        lds_barrier();
+        // step A on plane p, all TR + 4H rows
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u)
+            if (ok[u])
+                leap_vec<T, D, VW, false>(k, c0[u], p0[u], n0[u], L0 + (rowA[u] + H) * N, col[u], pi0[u], fac_lap,
+                                          fac_pot, inv_dx2, fc2, fc3, fc4, dt, q1n[u], n1[u], sc, nacc);
+        // step B on plane p-1, TR + 2H rows
+        if (doB) {
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u)
+                if (inB[u])
+                    leap_vec<T, D, VW, false>(k, c1[u], p1[u], n1[u], L1 + rowA[u] * N, col[u], q1c[u], fac_lap,
+                                              fac_pot, inv_dx2, fc2, fc3, fc4, dt, q2n[u], n2[u], sc, nacc);
+        }
+        // step C on plane p-2, owned rows: to HBM
+        if (doC) {
+            const long long baseq = (long long)wrapN(p - 2) * NN;
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u) {
+                if (!inC[u]) continue;
+                V pn, phn;
+                leap_vec<T, D, VW, ENERGY>(k, c2[u], p2[u], n2[u], L2 + (rowA[u] - H) * N, col[u], q2c[u], fac_lap,
+                                           fac_pot, inv_dx2, fc2, fc3, fc4, dt, pn, phn, sc, acc);
+                store_vec<T, VW>(pi_out + baseq + off[u], pn, nt);
+                store_vec<T, VW>(phi_out + baseq + off[u], phn, nt);
+            }
+        }
+        lds_barrier();
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u) {
+            p0[u] = c0[u];
+            c0[u] = n0[u];
+            p1[u] = c1[u];
+            c1[u] = n1[u];
+            p2[u] = c2[u];
+            c2[u] = n2[u];
+            q1c[u] = q1n[u];
+            q2c[u] = q2n[u];
+        }
+    }
+    if constexpr (ENERGY) block_partials<NQ>(acc, partials + (long long)lt * NQ);
+}
+
+} // namespace cb

@@ -18,7 +18,7 @@ struct TiledCfg {
     int NI;        // planes per block
     int pf;        // 2: next-but-one plane (and next halo rows) prefetched into registers; 1: none
     int nt;        // 1: non-temporal stores of phi and pi
-    int fuse;      // steps per launch: 1, or 2 (temporal blocking, static box only)
+    int fuse;      // steps per launch: 1, or 2 / 3 (temporal blocking, static box only)
 };
 
 // Geometry of one launch, derived from (N, cfg).
@@ -104,6 +104,13 @@ bool step2_valid(const SimConst &k, int fp32, const TiledCfg &c);
 int step2_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap);
 size_t step2_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
 hipError_t launch_step2(const SimConst &k, int fp32, int energy, const TiledCfg &c, const StepPtrs &p,
+                        const DevScalars *sc, double *partials, hipStream_t st, int *nblocks);
+
+// Three steps per launch (cb_step3.hip): static box only; pi must ping-pong.
+bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c);
+int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap);
+size_t step3_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
+hipError_t launch_step3(const SimConst &k, int fp32, int energy, const TiledCfg &c, const StepPtrs &p,
                         const DevScalars *sc, double *partials, hipStream_t st, int *nblocks);
 
 // Registry of chosen configurations (the autotuner writes, the step reads).

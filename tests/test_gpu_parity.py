@@ -258,17 +258,18 @@ def test_long_1d_spectrum_matches_oracle(N):
     np.testing.assert_allclose(d32[big], do[big], rtol=1e-3)
 
 
+@pytest.mark.parametrize("fuse", [2, 3])
 @pytest.mark.parametrize("name", ["2d_mode1", "3d_mode-2", "3d_mode1"])
-def test_two_step_variants_bit_identical_with_odd_counts(name):
-    """Variants that fuse two leapfrog steps per pass (static box) give the
-    single-step fields bit for bit, also when a call has an odd step count
-    (the odd step runs on a one-step companion variant)."""
+def test_multi_step_variants_bit_identical_with_odd_counts(name, fuse):
+    """Variants that fuse two or three leapfrog steps per pass (static box)
+    give the single-step fields bit for bit, also when a call's step count is
+    not a multiple of theirs (the rest runs on a one-step companion)."""
     kw = CASES[name]
     ref = cb.Bubble(**kw)
     ref.select_kernel(-1)
     n, _ = ref.kernel_variants()
-    fused = [v for v in range(n) if ref.variant_info(v)["fuse"] == 2]
-    assert fused, "no two-step variant for " + name
+    fused = [v for v in range(n) if ref.variant_info(v)["fuse"] == fuse]
+    assert fused, "no %d-step variant for %s" % (fuse, name)
     for chunk in (7, 40, 1, 2):
         ref.step(chunk)
     phi0, pi0 = ref.fields()
@@ -296,3 +297,6 @@ def test_default_variant_follows_the_lattice():
     exp = cb.Bubble(precision="fp32", **params(D=3, readxns=-2, nGrids=128, L=16.0, expansion=1, Mpl=10.0))
     n, sel = exp.kernel_variants()
     assert 0 <= sel < n and exp.variant_info(sel)["fuse"] == 1
+    wide = cb.Bubble(precision="fp32", **params(D=2, readxns=-2, nGrids=2048, L=128.0))
+    n, sel = wide.kernel_variants()
+    assert 0 <= sel < n and wide.variant_info(sel)["fuse"] == 3
