@@ -29,8 +29,7 @@ k_step_simple(SimConst k, const T *__restrict__ phi_in, T *__restrict__ phi_out,
     const C dt = (C)k.dt;
     const C s_prev = LAGGED ? (C)sc->s_pending : (C)0;
     const C fac_lap = (C)sc->fac_lap, fac_pot = (C)sc->fac_pot;
-    const C inv_dx2 = (C)(1.0 / k.dx2);
-    const C fc2 = (C)k.c2, fc3 = (C)k.c3, fc4 = (C)k.c4;
+    const StepF32 sf = step_f32(k, sc);
     double acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
@@ -49,22 +48,30 @@ k_step_simple(SimConst k, const T *__restrict__ phi_in, T *__restrict__ phi_out,
         C P[3], M[3];
 #pragma unroll
         for (int a = 0; a < D; ++a) { P[a] = tap(nb[2 * a]); M[a] = tap(nb[2 * a + 1]); }
-        C dd;
-        if constexpr (D == 1) {
-            dd = P[0] - (C)2.0 * ph + M[0];
-        } else {
-            dd = (C)0.0;
-#pragma unroll
-            for (int a = 0; a < D; ++a) dd += P[a] - (C)2.0 * ph + M[a];
-        }
-        C F;
-        if constexpr (sizeof(T) == 8) F = fac_lap * dd / k.dx2 - fac_pot * vprime(ph, k);
-        else F = fac_lap * dd * inv_dx2 - fac_pot * vprime_f(ph, fc2, fc3, fc4);
         const C po = (C)pi_in[s];
-        const C pn = po + dt * F;
-        pi_out[s] = (T)pn;
-        if constexpr (LAGGED) phi_out[s] = (T)ph;
-        else phi_out[s] = (T)(ph + dt * pn);
+        C pn;
+        if constexpr (sizeof(T) == 8) {
+            C dd;
+            if constexpr (D == 1) {
+                dd = P[0] - (C)2.0 * ph + M[0];
+            } else {
+                dd = (C)0.0;
+#pragma unroll
+                for (int a = 0; a < D; ++a) dd += P[a] - (C)2.0 * ph + M[a];
+            }
+            const C F = fac_lap * dd / k.dx2 - fac_pot * vprime(ph, k);
+            pn = po + dt * F;
+            pi_out[s] = (T)pn;
+            if constexpr (LAGGED) phi_out[s] = (T)ph;
+            else phi_out[s] = (T)(ph + dt * pn);
+        } else {
+            // axis 0, the fastest axis, then axis 1 of 3D
+            const C F = force_f32<D>(ph, P[0], M[0], P[D - 1], M[D - 1], P[D == 3 ? 1 : 0], M[D == 3 ? 1 : 0], sf);
+            pn = __builtin_fmaf(sf.dt, F, po);
+            pi_out[s] = pn;
+            if constexpr (LAGGED) phi_out[s] = ph;
+            else phi_out[s] = __builtin_fmaf(sf.dt, pn, ph);
+        }
         if constexpr (ENERGY) {
             double fwd[3] = {(double)P[0], D >= 2 ? (double)P[1] : 0.0, D == 3 ? (double)P[2] : 0.0};
             energy_terms<D, sizeof(T) == 8>((double)ph, fwd, (double)pn, (double)po, k, sc, acc);

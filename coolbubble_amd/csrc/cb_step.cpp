@@ -126,15 +126,15 @@ int companion_variant(const cb_sim *s, const TiledCfg &f)
 // this kind, measured over 2D N = 256-2048, 3D N = 64-512, static and
 // expanding, FP32 and FP64.  Small lattices (3D below 2^19 sites, 2D below
 // 512^2) fill the GPU only with the one-site-per-thread kernel; large static
-// ones take two steps per pass (three for 2D from 2048^2).
+// ones take two or three steps per pass.
 int default_variant(const cb_sim *s)
 {
     const int D = s->p.D, N = s->p.nGrids;
     if (s->variants.empty() || (D == 3 ? s->nsites < (1LL << 19) : N < 512)) return -1;
     struct Want { int fuse, nthr, TR, NI, pf, nt; } w;
     if (s->lagged) w = s->fp32 ? Want{1, 256, 4, 16, 1, 1} : Want{1, 256, 2, 32, 1, 1};
-    else if (D == 3 && N >= 512) w = {2, 512, 2, 32, 1, 1};
-    else if (D == 3 && N >= 192) w = {2, 256, 4, 16, 1, 0};
+    else if (D == 3 && N >= 512) w = {2, 1024, 4, 64, 1, 1};
+    else if (D == 3 && N >= 192) w = {3, 512, 4, 32, 1, 0};
     else if (D == 3) w = {1, 256, 8, 8, 1, 1};
     else if (N >= 2048) w = {3, 512, 1, N / 256, 1, 0};
     else if (N >= 1024) w = {2, 256, 1, 8, 1, 1};

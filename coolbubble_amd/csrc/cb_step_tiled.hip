@@ -52,8 +52,7 @@ __device__ inline void tile_step(const SimConst &k, const TiledGeom &g, int lt, 
     const C dt = (C)k.dt;
     const C s_prev = LAGGED ? (C)sc->s_pending : (C)0;
     const C fac_lap = (C)sc->fac_lap, fac_pot = (C)sc->fac_pot;
-    const C inv_dx2 = (C)(1.0 / k.dx2);
-    const C fc2 = (C)k.c2, fc3 = (C)k.c3, fc4 = (C)k.c4;
+    const StepF32 sf = step_f32(k, sc);
 
     int rowq[ITEMS], col[ITEMS];
     bool ok[ITEMS];
@@ -168,17 +167,25 @@ __device__ inline void tile_step(const SimConst &k, const TiledGeom &g, int lt, 
                 const C fM = (e == 0) ? fl : cur[u].v[e - 1];
                 const C fP = (e == VW - 1) ? fr : cur[u].v[e + 1];
                 const C mM = prev[u].v[e], mP = nxt[u].v[e];
-                C dd = (C)0.0;
-                dd += mP - (C)2.0 * c + mM;
-                if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * c + rm.v[e];
-                dd += fP - (C)2.0 * c + fM;
-                C F;
-                if constexpr (sizeof(T) == 8) F = fac_lap * dd / k.dx2 - fac_pot * vprime(c, k);
-                else F = fac_lap * dd * inv_dx2 - fac_pot * vprime_f(c, fc2, fc3, fc4);
-                const C pnew = po.v[e] + dt * F;
+                C pnew;
+                if constexpr (sizeof(T) == 8) {
+                    C dd = (C)0.0;
+                    dd += mP - (C)2.0 * c + mM;
+                    if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * c + rm.v[e];
+                    dd += fP - (C)2.0 * c + fM;
+                    const C F = fac_lap * dd / k.dx2 - fac_pot * vprime(c, k);
+                    pnew = po.v[e] + dt * F;
+                    if constexpr (LAGGED) phn.v[e] = c;
+                    else phn.v[e] = c + dt * pnew;
+                } else {
+                    C rP = 0, rM = 0;
+                    if constexpr (D == 3) { rP = rp.v[e]; rM = rm.v[e]; }
+                    const C F = force_f32<D>(c, mP, mM, fP, fM, rP, rM, sf);
+                    pnew = __builtin_fmaf(sf.dt, F, po.v[e]);
+                    if constexpr (LAGGED) phn.v[e] = c;
+                    else phn.v[e] = __builtin_fmaf(sf.dt, pnew, c);
+                }
                 pn.v[e] = pnew;
-                if constexpr (LAGGED) phn.v[e] = c;
-                else phn.v[e] = c + dt * pnew;
                 if constexpr (ENERGY) {
                     double fwd[3];
                     fwd[0] = (double)mP;

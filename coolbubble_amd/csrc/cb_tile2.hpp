@@ -58,8 +58,7 @@ __device__ inline void tile_step2(const SimConst &k, const TiledGeom &g, int lt,
 
     const C dt = (C)k.dt;
     const C fac_lap = (C)sc->fac_lap, fac_pot = (C)sc->fac_pot;
-    const C inv_dx2 = (C)(1.0 / k.dx2);
-    const C fc2 = (C)k.c2, fc3 = (C)k.c3, fc4 = (C)k.c4;
+    const StepF32 sf = step_f32(k, sc);
 
     int rowA[ITEMS], col[ITEMS];
     bool ok[ITEMS], inner[ITEMS];
@@ -152,16 +151,23 @@ __device__ inline void tile_step2(const SimConst &k, const TiledGeom &g, int lt,
                 const C c = c0[u].v[e];
                 const C fM = (e == 0) ? fl : c0[u].v[e - 1];
                 const C fP = (e == VW - 1) ? fr : c0[u].v[e + 1];
-                C dd = (C)0.0;
-                dd += n0[u].v[e] - (C)2.0 * c + p0[u].v[e];
-                if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * c + rm.v[e];
-                dd += fP - (C)2.0 * c + fM;
-                C F;
-                if constexpr (sizeof(T) == 8) F = fac_lap * dd / k.dx2 - fac_pot * vprime(c, k);
-                else F = fac_lap * dd * inv_dx2 - fac_pot * vprime_f(c, fc2, fc3, fc4);
-                const C pnew = pi0[u].v[e] + dt * F;
-                q1n[u].v[e] = pnew;
-                n1[u].v[e] = c + dt * pnew;
+                if constexpr (sizeof(T) == 8) {
+                    C dd = (C)0.0;
+                    dd += n0[u].v[e] - (C)2.0 * c + p0[u].v[e];
+                    if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * c + rm.v[e];
+                    dd += fP - (C)2.0 * c + fM;
+                    const C F = fac_lap * dd / k.dx2 - fac_pot * vprime(c, k);
+                    const C pnew = pi0[u].v[e] + dt * F;
+                    q1n[u].v[e] = pnew;
+                    n1[u].v[e] = c + dt * pnew;
+                } else {
+                    C rP = 0, rM = 0;
+                    if constexpr (D == 3) { rP = rp.v[e]; rM = rm.v[e]; }
+                    const C F = force_f32<D>(c, n0[u].v[e], p0[u].v[e], fP, fM, rP, rM, sf);
+                    const C pnew = __builtin_fmaf(sf.dt, F, pi0[u].v[e]);
+                    q1n[u].v[e] = pnew;
+                    n1[u].v[e] = __builtin_fmaf(sf.dt, pnew, c);
+                }
             }
         }
         // step B on plane p-1, owned rows
@@ -185,16 +191,23 @@ __device__ inline void tile_step2(const SimConst &k, const TiledGeom &g, int lt,
                     const C fM = (e == 0) ? fl : c1[u].v[e - 1];
                     const C fP = (e == VW - 1) ? fr : c1[u].v[e + 1];
                     const C mM = p1[u].v[e], mP = n1[u].v[e];
-                    C dd = (C)0.0;
-                    dd += mP - (C)2.0 * c + mM;
-                    if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * c + rm.v[e];
-                    dd += fP - (C)2.0 * c + fM;
-                    C F;
-                    if constexpr (sizeof(T) == 8) F = fac_lap * dd / k.dx2 - fac_pot * vprime(c, k);
-                    else F = fac_lap * dd * inv_dx2 - fac_pot * vprime_f(c, fc2, fc3, fc4);
-                    const C pnew = q1c[u].v[e] + dt * F;
+                    C pnew;
+                    if constexpr (sizeof(T) == 8) {
+                        C dd = (C)0.0;
+                        dd += mP - (C)2.0 * c + mM;
+                        if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * c + rm.v[e];
+                        dd += fP - (C)2.0 * c + fM;
+                        const C F = fac_lap * dd / k.dx2 - fac_pot * vprime(c, k);
+                        pnew = q1c[u].v[e] + dt * F;
+                        phn.v[e] = c + dt * pnew;
+                    } else {
+                        C rP = 0, rM = 0;
+                        if constexpr (D == 3) { rP = rp.v[e]; rM = rm.v[e]; }
+                        const C F = force_f32<D>(c, mP, mM, fP, fM, rP, rM, sf);
+                        pnew = __builtin_fmaf(sf.dt, F, q1c[u].v[e]);
+                        phn.v[e] = __builtin_fmaf(sf.dt, pnew, c);
+                    }
                     pn.v[e] = pnew;
-                    phn.v[e] = c + dt * pnew;
                     if constexpr (ENERGY) {
                         double fwd[3];
                         fwd[0] = (double)mP;

@@ -33,8 +33,8 @@ namespace cb {
 // vector's edges), piv the incoming pi.  Writes the kicked pi and drifted phi.
 template <typename T, int D, int VW, bool ENERGY>
 __device__ inline void leap_vec(const SimConst &k, const Vec<T, VW> &c, const Vec<T, VW> &mM, const Vec<T, VW> &mP,
-                                const T *lrow, int col, const Vec<T, VW> &piv, T fac_lap, T fac_pot, T inv_dx2,
-                                T fc2, T fc3, T fc4, T dt, Vec<T, VW> &pout, Vec<T, VW> &phout,
+                                const T *lrow, int col, const Vec<T, VW> &piv, T fac_lap, T fac_pot,
+                                const StepF32 &sf, T dt, Vec<T, VW> &pout, Vec<T, VW> &phout,
                                 const DevScalars *sc, double (&acc)[NQ])
 {
     using V = Vec<T, VW>;
@@ -52,16 +52,23 @@ __device__ inline void leap_vec(const SimConst &k, const Vec<T, VW> &c, const Ve
         const C x = c.v[e];
         const C fM = (e == 0) ? fl : c.v[e - 1];
         const C fP = (e == VW - 1) ? fr : c.v[e + 1];
-        C dd = (C)0.0;
-        dd += mP.v[e] - (C)2.0 * x + mM.v[e];
-        if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * x + rm.v[e];
-        dd += fP - (C)2.0 * x + fM;
-        C F;
-        if constexpr (sizeof(T) == 8) F = fac_lap * dd / k.dx2 - fac_pot * vprime(x, k);
-        else F = fac_lap * dd * inv_dx2 - fac_pot * vprime_f(x, fc2, fc3, fc4);
-        const C pnew = piv.v[e] + dt * F;
+        C pnew;
+        if constexpr (sizeof(T) == 8) {
+            C dd = (C)0.0;
+            dd += mP.v[e] - (C)2.0 * x + mM.v[e];
+            if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * x + rm.v[e];
+            dd += fP - (C)2.0 * x + fM;
+            const C F = fac_lap * dd / k.dx2 - fac_pot * vprime(x, k);
+            pnew = piv.v[e] + dt * F;
+            phout.v[e] = x + dt * pnew;
+        } else {
+            C rP = 0, rM = 0;
+            if constexpr (D == 3) { rP = rp.v[e]; rM = rm.v[e]; }
+            const C F = force_f32<D>(x, mP.v[e], mM.v[e], fP, fM, rP, rM, sf);
+            pnew = __builtin_fmaf(sf.dt, F, piv.v[e]);
+            phout.v[e] = __builtin_fmaf(sf.dt, pnew, x);
+        }
         pout.v[e] = pnew;
-        phout.v[e] = x + dt * pnew;
         if constexpr (ENERGY) {
             double fwd[3];
             fwd[0] = (double)mP.v[e];
@@ -101,8 +108,7 @@ __device__ inline void tile_step3(const SimConst &k, const TiledGeom &g, int lt,
 
     const T dt = (T)k.dt;
     const T fac_lap = (T)sc->fac_lap, fac_pot = (T)sc->fac_pot;
-    const T inv_dx2 = (T)(1.0 / k.dx2);
-    const T fc2 = (T)k.c2, fc3 = (T)k.c3, fc4 = (T)k.c4;
+    const StepF32 sf = step_f32(k, sc);
 
     // item u: row rowA of the step-A rows (tile rows r0-2H+rowA), vector col
     int rowA[ITEMS], col[ITEMS];
@@ -194,14 +200,14 @@ __device__ inline void tile_step3(const SimConst &k, const TiledGeom &g, int lt,
         for (int u = 0; u < ITEMS; ++u)
             if (ok[u])
                 leap_vec<T, D, VW, false>(k, c0[u], p0[u], n0[u], L0 + (rowA[u] + H) * N, col[u], pi0[u], fac_lap,
-                                          fac_pot, inv_dx2, fc2, fc3, fc4, dt, q1n[u], n1[u], sc, nacc);
+                                          fac_pot, sf, dt, q1n[u], n1[u], sc, nacc);
         // step B on plane p-1, TR + 2H rows
         if (doB) {
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u)
                 if (inB[u])
                     leap_vec<T, D, VW, false>(k, c1[u], p1[u], n1[u], L1 + rowA[u] * N, col[u], q1c[u], fac_lap,
-                                              fac_pot, inv_dx2, fc2, fc3, fc4, dt, q2n[u], n2[u], sc, nacc);
+                                              fac_pot, sf, dt, q2n[u], n2[u], sc, nacc);
         }
         // step C on plane p-2, owned rows: to HBM
         if (doC) {
@@ -211,7 +217,7 @@ __device__ inline void tile_step3(const SimConst &k, const TiledGeom &g, int lt,
                 if (!inC[u]) continue;
                 V pn, phn;
                 leap_vec<T, D, VW, ENERGY>(k, c2[u], p2[u], n2[u], L2 + (rowA[u] - H) * N, col[u], q2c[u], fac_lap,
-                                           fac_pot, inv_dx2, fc2, fc3, fc4, dt, pn, phn, sc, acc);
+                                           fac_pot, sf, dt, pn, phn, sc, acc);
                 store_vec<T, VW>(pi_out + baseq + off[u], pn, nt);
                 store_vec<T, VW>(phi_out + baseq + off[u], phn, nt);
             }

@@ -293,7 +293,7 @@ def test_default_variant_follows_the_lattice():
     assert small.kernel_variants()[1] == -1
     big = cb.Bubble(precision="fp32", **params(D=3, readxns=-2, nGrids=256, L=32.0))
     n, sel = big.kernel_variants()
-    assert 0 <= sel < n and big.variant_info(sel)["fuse"] == 2
+    assert 0 <= sel < n and big.variant_info(sel)["fuse"] == 3
     exp = cb.Bubble(precision="fp32", **params(D=3, readxns=-2, nGrids=128, L=16.0, expansion=1, Mpl=10.0))
     n, sel = exp.kernel_variants()
     assert 0 <= sel < n and exp.variant_info(sel)["fuse"] == 1
