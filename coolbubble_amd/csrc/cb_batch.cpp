@@ -119,6 +119,15 @@ void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size
 
 int member_work(const BatchMember &m) { return m.fused ? std::max(m.ntiles, m.ntiles1) : m.ntiles; }
 
+int member_items(const cb_sim *s)
+{
+    if (s->variant < 0) return 1;
+    const TiledCfg &c = s->variants[(size_t)s->variant];
+    int it = c.items;
+    if (c.fuse >= 2 && s->single_variant >= 0) it = std::max(it, s->variants[(size_t)s->single_variant].items);
+    return it;
+}
+
 // Host side of n steps run by batched launches: buffer parities, t, step.
 void advance_host(cb_sim *s, int n)
 {
@@ -165,6 +174,7 @@ struct cb_batch {
     // captured (their pointers are baked into the graph)
     std::vector<int> cap_par;
     int items = cb::kBatchItems, pf = 1, nt = 1, ni = 32;   // batched launch shape (cb_batch_autotune)
+    int items1 = 1, items2 = 1;         // vectors per thread the members need (one- / two-step kernels)
 };
 
 namespace {
@@ -210,7 +220,7 @@ cb_status enqueue_steps(cb_batch *b, int n)
     for (int it = 0; it < n; ++it) {
         const bool energy = b->any_lagged || it == n - 1;
         if (!b->tiled.empty()) {
-            HIP_TRY(cb::launch_batch_step(b->fp32, energy, 0, b->items, b->pf, b->d_tab, (int)b->tiled.size(),
+            HIP_TRY(cb::launch_batch_step(b->fp32, energy, 0, b->items1, b->pf, b->d_tab, (int)b->tiled.size(),
                                           b->total_tiles, b->grid, b->shm, it, b->st));
             if (energy) HIP_TRY(cb::launch_batch_finalize(b->d_tab, (int)b->tiled.size(), it, b->st));
         }
@@ -221,7 +231,7 @@ cb_status enqueue_steps(cb_batch *b, int n)
         const cb::BatchMember *d2 = b->d_tab + b->tiled.size();
         for (int p = 0; 2 * p < n; ++p) {
             const bool energy = p == (n - 1) / 2;
-            HIP_TRY(cb::launch_batch_step2(b->fp32, energy, 0, cb::kBatchItems, d2, (int)b->paired.size(),
+            HIP_TRY(cb::launch_batch_step2(b->fp32, energy, 0, b->items2, d2, (int)b->paired.size(),
                                            b->total_tiles2, b->grid2, b->shm2, p, b->st));
             if (energy) HIP_TRY(cb::launch_batch_finalize2(d2, (int)b->paired.size(), p, b->st));
         }
@@ -263,6 +273,9 @@ cb_status build_table(cb_batch *b)
         cb::select_variant(s, pick);
         (pick >= 0 ? b->tiled : b->general).push_back((int)i);
     }
+    b->items1 = b->items2 = 1;
+    for (int i : b->tiled) b->items1 = std::max(b->items1, cb::member_items(b->mem[(size_t)i]));
+    for (int i : b->paired) b->items2 = std::max(b->items2, cb::member_items(b->mem[(size_t)i]));
     const size_t n1 = b->tiled.size(), n2 = b->paired.size();
     if (b->d_tab) { hipFree(b->d_tab); b->d_tab = nullptr; }
     if (b->h_tab) { hipHostFree(b->h_tab); b->h_tab = nullptr; }

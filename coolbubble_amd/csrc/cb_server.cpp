@@ -290,16 +290,20 @@ cb_status tick(cb_server *sv)
     for (size_t t = 0; t < pair.size(); ++t)
         for (int p = 0; 2 * p + 2 < tab2[t].nsteps; ++p) plain2[(size_t)p] = 1;
     const int cap = sv->cus * 8;
+    // the smallest kernels that step every member (fewer registers)
+    int items1 = 1, items2 = 1;
+    for (size_t i : single) items1 = std::max(items1, cb::member_items(act[i]->sim));
+    for (size_t i : pair) items2 = std::max(items2, cb::member_items(act[i]->sim));
     for (int it = 0; it < L1 && e == hipSuccess; ++it) {
         const bool energy = energy1[(size_t)it] != 0;
-        e = cb::launch_batch_step(fp32, energy, plain1[(size_t)it], cb::kBatchItems, 1, sv->d_tab, (int)single.size(),
+        e = cb::launch_batch_step(fp32, energy, plain1[(size_t)it], items1, 1, sv->d_tab, (int)single.size(),
                                   tiles1, std::min(tiles1, cap), shm1, it, sv->st);
         if (e == hipSuccess && energy) e = cb::launch_batch_finalize(sv->d_tab, (int)single.size(), it, sv->st);
     }
     const cb::BatchMember *d_tab2 = sv->d_tab + single.size();
     for (int p = 0; 2 * p < L2 && e == hipSuccess; ++p) {
         const bool energy = energy2[(size_t)p] != 0;
-        e = cb::launch_batch_step2(fp32, energy, plain2[(size_t)p], cb::kBatchItems, d_tab2, (int)pair.size(), tiles2,
+        e = cb::launch_batch_step2(fp32, energy, plain2[(size_t)p], items2, d_tab2, (int)pair.size(), tiles2,
                                    std::min(tiles2, cap), shm2, p, sv->st);
         if (e == hipSuccess && energy) e = cb::launch_batch_finalize2(d_tab2, (int)pair.size(), p, sv->st);
     }

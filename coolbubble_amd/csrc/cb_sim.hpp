@@ -80,6 +80,10 @@ int companion_variant(const cb_sim *s, const TiledCfg &f);
 void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size_t *shm);
 // work items a member spans in a batched launch
 int member_work(const BatchMember &m);
+// vectors per thread a batched kernel needs for a member: its variant's and,
+// for two steps per launch, its one-step companion's (a kernel with more
+// steps every tile of it the same way, but holds more registers)
+int member_items(const cb_sim *s);
 // host side of n steps run by batched launches (one-step / two-step members)
 void advance_host(cb_sim *s, int n);
 void advance_host2(cb_sim *s, int n);

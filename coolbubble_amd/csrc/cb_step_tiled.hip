@@ -231,7 +231,10 @@ k_step_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restric
 // members' first work items).  Per-member D and lagging are uniform per
 // block.  The per-tile arithmetic and partials are those of k_step_tiled
 // with the same config, so a member steps bit-identically alone or batched.
-template <typename T, bool ENERGY, int ITEMS, int PF>
+// DSEL = 2 or 3 runs only the members of that dimension: a kernel then holds
+// one dimension's tile code, and the energy kernels, whose FP64 sums need
+// many registers, keep their occupancy (0: every member).
+template <typename T, bool ENERGY, int ITEMS, int PF, int DSEL>
 __global__ void __launch_bounds__(256)
 k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int step_it, int sel)
 {
@@ -248,6 +251,7 @@ k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int ste
             else hi = mid - 1;
         }
         const BatchMember &m = table[lo];
+        if (DSEL != 0 && m.k.D != DSEL) continue;
         if (step_it >= m.nsteps) continue;     // this member stopped at an event earlier in the sequence
         // members measure energies every step when expanding, else at their energy step
         const bool want = m.lagged || m.energy_it < 0 || m.energy_it == step_it;
@@ -259,14 +263,14 @@ k_batch_step(const BatchMember *__restrict__ table, int nmem, int total, int ste
         T *phi_out = (T *)m.phi[pphi ^ 1];
         const T *pi_in = (const T *)m.pi[ppi];
         T *pi_out = m.lagged ? (T *)m.pi[ppi ^ 1] : (T *)m.pi[ppi];
-        if (m.k.D == 3) {
+        if (DSEL != 2 && m.k.D == 3) {
             if (m.lagged)
                 tile_step<T, 3, VW, ITEMS, 256, true, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
                                                              m.partials, lds);
             else
                 tile_step<T, 3, VW, ITEMS, 256, false, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
                                                               m.partials, lds);
-        } else {
+        } else if (DSEL != 3) {
             tile_step<T, 2, VW, ITEMS, 256, false, ENERGY, PF>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc,
                                                           m.partials, lds);
         }
@@ -281,11 +285,11 @@ __global__ void __launch_bounds__(256) k_batch_finalize(const BatchMember *__res
     finalize_member(m.k, m.partials, m.ntiles, m.sc, m.lagged ? 0 : 1);
 }
 
-template <typename T, bool ENERGY>
+template <typename T, bool ENERGY, int DSEL>
 static void batch_launch_ip(int items, int pf, const BatchMember *table, int nmem, int total, int grid, size_t shm,
                             int step_it, int sel, hipStream_t st)
 {
-#define CB_BATCH_L(I, P) k_batch_step<T, ENERGY, I, P><<<grid, 256, shm, st>>>(table, nmem, total, step_it, sel)
+#define CB_BATCH_L(I, P) k_batch_step<T, ENERGY, I, P, DSEL><<<grid, 256, shm, st>>>(table, nmem, total, step_it, sel)
     if (pf == 2) {
         if (items == 1) CB_BATCH_L(1, 2);
         else if (items == 2) CB_BATCH_L(2, 2);
@@ -303,14 +307,21 @@ hipError_t launch_batch_step(int fp32, int energy, int split, int items, int pf,
 {
     // sel 0: every member, plain; 1: members not measuring energies, plain;
     // 2 (energy kernel): members measuring energies
+    // the energy kernels run per dimension (see k_batch_step)
     if (fp32) {
         if (!energy || split)
-            batch_launch_ip<float, false>(items, pf, table, nmem, total_tiles, grid, shm, step_it, energy ? 1 : 0, st);
-        if (energy) batch_launch_ip<float, true>(items, pf, table, nmem, total_tiles, grid, shm, step_it, 2, st);
+            batch_launch_ip<float, false, 0>(items, pf, table, nmem, total_tiles, grid, shm, step_it, energy ? 1 : 0, st);
+        if (energy) {
+            batch_launch_ip<float, true, 2>(items, pf, table, nmem, total_tiles, grid, shm, step_it, 2, st);
+            batch_launch_ip<float, true, 3>(items, pf, table, nmem, total_tiles, grid, shm, step_it, 2, st);
+        }
     } else {
         if (!energy || split)
-            batch_launch_ip<double, false>(items, pf, table, nmem, total_tiles, grid, shm, step_it, energy ? 1 : 0, st);
-        if (energy) batch_launch_ip<double, true>(items, pf, table, nmem, total_tiles, grid, shm, step_it, 2, st);
+            batch_launch_ip<double, false, 0>(items, pf, table, nmem, total_tiles, grid, shm, step_it, energy ? 1 : 0, st);
+        if (energy) {
+            batch_launch_ip<double, true, 2>(items, pf, table, nmem, total_tiles, grid, shm, step_it, 2, st);
+            batch_launch_ip<double, true, 3>(items, pf, table, nmem, total_tiles, grid, shm, step_it, 2, st);
+        }
     }
     return hipGetLastError();
 }
@@ -321,8 +332,9 @@ hipError_t launch_batch_finalize(const BatchMember *table, int nmem, int step_it
     return hipGetLastError();
 }
 
-// Batched two-step launch: every tile of every static two-step member.
-template <typename T, bool ENERGY, int ITEMS>
+// Batched two-step launch: every tile of every static two-step member (DSEL
+// as in k_batch_step).
+template <typename T, bool ENERGY, int ITEMS, int DSEL>
 __global__ void __launch_bounds__(256)
 k_batch_step2(const BatchMember *__restrict__ table, int nmem, int total, int pit, int sel)
 {
@@ -340,6 +352,7 @@ k_batch_step2(const BatchMember *__restrict__ table, int nmem, int total, int pi
             else hi = mid - 1;
         }
         const BatchMember &m = table[lo];
+        if (DSEL != 0 && m.k.D != DSEL) continue;
         if (s0 >= m.nsteps) continue;
         const bool two = s0 + 1 < m.nsteps;
         const bool want = m.energy_it < 0 || m.energy_it == (two ? s0 + 1 : s0);
@@ -353,13 +366,13 @@ k_batch_step2(const BatchMember *__restrict__ table, int nmem, int total, int pi
             if (lt >= m.ntiles) continue;
             const T *pi_in = (const T *)m.pi[ppi];
             T *pi_out = (T *)m.pi[ppi ^ 1];
-            if (m.k.D == 3) tile_step2<T, 3, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
-            else tile_step2<T, 2, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
+            if (DSEL != 2 && m.k.D == 3) tile_step2<T, 3, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
+            else if (DSEL != 3) tile_step2<T, 2, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
         } else {                   // the odd last step: the one-step companion, pi in place
             if (lt >= m.ntiles1) continue;
             T *pi_io = (T *)m.pi[ppi];
-            if (m.k.D == 3) tile_step<T, 3, VW, ITEMS, 256, false, ENERGY, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
-            else tile_step<T, 2, VW, ITEMS, 256, false, ENERGY, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
+            if (DSEL != 2 && m.k.D == 3) tile_step<T, 3, VW, ITEMS, 256, false, ENERGY, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
+            else if (DSEL != 3) tile_step<T, 2, VW, ITEMS, 256, false, ENERGY, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
         }
     }
 }
@@ -374,12 +387,12 @@ __global__ void __launch_bounds__(256) k_batch_finalize2(const BatchMember *__re
     finalize_member(m.k, m.partials, two ? m.ntiles : m.ntiles1, m.sc, 1);
 }
 
-template <typename T, bool ENERGY>
+template <typename T, bool ENERGY, int DSEL>
 static void batch2_launch_i(int items, const BatchMember *table, int nmem, int total, int grid, size_t shm, int pit,
                             int sel, hipStream_t st)
 {
-    if (items == 1) k_batch_step2<T, ENERGY, 1><<<grid, 256, shm, st>>>(table, nmem, total, pit, sel);
-    else k_batch_step2<T, ENERGY, 2><<<grid, 256, shm, st>>>(table, nmem, total, pit, sel);
+    if (items == 1) k_batch_step2<T, ENERGY, 1, DSEL><<<grid, 256, shm, st>>>(table, nmem, total, pit, sel);
+    else k_batch_step2<T, ENERGY, 2, DSEL><<<grid, 256, shm, st>>>(table, nmem, total, pit, sel);
 }
 
 hipError_t launch_batch_step2(int fp32, int energy, int split, int items, const BatchMember *table, int nmem,
@@ -387,11 +400,17 @@ hipError_t launch_batch_step2(int fp32, int energy, int split, int items, const 
 {
     if (items > 2) return hipErrorInvalidValue;
     if (fp32) {
-        if (!energy || split) batch2_launch_i<float, false>(items, table, nmem, total_tiles, grid, shm, pit, energy ? 1 : 0, st);
-        if (energy) batch2_launch_i<float, true>(items, table, nmem, total_tiles, grid, shm, pit, 2, st);
+        if (!energy || split) batch2_launch_i<float, false, 0>(items, table, nmem, total_tiles, grid, shm, pit, energy ? 1 : 0, st);
+        if (energy) {
+            batch2_launch_i<float, true, 2>(items, table, nmem, total_tiles, grid, shm, pit, 2, st);
+            batch2_launch_i<float, true, 3>(items, table, nmem, total_tiles, grid, shm, pit, 2, st);
+        }
     } else {
-        if (!energy || split) batch2_launch_i<double, false>(items, table, nmem, total_tiles, grid, shm, pit, energy ? 1 : 0, st);
-        if (energy) batch2_launch_i<double, true>(items, table, nmem, total_tiles, grid, shm, pit, 2, st);
+        if (!energy || split) batch2_launch_i<double, false, 0>(items, table, nmem, total_tiles, grid, shm, pit, energy ? 1 : 0, st);
+        if (energy) {
+            batch2_launch_i<double, true, 2>(items, table, nmem, total_tiles, grid, shm, pit, 2, st);
+            batch2_launch_i<double, true, 3>(items, table, nmem, total_tiles, grid, shm, pit, 2, st);
+        }
     }
     return hipGetLastError();
 }
