@@ -378,8 +378,9 @@ def main():
                           "steps_per_launch": steps_per_launch32,
                           "note": "achieved = 16 B per site-update (read phi, pi; write pi, phi) x site-updates/s "
                                   "over whole steps (energy launch and finalisation included), HIP events on the "
-                                  "simulation's stream; a two-step variant reads and writes each byte once per "
-                                  "two site-updates, so frac can exceed 1 while HBM delivers less than its peak. "
+                                  "simulation's stream; a two- or three-step variant reads and writes each byte once per "
+                                  "two or three site-updates, so frac can exceed 1 while HBM delivers less than "
+                                  "its peak. "
                                   "traffic = measured HBM bytes per launch of the selected variant "
                                   "(profiles/hbm_traffic.json, rocprofv3 FETCH_SIZE + WRITE_SIZE); "
                                   "delivered_GBps_profiled = traffic / the kernel's profiled average launch"},

@@ -40,6 +40,7 @@ struct cb_sim {
     std::vector<cb::TiledCfg> variants;   // tiled configurations valid for this lattice
     int max_blocks = 0;         // capacity of `partials`, in blocks
     int single_variant = -1;    // one-step variant used for odd steps when `variant` fuses two
+    int tiled_energy = -1;      // registry index of the energy launches of a multi-step variant
     double t = 0.0;
     long long step = 0;
     std::vector<double> xns;    // centres (the __xns.dat rows)

@@ -58,7 +58,7 @@ static hipError_t launch_multi(cb_sim *s, int fuse, bool energy)
     p.pi_in = s->pi[s->ipi];
     p.pi_out = s->pi[s->ipi ^ 1];
     int nblocks = 0;
-    const TiledCfg c = tiled_get(s->tiled);
+    const TiledCfg c = tiled_get(energy && s->tiled_energy >= 0 ? s->tiled_energy : s->tiled);
     hipError_t e = fuse == 3 ? launch_step3(s->k, s->fp32, energy, c, p, s->dsc, s->partials, s->st, &nblocks)
                              : launch_step2(s->k, s->fp32, energy, c, p, s->dsc, s->partials, s->st, &nblocks);
     if (e == hipSuccess && energy) e = launch_finalize(s->k, s->partials, nblocks, s->dsc, 1, s->st);
@@ -102,23 +102,30 @@ cb_status step_n(cb_sim *s, int nsteps)
     return CB_OK;
 }
 
-// The one-step variant that takes the odd last step of a two-step variant f:
-// same threads per block, prefetch 1, at most 2 vectors per thread (so the
-// batched launches can run it too), the tallest tile, then the chunk length
+// The one-step variant that takes the steps of a call left over by a two- or
+// three-step variant f: prefetch 1, at most 2 vectors per thread (so the
+// batched launches can run it too), f's threads per block where a one-step
+// variant has them (else 256), the tallest tile, then the chunk length
 // nearest f's, then f's store kind; any one-step variant if none fits.
 int companion_variant(const cb_sim *s, const TiledCfg &f)
 {
-    int best = -1, first = -1;
-    double best_score = -1e300;
-    for (size_t v = 0; v < s->variants.size(); ++v) {
-        const TiledCfg &c = s->variants[v];
-        if (c.fuse != 1) continue;
-        if (first < 0) first = (int)v;
-        if (c.nthr != f.nthr || c.pf != 1 || c.items > 2) continue;
-        const double score = 1000.0 * c.TR - std::fabs(std::log2((double)c.NI / f.NI)) + (c.nt == f.nt ? 0.5 : 0.0);
-        if (score > best_score) { best_score = score; best = (int)v; }
+    int first = -1;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int nthr = pass == 0 ? f.nthr : 256;
+        int best = -1;
+        double best_score = -1e300;
+        for (size_t v = 0; v < s->variants.size(); ++v) {
+            const TiledCfg &c = s->variants[v];
+            if (c.fuse != 1) continue;
+            if (first < 0) first = (int)v;
+            if (c.nthr != nthr || c.pf != 1 || c.items > 2) continue;
+            const double score =
+                1000.0 * c.TR - std::fabs(std::log2((double)c.NI / f.NI)) + (c.nt == f.nt ? 0.5 : 0.0);
+            if (score > best_score) { best_score = score; best = (int)v; }
+        }
+        if (best >= 0) return best;
     }
-    return best >= 0 ? best : first;
+    return first;
 }
 
 // The variant a simulation starts with (cb_autotune can do better): the
@@ -133,7 +140,7 @@ int default_variant(const cb_sim *s)
     if (s->variants.empty() || (D == 3 ? s->nsites < (1LL << 19) : N < 512)) return -1;
     struct Want { int fuse, nthr, TR, NI, pf, nt; } w;
     if (s->lagged) w = s->fp32 ? Want{1, 256, 4, 16, 1, 1} : Want{1, 256, 2, 32, 1, 1};
-    else if (D == 3 && N >= 512) w = {2, 1024, 4, 64, 1, 1};
+    else if (D == 3 && N >= 512) w = {3, 1024, 4, 32, 1, 1};
     else if (D == 3 && N >= 192) w = {3, 512, 4, 32, 1, 0};
     else if (D == 3) w = {1, 256, 8, 8, 1, 1};
     else if (N >= 2048) w = {3, 512, 1, N / 256, 1, 0};
@@ -166,6 +173,25 @@ cb_status select_variant(cb_sim *s, int variant)
             }
         }
         s->single_variant = companion_variant(s, s->variants[(size_t)variant]);
+    }
+    s->tiled_energy = -1;
+    if (variant >= 0 && s->variants[(size_t)variant].fuse >= 2) {
+        // the energy launches' FP64 sums need more registers: with 1024
+        // threads the three-step tile and the two-vector two-step tile spill
+        // them, so those launches take the same tile (TR, NI: the same sites,
+        // so identical fields; the energy sums in another order) with 512
+        // threads where such a variant exists
+        const TiledCfg &f = s->variants[(size_t)variant];
+        int e = variant;
+        if (f.nthr == 1024 && (f.fuse == 3 || f.items > 1))
+            for (size_t v = 0; v < s->variants.size(); ++v) {
+                const TiledCfg &c = s->variants[v];
+                if (c.fuse == f.fuse && c.TR == f.TR && c.NI == f.NI && c.nt == f.nt && c.pf == f.pf && c.nthr == 512) {
+                    e = (int)v;
+                    break;
+                }
+            }
+        s->tiled_energy = tiled_register(s->variants[(size_t)e]);
     }
     s->variant = variant;
     s->tiled = variant < 0 ? -1 : tiled_register(s->variants[(size_t)variant]);
