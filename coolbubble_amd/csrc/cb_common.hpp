@@ -53,6 +53,7 @@ struct SimConst {
     int N, D;
     int alpha, expanding;
     double dt, dx, dx2, dxLD;
+    double inv_dx2;         // 1 / dx^2 when dx^2 is a power of two (the product is then the quotient), else 0
     double c, DeltaV;
     double c2, c3, c4;      // V' coefficients 2c, 3(2c+4), 4(c+3)
     double c5, c6;          // V coefficients 2c+4, c+3
@@ -134,6 +135,14 @@ __device__ inline double vprime(double phi, const SimConst &k)
 __device__ inline double vpot(double phi, const SimConst &k)
 {
     return ((k.c * (phi * phi) - k.c5 * cube_rn(phi)) + k.c6 * quart_rn(phi)) + k.DeltaV;
+}
+
+// (a^(D-2+alpha) dd) / dx^2 as the reference divides (src/bubble.cpp:741);
+// a multiplication by the exact reciprocal when dx^2 is a power of two
+// (e.g. L = 64, N = 512), which gives the same double.
+__device__ inline double lap_term(double fl_dd, const SimConst &k)
+{
+    return k.inv_dx2 != 0.0 ? fl_dd * k.inv_dx2 : fl_dd / k.dx2;
 }
 
 // FP32 path: the force of one site from its neighbours, with fused

@@ -59,7 +59,7 @@ k_step_simple(SimConst k, const T *__restrict__ phi_in, T *__restrict__ phi_out,
 #pragma unroll
                 for (int a = 0; a < D; ++a) dd += P[a] - (C)2.0 * ph + M[a];
             }
-            const C F = fac_lap * dd / k.dx2 - fac_pot * vprime(ph, k);
+            const C F = lap_term(fac_lap * dd, k) - fac_pot * vprime(ph, k);
             pn = po + dt * F;
             pi_out[s] = (T)pn;
             if constexpr (LAGGED) phi_out[s] = (T)ph;

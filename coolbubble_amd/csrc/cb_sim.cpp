@@ -187,6 +187,10 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
     k.dt = p.dt;
     k.dx = (double)L / (double)g;
     k.dx2 = k.dx * k.dx;
+    {
+        int ex = 0;
+        k.inv_dx2 = std::frexp(k.dx2, &ex) == 0.5 ? 1.0 / k.dx2 : 0.0;
+    }
     k.dxLD = std::pow(k.dx / L, D);
     k.c = c;
     k.DeltaV = 1.;

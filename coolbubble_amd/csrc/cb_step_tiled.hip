@@ -173,7 +173,7 @@ __device__ inline void tile_step(const SimConst &k, const TiledGeom &g, int lt, 
                     dd += mP - (C)2.0 * c + mM;
                     if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * c + rm.v[e];
                     dd += fP - (C)2.0 * c + fM;
-                    const C F = fac_lap * dd / k.dx2 - fac_pot * vprime(c, k);
+                    const C F = lap_term(fac_lap * dd, k) - fac_pot * vprime(c, k);
                     pnew = po.v[e] + dt * F;
                     if constexpr (LAGGED) phn.v[e] = c;
                     else phn.v[e] = c + dt * pnew;

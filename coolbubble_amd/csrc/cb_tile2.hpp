@@ -156,7 +156,7 @@ __device__ inline void tile_step2(const SimConst &k, const TiledGeom &g, int lt,
                     dd += n0[u].v[e] - (C)2.0 * c + p0[u].v[e];
                     if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * c + rm.v[e];
                     dd += fP - (C)2.0 * c + fM;
-                    const C F = fac_lap * dd / k.dx2 - fac_pot * vprime(c, k);
+                    const C F = lap_term(fac_lap * dd, k) - fac_pot * vprime(c, k);
                     const C pnew = pi0[u].v[e] + dt * F;
                     q1n[u].v[e] = pnew;
                     n1[u].v[e] = c + dt * pnew;
@@ -197,7 +197,7 @@ __device__ inline void tile_step2(const SimConst &k, const TiledGeom &g, int lt,
                         dd += mP - (C)2.0 * c + mM;
                         if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * c + rm.v[e];
                         dd += fP - (C)2.0 * c + fM;
-                        const C F = fac_lap * dd / k.dx2 - fac_pot * vprime(c, k);
+                        const C F = lap_term(fac_lap * dd, k) - fac_pot * vprime(c, k);
                         pnew = q1c[u].v[e] + dt * F;
                         phn.v[e] = c + dt * pnew;
                     } else {

@@ -58,7 +58,7 @@ __device__ inline void leap_vec(const SimConst &k, const Vec<T, VW> &c, const Ve
             dd += mP.v[e] - (C)2.0 * x + mM.v[e];
             if constexpr (D == 3) dd += rp.v[e] - (C)2.0 * x + rm.v[e];
             dd += fP - (C)2.0 * x + fM;
-            const C F = fac_lap * dd / k.dx2 - fac_pot * vprime(x, k);
+            const C F = lap_term(fac_lap * dd, k) - fac_pot * vprime(x, k);
             pnew = piv.v[e] + dt * F;
             phout.v[e] = x + dt * pnew;
         } else {
