@@ -95,3 +95,20 @@ def test_run_rejects_bad_options_and_cfl_without_device(tmp_path):
         cb.run(str(tmp_path), nt=10, D=2, nGrids=64, L=4.0, dt=0.1)
     assert e.value.status == 2
     assert not os.listdir(tmp_path)       # nothing created before the stability check passes
+
+
+@pytest.mark.parametrize("opt", [dict(field_snapshots=2), dict(memorysave=-1), dict(deflate=10),
+                                 dict(wall_tracking=3), dict(wall_track_every=0), dict(snapshot_spacing=2)])
+def test_run_rejects_bad_output_options(tmp_path, opt):
+    """Output options are checked before anything is created (no device needed)."""
+    with pytest.raises(cb.CBError) as e:
+        cb.run(str(tmp_path), nt=10, D=2, nGrids=64, L=8.0, dt=0.025, **opt)
+    assert e.value.status == 1
+    assert not os.listdir(tmp_path)
+
+
+def test_run_defaults_are_the_references():
+    o = cb.RunOpts()
+    assert cb.lib().cb_default_run_opts(ctypes.byref(o)) == 0
+    assert (o.nt, o.snapshots, o.snapshot_spacing, o.energy_every, o.fft, o.full1d_every) == (1000, 20, 0, 40, 1, 100)
+    assert (o.field_snapshots, o.memorysave, o.deflate, o.wall_tracking, o.wall_track_every) == (1, 0, 9, 0, 400)
