@@ -496,11 +496,20 @@ static hipError_t make_twiddles(int n, void **out, hipStream_t s)
     return err;
 }
 
+// A kernel launched with more than 64 KiB of dynamic LDS asks for it first.
+template <typename K>
+static hipError_t lds_opt_in(K kernel, size_t shm)
+{
+    if (shm <= 65536) return hipSuccess;
+    if (shm > kLdsBlockMax) return hipErrorInvalidValue;
+    return hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+}
+
 // Rows / columns per block for an LDS budget.
 static int batch_for(int n, size_t csz)
 {
     size_t per = 2 * (size_t)n * csz;
-    int b = (int)(kLdsBudget / per);
+    int b = (int)(kLdsBlockMax / per);
     return b < 1 ? 0 : (b > 16 ? 16 : b);
 }
 
@@ -522,9 +531,13 @@ static hipError_t forward_r2c(const Stages &st, const cpx<R> *tw, const T *field
     const int RB = batch_for(N, sizeof(cpx<R>));
     if (RB < 1) return hipErrorInvalidValue;
     size_t shm = 2 * (size_t)RB * N * sizeof(cpx<R>);
+    hipError_t e = lds_opt_in(k_rows_r2c<T, R>, shm);
+    if (e != hipSuccess) return e;
     k_rows_r2c<T, R><<<(unsigned)((rows + RB - 1) / RB), 256, shm, s>>>(field, H, rows, RB, st, tw);
     const int CB = batch_for(N, sizeof(cpx<R>));
     shm = 2 * (size_t)CB * N * sizeof(cpx<R>);
+    e = lds_opt_in(k_cols<R>, shm);
+    if (e != hipSuccess) return e;
     if (D == 3) {   // axis 1: [N][N][nc]
         const long long tiles = ((long long)nc + CB - 1) / CB;
         k_cols<R><<<(unsigned)(N * tiles), 256, shm, s>>>(H, nc, CB, st, tw, -1);
@@ -546,9 +559,13 @@ static hipError_t long_row(FftPlan *p, const R *field, double pref, double dk, d
     k_real_to_cpx<R, R><<<(N + 255) / 256, 256, 0, st>>>(field, H, N);
     const int CB = batch_for(N1, sizeof(cpx<R>));
     size_t shm = 2 * (size_t)CB * N1 * sizeof(cpx<R>);
+    hipError_t e = lds_opt_in(k_cols<R>, shm);
+    if (e != hipSuccess) return e;
     k_cols<R><<<(unsigned)((N2 + CB - 1) / CB), 256, shm, st>>>(H, N2, CB, p->st1, (const cpx<R> *)p->tw1, -1);
     k_twiddle4<R><<<(N + 255) / 256, 256, 0, st>>>(H, N1, N2, (const cpx<R> *)p->tw, N);
     shm = 2 * (size_t)N2 * sizeof(cpx<R>);
+    e = lds_opt_in(k_cols<R>, shm);
+    if (e != hipSuccess) return e;
     k_cols<R><<<(unsigned)N1, 256, shm, st>>>(H, 1, 1, p->st2, (const cpx<R> *)p->tw2, -1);
     k_bin1d<R><<<(p->nBins + 255) / 256, 256, 0, st>>>(H, N1, N2, N, p->nBins, dk, pref, Pnorm, p->bins);
     return hipGetLastError();
@@ -637,7 +654,7 @@ static cb_status plan_for(cb_sim *s)
         // weights and weighted k per bin, once
         const int nB = p->nBins;
         const size_t shm = sizeof(double) * 3 * (size_t)nB;
-        if (shm > kLdsBudget) {
+        if (shm > kLdsBlockMax || lds_opt_in(k_bin<double, true>, shm) != hipSuccess) {
             fft_destroy(p);
             set_error("too many spectrum bins for LDS");
             return CB_UNSUPPORTED;
@@ -694,7 +711,10 @@ cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int c
         else e = forward_r2c<double, double>(p->st, (const cpx<double> *)p->tw, (const double *)field, (cpx<double> *)p->H, D, s->st);
         HIP_TRY(e);
         const size_t shm = sizeof(double) * 3 * (size_t)nB;
-        if (shm > kLdsBudget) { set_error("too many spectrum bins for LDS"); return CB_UNSUPPORTED; }
+        if (shm > kLdsBlockMax || (s->fp32 ? lds_opt_in(k_bin<float>, shm) : lds_opt_in(k_bin<double>, shm)) != hipSuccess) {
+            set_error("too many spectrum bins for LDS");
+            return CB_UNSUPPORTED;
+        }
         if (s->fp32) k_bin<float><<<p->binblocks, 256, shm, s->st>>>((const cpx<float> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm, p->binpart);
         else k_bin<double><<<p->binblocks, 256, shm, s->st>>>((const cpx<double> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm, p->binpart);
         HIP_TRY(hipGetLastError());
@@ -769,6 +789,12 @@ cb_status thermal_field(cb_sim *s, void *field, double T, double m, int velocity
         cleanup();
         if (e != hipSuccess) { set_error(std::string("thermal IC: ") + hipGetErrorString(e)); return CB_HIP_ERROR; }
         return CB_OK;
+    }
+    if (lds_opt_in(k_cols<double>, shm) != hipSuccess || lds_opt_in(k_rows_c2r<float, double>, shm) != hipSuccess ||
+        lds_opt_in(k_rows_c2r<double, double>, shm) != hipSuccess) {
+        cleanup();
+        set_error("thermal IC: FFT length too long for LDS");
+        return CB_UNSUPPORTED;
     }
     {   // axis 0 then (3D) axis 1, backward
         const long long inner = rows / N * nc;

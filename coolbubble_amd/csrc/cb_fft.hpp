@@ -12,6 +12,9 @@ struct cpx {
 
 static constexpr int kMaxStages = 40;
 static constexpr size_t kLdsBudget = 65536;
+// LDS a block may use on gfx950 (the binning histograms go up to it: 3 doubles
+// per bin, so up to 6826 bins, e.g. 2D N up to ~9600)
+static constexpr size_t kLdsBlockMax = 160 * 1024;
 
 struct Stages {
     int n;

@@ -326,3 +326,16 @@ def test_default_variant_follows_the_lattice():
     wide = cb.Bubble(precision="fp32", **params(D=2, readxns=-2, nGrids=2048, L=128.0))
     n, sel = wide.kernel_variants()
     assert 0 <= sel < n and wide.variant_info(sel)["fuse"] == 3
+
+
+def test_spectrum_with_more_bins_than_64k_of_lds_matches_oracle():
+    """2D N = 4096 has 2896 |k| bins: the binning histogram (3 doubles per
+    bin) needs more than 64 KiB of LDS per block, which gfx950 allows."""
+    kw = params(D=2, readxns=-2, nGrids=4096, L=512.0)
+    o, g, _, _ = run_pair(kw, [2])
+    ko, kpo, do = o.spectrum()
+    kg, kpg, dg = g.spectrum()
+    assert len(ko) == len(kg) > 2730
+    np.testing.assert_allclose(kg, ko, rtol=1e-13, atol=0)
+    floor = 1e-13 * np.max(do)
+    assert np.all(np.abs(dg - do) <= TOL * np.abs(do) + floor), np.max(np.abs(dg - do) / (np.abs(do) + floor))
