@@ -54,34 +54,51 @@ int pick_variant(const cb_sim *s, int items_max, int pf, int nt, int ni_pref)
 
 int batch_variant(const cb_sim *s) { return pick_variant(s, kBatchItems, 1, 1, 32); }
 
-// Two-step tile of a static member for the batched two-step launch: 256
-// threads, at most kBatchItems vectors per thread, at most 4 rows per tile
-// (the tallest of those), chunks nearest 16 planes (3D) or 8 (2D),
-// non-temporal stores preferred; its one-step companion must fit the
-// batched launches too.  -1 if there is none.
-int pick_variant2(const cb_sim *s)
+// Multi-step tile of a static member for the batched multi-step launch:
+// 256 threads, at most kBatchItems vectors per thread, prefetch 1, its
+// one-step companion fitting the batched launches too.  Three steps per
+// launch on the lattices where the lone stepper takes three (3D from
+// N = 192, 2D from N = 2048) and such a tile of 2 rows or more exists, else
+// two; the tallest tile of at most 4 rows, chunks nearest 16 planes (8 in
+// 2D), non-temporal stores preferred.  On smaller lattices the three-step
+// tile's overlap costs more than the HBM pass it saves: an ensemble of
+// 64 x 3D N = 128 and 16 x 2D N = 1024 members runs at 378 us per step
+// with three steps per launch, 305 us with two.  At most max_fuse steps.
+// -1 if there is none.
+int pick_variant_multi(const cb_sim *s, int max_fuse)
 {
     if (s->lagged || s->p.D < 2) return -1;
     const double ni_pref = s->p.D == 3 ? 16.0 : 8.0;
-    int best = -1;
-    double best_score = -1e300;
-    for (size_t v = 0; v < s->variants.size(); ++v) {
-        const TiledCfg &c = s->variants[v];
-        if (c.fuse != 2 || c.nthr != 256 || c.items > kBatchItems || c.TR > 4) continue;
-        const int comp = companion_variant(s, c);
-        if (comp < 0) continue;
-        const TiledCfg &c1 = s->variants[(size_t)comp];
-        if (c1.nthr != 256 || c1.items > kBatchItems || c1.pf != 1) continue;
-        const double score = 1000.0 * c.TR - std::fabs(std::log2(c.NI / ni_pref)) + (c.nt ? 0.5 : 0.0);
-        if (score > best_score) { best_score = score; best = (int)v; }
+    const bool large = s->p.D == 3 ? s->p.nGrids >= 192 : s->p.nGrids >= 2048;
+    for (int fuse : {3, 2}) {
+        if (fuse > max_fuse || (fuse == 3 && !large)) continue;
+        int best = -1;
+        double best_score = -1e300;
+        for (size_t v = 0; v < s->variants.size(); ++v) {
+            const TiledCfg &c = s->variants[v];
+            if (c.fuse != fuse || c.nthr != 256 || c.pf != 1 || c.items > kBatchItems || c.TR > 4) continue;
+            if (fuse == 3 && s->p.D == 3 && c.TR < 2) continue;   // overlap of 3 rows per owned row
+            const int comp = companion_variant(s, c);
+            if (comp < 0) continue;
+            const TiledCfg &c1 = s->variants[(size_t)comp];
+            if (c1.nthr != 256 || c1.items > kBatchItems || c1.pf != 1) continue;
+            const double score = 1000.0 * c.TR - std::fabs(std::log2(c.NI / ni_pref)) + (c.nt ? 0.5 : 0.0);
+            if (score > best_score) { best_score = score; best = (int)v; }
+        }
+        if (best >= 0) return best;
     }
-    return best;
+    return -1;
 }
 
+// Two steps per launch: a server's launches mix requests of every size, and
+// a table holding both tile kinds needs the kernel that holds both (fewer
+// waves per SIMD for all); with three steps allowed for the large requests
+// the serving mix of 200 Poisson-arriving requests ran at p50 3.15 s against
+// 2.86 s.
 int server_variant(const cb_sim *s)
 {
-    const int v2 = pick_variant2(s);
-    return v2 >= 0 ? v2 : batch_variant(s);
+    const int vm = pick_variant_multi(s, 2);
+    return vm >= 0 ? vm : batch_variant(s);
 }
 
 // Table entry of a member stepping with its selected tiled variant.
@@ -105,12 +122,13 @@ void fill_member(BatchMember &m, const cb_sim *s, int tile_begin, int fp32, size
     m.nsteps = 1 << 30;
     m.energy_it = -1;
     size_t b = 0;
-    if (c.fuse == 2) {
+    if (c.fuse >= 2) {
         const TiledCfg c1 = s->variants[(size_t)s->single_variant];
-        m.fused = 1;
+        m.fused = c.fuse;
         m.g1 = tiled_geom(s->k, c1);
         m.ntiles1 = tiled_blocks(s->k, c1);
-        b = std::max(step2_lds_bytes(s->k, fp32, c), tiled_lds_bytes(s->k, fp32, c1));
+        const size_t bm = c.fuse == 3 ? step3_lds_bytes(s->k, fp32, c) : step2_lds_bytes(s->k, fp32, c);
+        b = std::max(bm, tiled_lds_bytes(s->k, fp32, c1));
     } else {
         b = tiled_lds_bytes(s->k, fp32, c);
     }
@@ -139,13 +157,15 @@ void advance_host(cb_sim *s, int n)
     s->step += n;
 }
 
-// n steps of a two-step member: n / 2 two-step launches (phi and pi flip
-// once each) and, for odd n, one one-step launch (phi flips).
-void advance_host2(cb_sim *s, int n)
+// n steps of a multi-step member (F steps per launch): n / F multi-step
+// launches (phi and pi flip once each), then n % F one-step launches (phi
+// flips).
+void advance_host_multi(cb_sim *s, int n)
 {
-    const int pairs = n / 2, odd = n & 1;
-    if ((pairs + odd) & 1) s->iphi ^= 1;
-    if (pairs & 1) s->ipi ^= 1;
+    const int F = s->variants[(size_t)s->variant].fuse;
+    const int q = n / F, r = n % F;
+    if ((q + r) & 1) s->iphi ^= 1;
+    if (q & 1) s->ipi ^= 1;
     for (int it = 0; it < n; ++it) s->t += s->p.dt;
     s->step += n;
 }
@@ -155,18 +175,21 @@ void advance_host2(cb_sim *s, int n)
 struct cb_batch {
     std::vector<cb_sim *> mem;
     std::vector<int> tiled;        // member indices on the batched one-step kernel, table order
-    std::vector<int> paired;       // member indices on the batched two-step kernel, after them
+    std::vector<int> paired;       // member indices on the batched multi-step kernel, after them
     std::vector<int> general;      // member indices on their own general kernel
     hipStream_t st = nullptr;
     int fp32 = 0;
     bool any_lagged = false;
-    cb::BatchMember *h_tab = nullptr;   // pinned host copy
+    std::vector<cb::BatchMember> h_tab; // host copy (members' entries, parities filled in at upload)
+    cb::BatchMember *h_ring = nullptr;  // pinned staging slots of uploads in flight
+    hipEvent_t ring_ev[4] = {};         // each slot's copy done
+    int ring_next = 0;
     cb::BatchMember *d_tab = nullptr;
     int total_tiles = 0, grid = 0;
     size_t shm = 0;
-    int total_tiles2 = 0, grid2 = 0;    // the two-step part of the table
+    int total_tiles2 = 0, grid2 = 0;    // the multi-step part of the table
     size_t shm2 = 0;
-    int fused = 0;                      // static members take two steps per launch
+    int fused = 0;                      // static members take two or three steps per launch
     long long sites = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -175,6 +198,7 @@ struct cb_batch {
     std::vector<int> cap_par;
     int items = cb::kBatchItems, pf = 1, nt = 1, ni = 32;   // batched launch shape (cb_batch_autotune)
     int items1 = 1, items2 = 1;         // vectors per thread the members need (one- / two-step kernels)
+    int fuses2 = 0;                     // F of every multi-step member, 0 for a mix
 };
 
 namespace {
@@ -191,24 +215,39 @@ void batch_free(cb_batch *b)
         cb::release_sim(s);
     }
     if (b->d_tab) hipFree(b->d_tab);
-    if (b->h_tab) hipHostFree(b->h_tab);
+    if (b->h_ring) hipHostFree(b->h_ring);
+    for (hipEvent_t &e : b->ring_ev)
+        if (e) hipEventDestroy(e);
     if (b->st) hipStreamDestroy(b->st);
     delete b;
 }
 
 // Refresh the parities of the table from the members' current buffers.
-// nsteps: length of the launch sequence that follows (the two-step members
-// take their odd last step on the companion tile).
+// nsteps: length of the launch sequence that follows (the multi-step
+// members take its leftover steps on the companion tile, and measure
+// energies at its last step).
+// The copy is asynchronous: it goes through one of four pinned staging slots,
+// which is written again only once its previous copy has run.
 cb_status upload_table(cb_batch *b, int nsteps)
 {
     const size_t n1 = b->tiled.size(), n = n1 + b->paired.size();
+    if (!n) return CB_OK;
     for (size_t t = 0; t < n; ++t) {
         const cb_sim *s = b->mem[(size_t)(t < n1 ? b->tiled[t] : b->paired[t - n1])];
         b->h_tab[t].par_phi = s->iphi;
         b->h_tab[t].par_pi = s->ipi;
-        if (t >= n1) b->h_tab[t].nsteps = nsteps;
+        if (t >= n1) {
+            b->h_tab[t].nsteps = nsteps;
+            b->h_tab[t].energy_it = nsteps - 1;
+        }
     }
-    if (n) HIP_TRY(hipMemcpyAsync(b->d_tab, b->h_tab, sizeof(cb::BatchMember) * n, hipMemcpyHostToDevice, b->st));
+    const int k = b->ring_next;
+    b->ring_next = (k + 1) % 4;
+    HIP_TRY(hipEventSynchronize(b->ring_ev[k]));
+    cb::BatchMember *slot = b->h_ring + (size_t)k * n;
+    std::memcpy(slot, b->h_tab.data(), sizeof(cb::BatchMember) * n);
+    HIP_TRY(hipMemcpyAsync(b->d_tab, slot, sizeof(cb::BatchMember) * n, hipMemcpyHostToDevice, b->st));
+    HIP_TRY(hipEventRecord(b->ring_ev[k], b->st));
     return CB_OK;
 }
 
@@ -225,15 +264,27 @@ cb_status enqueue_steps(cb_batch *b, int n)
             if (energy) HIP_TRY(cb::launch_batch_finalize(b->d_tab, (int)b->tiled.size(), it, b->st));
         }
     }
-    // two-step members: launch p covers steps 2p and 2p + 1 (the last one,
-    // for odd n, only step n - 1 on the one-step companion)
+    // multi-step members: F steps per launch, then the leftover steps on
+    // the one-step companion; energies in each member's last launch (the
+    // others step plainly beside it)
     if (!b->paired.empty()) {
         const cb::BatchMember *d2 = b->d_tab + b->tiled.size();
-        for (int p = 0; 2 * p < n; ++p) {
-            const bool energy = p == (n - 1) / 2;
-            HIP_TRY(cb::launch_batch_step2(b->fp32, energy, 0, b->items2, d2, (int)b->paired.size(),
-                                           b->total_tiles2, b->grid2, b->shm2, p, b->st));
-            if (energy) HIP_TRY(cb::launch_batch_finalize2(d2, (int)b->paired.size(), p, b->st));
+        int nl = 0;
+        std::vector<int> lastl;
+        for (int i : b->paired) {
+            const int F = b->mem[(size_t)i]->variants[(size_t)b->mem[(size_t)i]->variant].fuse;
+            lastl.push_back(cb::multi_launches(F, n) - 1);
+            nl = std::max(nl, lastl.back() + 1);
+        }
+        for (int L = 0; L < nl; ++L) {
+            bool energy = false, plain = false;
+            for (int ll : lastl) {
+                if (ll == L) energy = true;
+                else if (ll > L) plain = true;
+            }
+            HIP_TRY(cb::launch_batch_multi(b->fp32, energy, energy && plain, b->items2, b->fuses2, d2, (int)b->paired.size(),
+                                           b->total_tiles2, b->grid2, b->shm2, L, b->st));
+            if (energy) HIP_TRY(cb::launch_batch_finalize_multi(d2, (int)b->paired.size(), L, b->st));
         }
     }
     for (int i : b->general) {
@@ -241,7 +292,7 @@ cb_status enqueue_steps(cb_batch *b, int n)
         if (r != CB_OK) return r;
     }
     for (int i : b->tiled) cb::advance_host(b->mem[(size_t)i], n);
-    for (int i : b->paired) cb::advance_host2(b->mem[(size_t)i], n);
+    for (int i : b->paired) cb::advance_host_multi(b->mem[(size_t)i], n);
     return CB_OK;
 }
 
@@ -264,7 +315,7 @@ cb_status build_table(cb_batch *b)
     b->general.clear();
     for (size_t i = 0; i < b->mem.size(); ++i) {
         cb_sim *s = b->mem[i];
-        const int pick2 = b->fused ? cb::pick_variant2(s) : -1;
+        const int pick2 = b->fused ? cb::pick_variant_multi(s, 3) : -1;
         if (pick2 >= 0 && cb::select_variant(s, pick2) == CB_OK) {
             b->paired.push_back((int)i);
             continue;
@@ -276,14 +327,24 @@ cb_status build_table(cb_batch *b)
     b->items1 = b->items2 = 1;
     for (int i : b->tiled) b->items1 = std::max(b->items1, cb::member_items(b->mem[(size_t)i]));
     for (int i : b->paired) b->items2 = std::max(b->items2, cb::member_items(b->mem[(size_t)i]));
+    bool two = false, three = false;
+    for (int i : b->paired) (b->mem[(size_t)i]->variants[(size_t)b->mem[(size_t)i]->variant].fuse == 3 ? three : two) = true;
+    b->fuses2 = two && three ? 0 : (three ? 3 : 2);
     const size_t n1 = b->tiled.size(), n2 = b->paired.size();
     if (b->d_tab) { hipFree(b->d_tab); b->d_tab = nullptr; }
-    if (b->h_tab) { hipHostFree(b->h_tab); b->h_tab = nullptr; }
+    if (b->h_ring) { hipHostFree(b->h_ring); b->h_ring = nullptr; }
+    b->h_tab.clear();
     b->total_tiles = b->total_tiles2 = 0;
     b->shm = b->shm2 = 0;
     b->grid = b->grid2 = 0;
     if (!n1 && !n2) return CB_OK;
-    if (hipHostMalloc((void **)&b->h_tab, sizeof(cb::BatchMember) * (n1 + n2), hipHostMallocDefault) != hipSuccess ||
+    b->h_tab.assign(n1 + n2, cb::BatchMember{});
+    for (hipEvent_t &e : b->ring_ev)
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            set_error("batch event creation failed");
+            return CB_HIP_ERROR;
+        }
+    if (hipHostMalloc((void **)&b->h_ring, sizeof(cb::BatchMember) * (n1 + n2) * 4, hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void **)&b->d_tab, sizeof(cb::BatchMember) * (n1 + n2)) != hipSuccess) {
         set_error("batch table allocation failed");
         return CB_OUT_OF_MEMORY;
@@ -408,11 +469,21 @@ cb_status cb_batch_step(cb_batch *b, int nsteps)
     }
     while (left >= kSegment) {
         HIP_TRY(hipGraphLaunch(b->exec, b->st));
-        for (cb_sim *s : b->mem) {
+        // host side of the segment, as enqueue_steps advanced it during the
+        // capture (a general member flips its buffers once per step: an even
+        // segment leaves them)
+        for (int i : b->tiled) cb::advance_host(b->mem[(size_t)i], kSegment);
+        for (int i : b->paired) cb::advance_host_multi(b->mem[(size_t)i], kSegment);
+        for (int i : b->general) {
+            cb_sim *s = b->mem[(size_t)i];
             for (int it = 0; it < kSegment; ++it) s->t += s->p.dt;
             s->step += kSegment;
         }
         left -= kSegment;
+        if (left >= kSegment) {   // the parities for the next replay
+            r = upload_table(b, kSegment);
+            if (r != CB_OK) return r;
+        }
     }
     if (left > 0) {
         r = upload_table(b, left);

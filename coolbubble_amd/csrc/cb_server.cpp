@@ -229,12 +229,12 @@ cb_status tick(cb_server *sv)
         if (m[i] > L) L = m[i];
         if (q->sim->lagged) any_lagged = true;
     }
-    // batched tables: one-step tiled requests first, then two-step ones
+    // batched tables: one-step tiled requests first, then multi-step ones
     std::vector<size_t> single, pair, general;
     for (size_t i = 0; i < act.size(); ++i) {
         const cb_sim *s = act[i]->sim;
         if (s->variant < 0) general.push_back(i);
-        else (s->variants[(size_t)s->variant].fuse == 2 ? pair : single).push_back(i);
+        else (s->variants[(size_t)s->variant].fuse >= 2 ? pair : single).push_back(i);
     }
     const size_t ntab = single.size() + pair.size();
     if (ntab > sv->tab_cap) {
@@ -264,36 +264,43 @@ cb_status tick(cb_server *sv)
         tiles1 += cb::member_work(sv->h_tab[t]);
         if (n > L1) L1 = n;
     }
-    // two-step table: ceil(L2 / 2) launches, launch p covering steps 2p, 2p + 1
+    // multi-step table: a member's launch covers F steps (2 or 3), then its
+    // leftover steps run one per launch on its companion tile; energies in
+    // its last launch
     cb::BatchMember *tab2 = sv->h_tab + single.size();
-    int tiles2 = 0, L2 = 0;
+    int tiles2 = 0, L2 = 0;     // L2: launches of the longest member
     size_t shm2 = 0;
-    std::vector<char> energy2((size_t)(L + 1) / 2, 0);
+    std::vector<int> last2(pair.size());
     for (size_t t = 0; t < pair.size(); ++t) {
         cb::fill_member(tab2[t], act[pair[t]]->sim, tiles2, fp32, &shm2);
         const int n = m[pair[t]];
         tab2[t].nsteps = n;
         tab2[t].energy_it = n - 1;
-        energy2[(size_t)(n - 1) / 2] = 1;
+        last2[t] = cb::multi_launches(tab2[t].fused, n) - 1;
         tiles2 += cb::member_work(tab2[t]);
-        if (n > L2) L2 = n;
+        if (last2[t] + 1 > L2) L2 = last2[t] + 1;
     }
+    std::vector<char> energy2((size_t)L2, 0);
+    for (size_t t = 0; t < pair.size(); ++t) energy2[(size_t)last2[t]] = 1;
     hipError_t e = hipSuccess;
     if (ntab) e = hipMemcpyAsync(sv->d_tab, sv->h_tab, sizeof(cb::BatchMember) * ntab, hipMemcpyHostToDevice, sv->st);
     // members taking part in each launch without measuring energies (they
     // get a plain launch of their own beside the energy launch)
-    std::vector<char> plain1((size_t)L, 0), plain2((size_t)(L + 1) / 2, 0);
+    std::vector<char> plain1((size_t)L, 0), plain2((size_t)L2, 0);
     for (size_t t = 0; t < single.size(); ++t) {
         if (sv->h_tab[t].lagged) continue;
         for (int it = 0; it < sv->h_tab[t].nsteps - 1; ++it) plain1[(size_t)it] = 1;
     }
     for (size_t t = 0; t < pair.size(); ++t)
-        for (int p = 0; 2 * p + 2 < tab2[t].nsteps; ++p) plain2[(size_t)p] = 1;
+        for (int l = 0; l < last2[t]; ++l) plain2[(size_t)l] = 1;
     const int cap = sv->cus * 8;
     // the smallest kernels that step every member (fewer registers)
     int items1 = 1, items2 = 1;
     for (size_t i : single) items1 = std::max(items1, cb::member_items(act[i]->sim));
     for (size_t i : pair) items2 = std::max(items2, cb::member_items(act[i]->sim));
+    bool two = false, three = false;    // F of the multi-step requests: one kernel for each, or both
+    for (size_t t = 0; t < pair.size(); ++t) (tab2[t].fused == 3 ? three : two) = true;
+    const int fuses2 = two && three ? 0 : (three ? 3 : 2);
     for (int it = 0; it < L1 && e == hipSuccess; ++it) {
         const bool energy = energy1[(size_t)it] != 0;
         e = cb::launch_batch_step(fp32, energy, plain1[(size_t)it], items1, 1, sv->d_tab, (int)single.size(),
@@ -301,11 +308,11 @@ cb_status tick(cb_server *sv)
         if (e == hipSuccess && energy) e = cb::launch_batch_finalize(sv->d_tab, (int)single.size(), it, sv->st);
     }
     const cb::BatchMember *d_tab2 = sv->d_tab + single.size();
-    for (int p = 0; 2 * p < L2 && e == hipSuccess; ++p) {
-        const bool energy = energy2[(size_t)p] != 0;
-        e = cb::launch_batch_step2(fp32, energy, plain2[(size_t)p], items2, d_tab2, (int)pair.size(), tiles2,
-                                   std::min(tiles2, cap), shm2, p, sv->st);
-        if (e == hipSuccess && energy) e = cb::launch_batch_finalize2(d_tab2, (int)pair.size(), p, sv->st);
+    for (int l = 0; l < L2 && e == hipSuccess; ++l) {
+        const bool energy = energy2[(size_t)l] != 0;
+        e = cb::launch_batch_multi(fp32, energy, plain2[(size_t)l], items2, fuses2, d_tab2, (int)pair.size(), tiles2,
+                                   std::min(tiles2, cap), shm2, l, sv->st);
+        if (e == hipSuccess && energy) e = cb::launch_batch_finalize_multi(d_tab2, (int)pair.size(), l, sv->st);
     }
     if (e != hipSuccess) { set_error(std::string("server tick: ") + hipGetErrorString(e)); return CB_HIP_ERROR; }
     for (size_t i : general) {
@@ -313,7 +320,7 @@ cb_status tick(cb_server *sv)
         if (r != CB_OK) return r;
     }
     for (size_t i : single) cb::advance_host(act[i]->sim, m[i]);
-    for (size_t i : pair) cb::advance_host2(act[i]->sim, m[i]);
+    for (size_t i : pair) cb::advance_host_multi(act[i]->sim, m[i]);
     if (hipStreamSynchronize(sv->st) != hipSuccess) { set_error("server tick: stream failed"); return CB_HIP_ERROR; }
     // events
     std::vector<Req *> keep;

@@ -71,10 +71,11 @@ bool params_ok(const cb_params *p, std::string &why);
 constexpr int kBatchItems = 2;
 int batch_variant(const cb_sim *s);
 int pick_variant(const cb_sim *s, int items_max, int pf, int nt, int ni_pref);
-// a static member's two-step tile for the batched two-step launch, or -1
-int pick_variant2(const cb_sim *s);
-// the variant the server gives a request: pick_variant2 when there is one,
-// else batch_variant
+// a static member's two- or three-step tile (at most max_fuse steps) for
+// the batched multi-step launch, or -1
+int pick_variant_multi(const cb_sim *s, int max_fuse);
+// the variant the server gives a request: a two-step tile from
+// pick_variant_multi when there is one, else batch_variant
 int server_variant(const cb_sim *s);
 int companion_variant(const cb_sim *s, const TiledCfg &f);
 // table entry of a member with its selected tiled variant (one- or two-step)
@@ -87,7 +88,7 @@ int member_work(const BatchMember &m);
 int member_items(const cb_sim *s);
 // host side of n steps run by batched launches (one-step / two-step members)
 void advance_host(cb_sim *s, int n);
-void advance_host2(cb_sim *s, int n);
+void advance_host_multi(cb_sim *s, int n);
 cb_status select_variant(cb_sim *s, int variant);
 // the variant cb_create selects (-1 = the general kernel)
 int default_variant(const cb_sim *s);

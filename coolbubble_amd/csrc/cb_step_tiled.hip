@@ -21,6 +21,7 @@
 
 #include "cb_device.hpp"
 #include "cb_tile2.hpp"
+#include "cb_tile3.hpp"
 #include "cb_tile_common.hpp"
 #include "cb_tiled.hpp"
 
@@ -30,7 +31,7 @@ namespace cb {
 // [lt's i-chunk].  lds holds two plane tiles (double buffer).  Ends with an
 // LDS barrier so a caller may run the next tile on the same LDS.
 template <typename T, int D, int VW, int ITEMS, int NTHR, bool LAGGED, bool ENERGY, int PF>
-__device__ inline void tile_step(const SimConst &k, const TiledGeom &g, int lt, const T *__restrict__ phi_in,
+__device__ __forceinline__ void tile_step(const SimConst &k, const TiledGeom &g, int lt, const T *__restrict__ phi_in,
                                  T *__restrict__ phi_out, const T *pi_in, T *pi_out,
                                  const DevScalars *__restrict__ sc, double *__restrict__ partials, T *lds)
 {
@@ -332,18 +333,26 @@ hipError_t launch_batch_finalize(const BatchMember *table, int nmem, int step_it
     return hipGetLastError();
 }
 
-// Batched two-step launch: every tile of every static two-step member (DSEL
-// as in k_batch_step).
-template <typename T, bool ENERGY, int ITEMS, int DSEL>
+// Batched multi-step launch L of a sequence: every tile of every static
+// member with a two- or three-step tile (BatchMember::fused = F, DSEL as in
+// k_batch_step).  A member taking n steps runs n / F multi-step launches
+// (phi and pi flip once each), then n % F launches of its one-step
+// companion (phi flips, pi in place), then is skipped; each launch is the
+// same tile of the same variant as a lone run, so a member steps
+// bit-identically alone or batched.  Energies as k_batch_step, at the
+// member's energy step (the last one its launch covers).  FSEL = 2 or 3
+// holds only the tile of that many steps (every member in the table has
+// F = FSEL), 0 both: the kernel that holds both takes more registers (90
+// VGPRs against 74 for the two-step one alone, FP32, one vector per thread).
+template <typename T, bool ENERGY, int ITEMS, int DSEL, int FSEL>
 __global__ void __launch_bounds__(256)
-k_batch_step2(const BatchMember *__restrict__ table, int nmem, int total, int pit, int sel)
+k_batch_multi(const BatchMember *__restrict__ table, int nmem, int total, int L, int sel)
 {
     constexpr int VW = sizeof(T) == 4 ? 4 : 2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     const int G = gridDim.x, b = blockIdx.x;
     const int lb = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
-    const int s0 = 2 * pit;
     for (int w = lb; w < total; w += G) {
         int lo = 0, hi = nmem - 1;
         while (lo < hi) {
@@ -353,22 +362,29 @@ k_batch_step2(const BatchMember *__restrict__ table, int nmem, int total, int pi
         }
         const BatchMember &m = table[lo];
         if (DSEL != 0 && m.k.D != DSEL) continue;
-        if (s0 >= m.nsteps) continue;
-        const bool two = s0 + 1 < m.nsteps;
-        const bool want = m.energy_it < 0 || m.energy_it == (two ? s0 + 1 : s0);
+        const int F = m.fused, q = m.nsteps / F, r = m.nsteps - q * F;
+        if (L >= q + r) continue;
+        const bool multi = L < q;
+        const int last = multi ? F * L + F - 1 : F * q + (L - q);
+        const bool want = m.energy_it < 0 || m.energy_it == last;
         if (ENERGY ? !want : (sel == 1 && want)) continue;
         const int lt = w - m.tile_begin;
-        const int pphi = (m.par_phi + pit) & 1;
-        const int ppi = (m.par_pi + pit) & 1;
+        const int pphi = (m.par_phi + L) & 1;
+        const int ppi = (m.par_pi + (multi ? L : q)) & 1;
         const T *phi_in = (const T *)m.phi[pphi];
         T *phi_out = (T *)m.phi[pphi ^ 1];
-        if (two) {                 // both steps: the two-step tile
+        if (multi) {
             if (lt >= m.ntiles) continue;
             const T *pi_in = (const T *)m.pi[ppi];
             T *pi_out = (T *)m.pi[ppi ^ 1];
-            if (DSEL != 2 && m.k.D == 3) tile_step2<T, 3, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
-            else if (DSEL != 3) tile_step2<T, 2, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
-        } else {                   // the odd last step: the one-step companion, pi in place
+            if (FSEL != 2 && (FSEL == 3 || F == 3)) {
+                if (DSEL != 2 && m.k.D == 3) tile_step3<T, 3, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
+                else if (DSEL != 3) tile_step3<T, 2, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
+            } else if (FSEL != 3) {
+                if (DSEL != 2 && m.k.D == 3) tile_step2<T, 3, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
+                else if (DSEL != 3) tile_step2<T, 2, ITEMS, 256, ENERGY>(m.k, m.g, lt, phi_in, phi_out, pi_in, pi_out, m.sc, m.partials, lds);
+            }
+        } else {                   // a step left over: the one-step companion, pi in place
             if (lt >= m.ntiles1) continue;
             T *pi_io = (T *)m.pi[ppi];
             if (DSEL != 2 && m.k.D == 3) tile_step<T, 3, VW, ITEMS, 256, false, ENERGY, 1>(m.k, m.g1, lt, phi_in, phi_out, pi_io, pi_io, m.sc, m.partials, lds);
@@ -377,47 +393,57 @@ k_batch_step2(const BatchMember *__restrict__ table, int nmem, int total, int pi
     }
 }
 
-__global__ void __launch_bounds__(256) k_batch_finalize2(const BatchMember *__restrict__ table, int pit)
+__global__ void __launch_bounds__(256) k_batch_finalize_multi(const BatchMember *__restrict__ table, int L)
 {
     const BatchMember &m = table[blockIdx.x];
-    const int s0 = 2 * pit;
-    if (s0 >= m.nsteps) return;
-    const bool two = s0 + 1 < m.nsteps;
-    if (m.energy_it >= 0 && m.energy_it != (two ? s0 + 1 : s0)) return;
-    finalize_member(m.k, m.partials, two ? m.ntiles : m.ntiles1, m.sc, 1);
+    const int F = m.fused, q = m.nsteps / F, r = m.nsteps - q * F;
+    if (L >= q + r) return;
+    const bool multi = L < q;
+    const int last = multi ? F * L + F - 1 : F * q + (L - q);
+    if (m.energy_it >= 0 && m.energy_it != last) return;
+    finalize_member(m.k, m.partials, multi ? m.ntiles : m.ntiles1, m.sc, 1);
 }
 
-template <typename T, bool ENERGY, int DSEL>
-static void batch2_launch_i(int items, const BatchMember *table, int nmem, int total, int grid, size_t shm, int pit,
-                            int sel, hipStream_t st)
+template <typename T, bool ENERGY, int DSEL, int FSEL>
+static void batch_multi_launch_i(int items, const BatchMember *table, int nmem, int total, int grid, size_t shm, int L,
+                                 int sel, hipStream_t st)
 {
-    if (items == 1) k_batch_step2<T, ENERGY, 1, DSEL><<<grid, 256, shm, st>>>(table, nmem, total, pit, sel);
-    else k_batch_step2<T, ENERGY, 2, DSEL><<<grid, 256, shm, st>>>(table, nmem, total, pit, sel);
+    if (items == 1) k_batch_multi<T, ENERGY, 1, DSEL, FSEL><<<grid, 256, shm, st>>>(table, nmem, total, L, sel);
+    else k_batch_multi<T, ENERGY, 2, DSEL, FSEL><<<grid, 256, shm, st>>>(table, nmem, total, L, sel);
 }
 
-hipError_t launch_batch_step2(int fp32, int energy, int split, int items, const BatchMember *table, int nmem,
-                              int total_tiles, int grid, size_t shm, int pit, hipStream_t st)
+template <typename T, int FSEL>
+static void batch_multi_launch_f(int energy, int split, int items, const BatchMember *table, int nmem, int total,
+                                 int grid, size_t shm, int L, hipStream_t st)
 {
-    if (items > 2) return hipErrorInvalidValue;
-    if (fp32) {
-        if (!energy || split) batch2_launch_i<float, false, 0>(items, table, nmem, total_tiles, grid, shm, pit, energy ? 1 : 0, st);
-        if (energy) {
-            batch2_launch_i<float, true, 2>(items, table, nmem, total_tiles, grid, shm, pit, 2, st);
-            batch2_launch_i<float, true, 3>(items, table, nmem, total_tiles, grid, shm, pit, 2, st);
-        }
-    } else {
-        if (!energy || split) batch2_launch_i<double, false, 0>(items, table, nmem, total_tiles, grid, shm, pit, energy ? 1 : 0, st);
-        if (energy) {
-            batch2_launch_i<double, true, 2>(items, table, nmem, total_tiles, grid, shm, pit, 2, st);
-            batch2_launch_i<double, true, 3>(items, table, nmem, total_tiles, grid, shm, pit, 2, st);
-        }
+    if (!energy || split) batch_multi_launch_i<T, false, 0, FSEL>(items, table, nmem, total, grid, shm, L, energy ? 1 : 0, st);
+    if (energy) {
+        batch_multi_launch_i<T, true, 2, FSEL>(items, table, nmem, total, grid, shm, L, 2, st);
+        batch_multi_launch_i<T, true, 3, FSEL>(items, table, nmem, total, grid, shm, L, 2, st);
     }
+}
+
+template <typename T>
+static void batch_multi_launch_t(int fuses, int energy, int split, int items, const BatchMember *table, int nmem,
+                                 int total, int grid, size_t shm, int L, hipStream_t st)
+{
+    if (fuses == 2) batch_multi_launch_f<T, 2>(energy, split, items, table, nmem, total, grid, shm, L, st);
+    else if (fuses == 3) batch_multi_launch_f<T, 3>(energy, split, items, table, nmem, total, grid, shm, L, st);
+    else batch_multi_launch_f<T, 0>(energy, split, items, table, nmem, total, grid, shm, L, st);
+}
+
+hipError_t launch_batch_multi(int fp32, int energy, int split, int items, int fuses, const BatchMember *table,
+                              int nmem, int total_tiles, int grid, size_t shm, int L, hipStream_t st)
+{
+    if (items > 2 || (fuses != 0 && fuses != 2 && fuses != 3)) return hipErrorInvalidValue;
+    if (fp32) batch_multi_launch_t<float>(fuses, energy, split, items, table, nmem, total_tiles, grid, shm, L, st);
+    else batch_multi_launch_t<double>(fuses, energy, split, items, table, nmem, total_tiles, grid, shm, L, st);
     return hipGetLastError();
 }
 
-hipError_t launch_batch_finalize2(const BatchMember *table, int nmem, int pit, hipStream_t st)
+hipError_t launch_batch_finalize_multi(const BatchMember *table, int nmem, int L, hipStream_t st)
 {
-    if (nmem > 0) k_batch_finalize2<<<nmem, 256, 0, st>>>(table, pit);
+    if (nmem > 0) k_batch_finalize_multi<<<nmem, 256, 0, st>>>(table, L);
     return hipGetLastError();
 }
 

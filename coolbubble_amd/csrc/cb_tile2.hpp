@@ -30,7 +30,7 @@
 namespace cb {
 
 template <typename T, int D, int ITEMS, int NTHR, bool ENERGY>
-__device__ inline void tile_step2(const SimConst &k, const TiledGeom &g, int lt, const T *__restrict__ phi_in,
+__device__ __forceinline__ void tile_step2(const SimConst &k, const TiledGeom &g, int lt, const T *__restrict__ phi_in,
                                   T *__restrict__ phi_out, const T *__restrict__ pi_in, T *__restrict__ pi_out,
                                   const DevScalars *__restrict__ sc, double *__restrict__ partials, T *lds)
 {

@@ -32,7 +32,7 @@ namespace cb {
 // values, lrow its row in LDS (j taps at lrow -+ N in 3D, k taps at the
 // vector's edges), piv the incoming pi.  Writes the kicked pi and drifted phi.
 template <typename T, int D, int VW, bool ENERGY>
-__device__ inline void leap_vec(const SimConst &k, const Vec<T, VW> &c, const Vec<T, VW> &mM, const Vec<T, VW> &mP,
+__device__ __forceinline__ void leap_vec(const SimConst &k, const Vec<T, VW> &c, const Vec<T, VW> &mM, const Vec<T, VW> &mP,
                                 const T *lrow, int col, const Vec<T, VW> &piv, T fac_lap, T fac_pot,
                                 const StepF32 &sf, T dt, Vec<T, VW> &pout, Vec<T, VW> &phout,
                                 const DevScalars *sc, double (&acc)[NQ])
@@ -80,7 +80,7 @@ __device__ inline void leap_vec(const SimConst &k, const Vec<T, VW> &c, const Ve
 }
 
 template <typename T, int D, int ITEMS, int NTHR, bool ENERGY>
-__device__ inline void tile_step3(const SimConst &k, const TiledGeom &g, int lt, const T *__restrict__ phi_in,
+__device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g, int lt, const T *__restrict__ phi_in,
                                   T *__restrict__ phi_out, const T *__restrict__ pi_in, T *__restrict__ pi_out,
                                   const DevScalars *__restrict__ sc, double *__restrict__ partials, T *lds)
 {

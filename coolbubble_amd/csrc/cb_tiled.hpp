@@ -58,8 +58,9 @@ struct BatchMember {
     int par_phi, par_pi;
     int nsteps;         // steps this member takes in the current launch sequence
     int energy_it;      // step index of this member's energy step; -1: every energy launch
-    // two-step members (launch_batch_step2): g / ntiles are the two-step
-    // tile's, g1 / ntiles1 the one-step companion's (odd last step)
+    // multi-step members (launch_batch_multi): fused = steps per launch (2
+    // or 3; 0 for one-step members); g / ntiles are the multi-step tile's,
+    // g1 / ntiles1 the one-step companion's (the leftover steps)
     int fused;
     int ntiles1;
     TiledGeom g1;
@@ -86,16 +87,22 @@ hipError_t launch_batch_step(int fp32, int energy, int split, int items, int pf,
                              int total_tiles, int grid, size_t shm, int step_it, hipStream_t st);
 // Finalise the energies of the members that measured them in launch step_it.
 hipError_t launch_batch_finalize(const BatchMember *table, int nmem, int step_it, hipStream_t st);
-// Batched two-step launch number pit of a sequence (static members with a
-// two-step variant of 256 threads and at most `items` vectors per thread):
-// it covers steps 2 pit and 2 pit + 1.  A member with both steps left runs
-// its two-step tile (phi and pi flip once per launch); one with only step
-// 2 pit left runs its one-step companion tile (the odd last step, as a lone
-// run takes it); the others are skipped.  Energies as launch_batch_step, at
-// the member's energy step.  A member spans max(ntiles, ntiles1) work items.
-hipError_t launch_batch_step2(int fp32, int energy, int split, int items, const BatchMember *table, int nmem,
-                              int total_tiles, int grid, size_t shm, int pit, hipStream_t st);
-hipError_t launch_batch_finalize2(const BatchMember *table, int nmem, int pit, hipStream_t st);
+// Batched multi-step launch number L of a sequence, for static members with
+// a two- or three-step variant (BatchMember::fused = steps per launch) of 256
+// threads and at most `items` vectors per thread.  A member taking nsteps
+// steps runs nsteps / F multi-step launches (phi and pi flip once each), then
+// nsteps % F launches of its one-step companion tile (as a lone run takes
+// the leftover steps), and is skipped after that.  Energies as
+// launch_batch_step, at the member's energy step.  A member spans
+// max(ntiles, ntiles1) work items.  fuses: 2 or 3 when every member has
+// that F (a leaner kernel), 0 for a mix.
+hipError_t launch_batch_multi(int fp32, int energy, int split, int items, int fuses, const BatchMember *table,
+                              int nmem, int total_tiles, int grid, size_t shm, int L, hipStream_t st);
+hipError_t launch_batch_finalize_multi(const BatchMember *table, int nmem, int L, hipStream_t st);
+// launches a member with F steps per launch takes for n steps, and the index
+// of the last step that its launch L covers
+inline int multi_launches(int F, int n) { return n / F + n % F; }
+inline int multi_last_step(int F, int n, int L) { return L < n / F ? F * L + F - 1 : F * (n / F) + (L - n / F); }
 size_t tiled_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
 TiledGeom tiled_geom(const SimConst &k, const TiledCfg &c);
 

@@ -25,6 +25,9 @@ REQS = [
     req(2, 45, 5.625, 4.0, 1, 50, 1, nBubbles=3),
     req(2, 128, 16.0, 6.0, -3, 170, 4, Trad=0.2),
     req(3, 32, 4.0, 4.0, 1, 45, 5, nBubbles=4, rand_seed=9),
+    # a lattice on which a lone run and a batch take three steps per launch
+    # (FP32): the server gives it two
+    req(3, 192, 24.0, 4.0, -2, 47, 2),
 ]
 
 
@@ -45,6 +48,8 @@ def test_results_identical_alone_and_in_the_mix(precision):
         v = cb.server_variant(alone)
         if v >= 0:
             alone.select_kernel(v)
+        if p["nGrids"] == 192:
+            assert alone.variant_info(v)["fuse"] == 2
         rows = [alone.energy_row()]
         spectra = [alone.spectrum()]
         steps_done = 0
