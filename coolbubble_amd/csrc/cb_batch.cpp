@@ -90,11 +90,10 @@ int pick_variant_multi(const cb_sim *s, int max_fuse)
     return -1;
 }
 
-// Two steps per launch: a server's launches mix requests of every size, and
-// a table holding both tile kinds needs the kernel that holds both (fewer
-// waves per SIMD for all); with three steps allowed for the large requests
-// the serving mix of 200 Poisson-arriving requests ran at p50 3.15 s against
-// 2.86 s.
+// Two steps per launch: with three allowed for the large requests (3D from
+// N = 192) a serving mix of 200 Poisson-arriving requests of every size ran
+// at p50 2.9-3.0 s against 1.9 s (a server's table mixes both tile kinds,
+// and its short runs to the next event leave steps over for the companion).
 int server_variant(const cb_sim *s)
 {
     const int vm = pick_variant_multi(s, 2);
