@@ -140,7 +140,7 @@ int default_variant(const cb_sim *s)
     if (s->variants.empty() || (D == 3 ? s->nsites < (1LL << 19) : N < 512)) return -1;
     struct Want { int fuse, nthr, TR, NI, pf, nt; } w;
     if (s->lagged) w = s->fp32 ? Want{1, 256, 4, 16, 1, 1} : Want{1, 256, 2, 32, 1, 1};
-    else if (D == 3 && N >= 512) w = {3, 1024, 4, 32, 1, 1};
+    else if (D == 3 && N >= 512) w = s->fp32 ? Want{3, 1024, 4, 32, 2, 1} : Want{2, 1024, 4, 64, 2, 1};
     else if (D == 3 && N >= 192) w = {3, 512, 4, 32, 1, 0};
     else if (D == 3) w = {1, 256, 8, 8, 1, 1};
     else if (N >= 2048) w = {3, 512, 1, N / 256, 1, 0};
@@ -180,13 +180,13 @@ cb_status select_variant(cb_sim *s, int variant)
         // threads the three-step tile and the two-vector two-step tile spill
         // them, so those launches take the same tile (TR, NI: the same sites,
         // so identical fields; the energy sums in another order) with 512
-        // threads where such a variant exists
+        // threads and no prefetch where such a variant exists
         const TiledCfg &f = s->variants[(size_t)variant];
         int e = variant;
         if (f.nthr == 1024 && (f.fuse == 3 || f.items > 1))
             for (size_t v = 0; v < s->variants.size(); ++v) {
                 const TiledCfg &c = s->variants[v];
-                if (c.fuse == f.fuse && c.TR == f.TR && c.NI == f.NI && c.nt == f.nt && c.pf == f.pf && c.nthr == 512) {
+                if (c.fuse == f.fuse && c.TR == f.TR && c.NI == f.NI && c.nt == f.nt && c.pf == 1 && c.nthr == 512) {
                     e = (int)v;
                     break;
                 }

@@ -8,7 +8,7 @@
 
 namespace cb {
 
-template <typename T, int D, int ITEMS, int NTHR, bool ENERGY>
+template <typename T, int D, int ITEMS, int NTHR, bool ENERGY, int PF>
 __global__ void __launch_bounds__(NTHR)
 k_step3_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restrict__ phi_out,
               const T *__restrict__ pi_in, T *__restrict__ pi_out, const DevScalars *__restrict__ sc,
@@ -17,7 +17,7 @@ k_step3_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restri
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int G = gridDim.x, b = blockIdx.x;
     const int lt = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;   // XCD-contiguous tiles
-    tile_step3<T, D, ITEMS, NTHR, ENERGY>(k, g, lt, phi_in, phi_out, pi_in, pi_out, sc, partials,
+    tile_step3<T, D, ITEMS, NTHR, ENERGY, PF>(k, g, lt, phi_in, phi_out, pi_in, pi_out, sc, partials,
                                           reinterpret_cast<T *>(smem_raw));
 }
 
@@ -50,7 +50,8 @@ bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c)
     if (c.vw != want_vw || k.N % c.vw != 0 || k.N < 8) return false;
     if (c.items != 1 && c.items != 2 && c.items != 4) return false;
     if (c.nthr != 256 && c.nthr != 512 && c.nthr != 1024) return false;
-    if (c.NI < 1 || c.pf != 1 || (c.nt != 0 && c.nt != 1)) return false;
+    if (c.NI < 1 || (c.nt != 0 && c.nt != 1)) return false;
+    if (c.pf != 1 && !(c.pf == 2 && c.items == 1)) return false;   // prefetch: one vector per thread
     if (k.D == 3 && (c.TR < 1 || k.N % c.TR != 0)) return false;
     const int NV = k.N / c.vw;
     const int rowsA = (k.D == 3) ? c.TR + 4 : 1;
@@ -66,11 +67,19 @@ static void launch3_t(const SimConst &k, int energy, const TiledCfg &c, const St
 {
     TiledGeom g = geom3(k, c);
     dim3 grid(g.ntr * g.nic), block(NTHR);
+    // (the energy launches do not prefetch: their FP64 sums take the registers)
     if (energy)
-        k_step3_tiled<T, D, ITEMS, NTHR, true><<<grid, block, shm, st>>>(
+        k_step3_tiled<T, D, ITEMS, NTHR, true, 1><<<grid, block, shm, st>>>(
             k, g, (const T *)p.phi_in, (T *)p.phi_out, (const T *)p.pi_in, (T *)p.pi_out, sc, partials);
-    else
-        k_step3_tiled<T, D, ITEMS, NTHR, false><<<grid, block, shm, st>>>(
+    else if constexpr (ITEMS == 1) {
+        if (c.pf == 2)
+            k_step3_tiled<T, D, 1, NTHR, false, 2><<<grid, block, shm, st>>>(
+                k, g, (const T *)p.phi_in, (T *)p.phi_out, (const T *)p.pi_in, (T *)p.pi_out, sc, partials);
+        else
+            k_step3_tiled<T, D, 1, NTHR, false, 1><<<grid, block, shm, st>>>(
+                k, g, (const T *)p.phi_in, (T *)p.phi_out, (const T *)p.pi_in, (T *)p.pi_out, sc, partials);
+    } else
+        k_step3_tiled<T, D, ITEMS, NTHR, false, 1><<<grid, block, shm, st>>>(
             k, g, (const T *)p.phi_in, (T *)p.phi_out, (const T *)p.pi_in, (T *)p.pi_out, sc, partials);
 }
 
@@ -117,7 +126,7 @@ int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
     int n = 0;
     const int vw = fp32 ? 4 : 2;
     const int NV = k.N / vw;
-    auto push = [&](int nthr, int TR, int NI, int nt) {
+    auto push = [&](int nthr, int TR, int NI, int nt, int pf) {
         const int rowsA = (k.D == 3) ? TR + 4 : 1;
         int items = (rowsA * NV + nthr - 1) / nthr;
         if (items == 3) items = 4;
@@ -129,18 +138,27 @@ int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
         if (items < 1) items = 1;
         // (more vectors per thread than this spill registers)
         if (items > (nthr == 256 ? 4 : nthr == 512 ? 2 : 1)) return;
-        TiledCfg c{vw, items, nthr, TR, NI, 1, nt, 3};
+        if (pf == 2 && items != 1) return;
+        TiledCfg c{vw, items, nthr, TR, NI, pf, nt, 3};
         if (n < cap && step3_valid(k, fp32, c)) out[n++] = c;
     };
+    // without prefetch, then (one vector per thread) with it
     if (k.D == 3) {
         for (int TR : {4, 2, 6, 8, 1})
             for (int nthr : {512, 1024, 256})
                 for (int NI : {32, 16})
-                    for (int nt : {1, 0}) push(nthr, TR, NI, nt);
+                    for (int nt : {1, 0}) push(nthr, TR, NI, nt, 1);
+        for (int TR : {4, 2, 6, 8, 1})
+            for (int nthr : {1024, 512, 256})
+                for (int NI : {32, 16, 64})
+                    for (int nt : {1, 0}) push(nthr, TR, NI, nt, 2);
     } else {
         for (int nthr : {256, 512})
             for (int NI : {16, 32, 8})
-                for (int nt : {1, 0}) push(nthr, 1, NI, nt);
+                for (int nt : {1, 0}) push(nthr, 1, NI, nt, 1);
+        for (int nthr : {256, 512})
+            for (int NI : {16, 32, 8})
+                for (int nt : {1, 0}) push(nthr, 1, NI, nt, 2);
     }
     return n;
 }

@@ -20,7 +20,8 @@
 // step), with the kinetic energy of the incoming pi_{n+1/2} for the
 // time-centred Ekin.  pi ping-pongs (the halo rows of pi_n are read while
 // neighbours write pi_{n+2}).  The LDS holds (TR+4) + (TR+2) rows of N (3D)
-// or 2 rows (2D); the caller's block has NTHR threads.
+// or 2 rows (2D); the caller's block has NTHR threads.  PF = 2 issues the
+// HBM loads of the next plane one plane ahead (as the three-step tile).
 #pragma once
 
 #include "cb_device.hpp"
@@ -29,7 +30,7 @@
 
 namespace cb {
 
-template <typename T, int D, int ITEMS, int NTHR, bool ENERGY>
+template <typename T, int D, int ITEMS, int NTHR, bool ENERGY, int PF = 1>
 __device__ __forceinline__ void tile_step2(const SimConst &k, const TiledGeom &g, int lt, const T *__restrict__ phi_in,
                                   T *__restrict__ phi_out, const T *__restrict__ pi_in, T *__restrict__ pi_out,
                                   const DevScalars *__restrict__ sc, double *__restrict__ partials, T *lds)
@@ -108,16 +109,52 @@ __device__ __forceinline__ void tile_step2(const SimConst &k, const TiledGeom &g
             }
     }
 
+    // PF == 2: loads for the next plane, in flight during this one (native
+    // vectors: arrays of Vec carried around the loop went to scratch)
+    typedef T nvec __attribute__((ext_vector_type(VW)));
+    nvec nx_n0[ITEMS], nx_pi[ITEMS], nx_h[ITEMS];
+    if constexpr (PF == 2) {
+        const long long b0 = (long long)wrapN(i0 - 1) * NN, b1 = (long long)wrapN(i0) * NN;
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u) {
+            nx_n0[u] = nx_pi[u] = nx_h[u] = (nvec)(T)0;
+            if (ok[u]) {
+                nx_n0[u] = *reinterpret_cast<const nvec *>(phi_in + b1 + off[u]);
+                nx_pi[u] = *reinterpret_cast<const nvec *>(pi_in + b0 + off[u]);
+            }
+            if (D == 3 && hok[u]) nx_h[u] = *reinterpret_cast<const nvec *>(phi_in + b0 + hoff[u]);
+        }
+    }
+
     for (int p = i0 - 1; p <= i1; ++p) {
         const long long base = (long long)wrapN(p) * NN;
         const long long baseN = (long long)wrapN(p + 1) * NN;
-        V pi0[ITEMS];
+        V pi0[ITEMS], hv[ITEMS];
+        if constexpr (PF == 2) {
+            // (p + 2 <= i1 + 2 <= N + 2: wrapN keeps every plane in the box)
+            const long long bq1 = (long long)wrapN(p + 2) * NN;
 #pragma unroll
-        for (int u = 0; u < ITEMS; ++u)
-            if (ok[u]) {
-                n0[u] = *reinterpret_cast<const V *>(phi_in + baseN + off[u]);
-                pi0[u] = *reinterpret_cast<const V *>(pi_in + base + off[u]);
+            for (int u = 0; u < ITEMS; ++u) {
+#pragma unroll
+                for (int e = 0; e < VW; ++e) {
+                    n0[u].v[e] = nx_n0[u][e];
+                    pi0[u].v[e] = nx_pi[u][e];
+                    hv[u].v[e] = nx_h[u][e];
+                }
+                if (ok[u]) {
+                    nx_n0[u] = *reinterpret_cast<const nvec *>(phi_in + bq1 + off[u]);
+                    nx_pi[u] = *reinterpret_cast<const nvec *>(pi_in + baseN + off[u]);
+                }
+                if (D == 3 && hok[u]) nx_h[u] = *reinterpret_cast<const nvec *>(phi_in + baseN + hoff[u]);
             }
+        } else {
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u)
+                if (ok[u]) {
+                    n0[u] = *reinterpret_cast<const V *>(phi_in + baseN + off[u]);
+                    pi0[u] = *reinterpret_cast<const V *>(pi_in + base + off[u]);
+                }
+        }
         // stage phi_n(p) and phi_n+1(p-1)
 #pragma unroll
         for (int u = 0; u < ITEMS; ++u)
@@ -125,7 +162,10 @@ __device__ __forceinline__ void tile_step2(const SimConst &k, const TiledGeom &g
         if constexpr (D == 3) {
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u)
-                if (hok[u]) *reinterpret_cast<V *>(L0 + hlds[u]) = *reinterpret_cast<const V *>(phi_in + base + hoff[u]);
+                if (hok[u]) {
+                    if constexpr (PF == 2) *reinterpret_cast<V *>(L0 + hlds[u]) = hv[u];
+                    else *reinterpret_cast<V *>(L0 + hlds[u]) = *reinterpret_cast<const V *>(phi_in + base + hoff[u]);
+                }
         }
         const bool doB = p - 1 >= i0;
         if (doB) {

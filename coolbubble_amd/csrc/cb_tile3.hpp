@@ -20,6 +20,10 @@
 // Energies, when asked for, are those of step C with the kinetic energy of
 // its incoming pi_{n+3/2} (the time-centred Ekin).  pi ping-pongs.  2D: one
 // row, the three levels on rows p, p-1, p-2 of the marching axis.
+// PF = 2 issues the HBM loads of the next plane (phi_n at p+2, pi at p+1,
+// the outer L0 rows of p+1) one plane ahead, so their latency overlaps a
+// plane of arithmetic instead of stalling the block at every plane (12 more
+// VGPRs per vector item).
 #pragma once
 
 #include "cb_device.hpp"
@@ -79,7 +83,7 @@ __device__ __forceinline__ void leap_vec(const SimConst &k, const Vec<T, VW> &c,
     }
 }
 
-template <typename T, int D, int ITEMS, int NTHR, bool ENERGY>
+template <typename T, int D, int ITEMS, int NTHR, bool ENERGY, int PF = 1>
 __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g, int lt, const T *__restrict__ phi_in,
                                   T *__restrict__ phi_out, const T *__restrict__ pi_in, T *__restrict__ pi_out,
                                   const DevScalars *__restrict__ sc, double *__restrict__ partials, T *lds)
@@ -163,16 +167,51 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
             }
     }
 
+    // PF == 2: loads for the next plane, in flight during this one
+    typedef T nvec __attribute__((ext_vector_type(VW)));
+    nvec nx_n0[ITEMS], nx_pi[ITEMS], nx_h[ITEMS];
+    if constexpr (PF == 2) {
+        const long long b0 = (long long)wrapN(i0 - 2) * NN, b1 = (long long)wrapN(i0 - 1) * NN;
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u) {
+            nx_n0[u] = nx_pi[u] = nx_h[u] = (nvec)(T)0;
+            if (ok[u]) {
+                nx_n0[u] = *reinterpret_cast<const nvec *>(phi_in + b1 + off[u]);
+                nx_pi[u] = *reinterpret_cast<const nvec *>(pi_in + b0 + off[u]);
+            }
+            if (D == 3 && hok[u]) nx_h[u] = *reinterpret_cast<const nvec *>(phi_in + b0 + hoff[u]);
+        }
+    }
+
     for (int p = i0 - 2; p <= i1 + 1; ++p) {
         const long long base = (long long)wrapN(p) * NN;
         const long long baseN = (long long)wrapN(p + 1) * NN;
-        V pi0[ITEMS];
+        V pi0[ITEMS], hv[ITEMS];
+        if constexpr (PF == 2) {
+            // (p + 2 <= i1 + 3 <= N + 3: wrapN keeps every plane in the box)
+            const long long bq = baseN, bq1 = (long long)wrapN(p + 2) * NN;
 #pragma unroll
-        for (int u = 0; u < ITEMS; ++u)
-            if (ok[u]) {
-                n0[u] = *reinterpret_cast<const V *>(phi_in + baseN + off[u]);
-                pi0[u] = *reinterpret_cast<const V *>(pi_in + base + off[u]);
+            for (int u = 0; u < ITEMS; ++u) {
+#pragma unroll
+                for (int e = 0; e < VW; ++e) {
+                    n0[u].v[e] = nx_n0[u][e];
+                    pi0[u].v[e] = nx_pi[u][e];
+                    hv[u].v[e] = nx_h[u][e];
+                }
+                if (ok[u]) {
+                    nx_n0[u] = *reinterpret_cast<const nvec *>(phi_in + bq1 + off[u]);
+                    nx_pi[u] = *reinterpret_cast<const nvec *>(pi_in + bq + off[u]);
+                }
+                if (D == 3 && hok[u]) nx_h[u] = *reinterpret_cast<const nvec *>(phi_in + bq + hoff[u]);
             }
+        } else {
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u)
+                if (ok[u]) {
+                    n0[u] = *reinterpret_cast<const V *>(phi_in + baseN + off[u]);
+                    pi0[u] = *reinterpret_cast<const V *>(pi_in + base + off[u]);
+                }
+        }
         const bool doB = p >= i0;            // step B on plane p-1 in [i0-1, i1]
         const bool doC = p >= i0 + 2;        // step C on plane p-2 in [i0, i1-1]
         // stage phi_n(p), phi_n+1(p-1), phi_n+2(p-2)
@@ -182,7 +221,10 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
         if constexpr (D == 3) {
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u)
-                if (hok[u]) *reinterpret_cast<V *>(L0 + hlds[u]) = *reinterpret_cast<const V *>(phi_in + base + hoff[u]);
+                if (hok[u]) {
+                    if constexpr (PF == 2) *reinterpret_cast<V *>(L0 + hlds[u]) = hv[u];
+                    else *reinterpret_cast<V *>(L0 + hlds[u]) = *reinterpret_cast<const V *>(phi_in + base + hoff[u]);
+                }
         }
         if (doB) {
 #pragma unroll

@@ -129,11 +129,14 @@ def test_every_fp32_kernel_variant_gives_the_same_fields(name):
     phi0, pi0 = ref.fields()
     e0 = ref.energy_row()
     n, _ = ref.kernel_variants()
-    fuses = set()
+    fuses, prefetch = set(), {2: set(), 3: set()}
     for v in range(n):
         g = cb.Bubble(precision="fp32", **kw)
         g.select_kernel(v)
-        fuses.add(g.variant_info(v)["fuse"])
+        info = g.variant_info(v)
+        fuses.add(info["fuse"])
+        if info["fuse"] >= 2:
+            prefetch[info["fuse"]].add(info["pf"])
         g.step(12)
         phi, pi = g.fields()
         np.testing.assert_array_equal(phi, phi0)
@@ -141,6 +144,8 @@ def test_every_fp32_kernel_variant_gives_the_same_fields(name):
         assert np.max(row_errors(e0[None], g.energy_row()[None])) <= 1e-13
         g.close()
     assert fuses == {1, 2, 3}
+    # multi-step tiles with and without the plane prefetch
+    assert prefetch == {2: {1, 2}, 3: {1, 2}}
 
 
 THERMAL = {
