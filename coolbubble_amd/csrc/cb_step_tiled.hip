@@ -100,17 +100,30 @@ __device__ __forceinline__ void tile_step(const SimConst &k, const TiledGeom &g,
         for (int u = 0; u < ITEMS; ++u)
             if (ok[u]) load_v(base + off[u], out[u], pout[u]);
     };
-    auto load_halo = [&](int i, V (&out)[ITEMS]) {
+    // The halo rows are held in native vectors: held as arrays of Vec
+    // written under the hok branch, the compiler kept them in scratch and
+    // waited for every load in flight to store them there.
+    typedef T hvec __attribute__((ext_vector_type(VW)));
+    auto load_halo = [&](int i, hvec (&out)[ITEMS]) {
         const long long base = (long long)i * NN;
-        V dummy;
 #pragma unroll
         for (int u = 0; u < ITEMS; ++u)
-            if (hok[u]) load_v(base + hoff[u], out[u], dummy);
+            if (hok[u]) {
+                hvec f = *reinterpret_cast<const hvec *>(phi_in + base + hoff[u]);
+                if constexpr (LAGGED) {
+                    const hvec p = *reinterpret_cast<const hvec *>(pi_in + base + hoff[u]);
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) f[e] = f[e] + dt * p[e] * s_prev;
+                }
+                out[u] = f;
+            }
     };
 
     V prev[ITEMS], cur[ITEMS], nxt[ITEMS], nx2[ITEMS];
     V pcur[ITEMS], pnxt[ITEMS], pnx2[ITEMS];
-    V hcur[ITEMS], hnxt[ITEMS];
+    hvec hcur[ITEMS], hnxt[ITEMS];
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) hcur[u] = hnxt[u] = (hvec)(T)0;
 
     double acc[NQ];
 #pragma unroll
@@ -147,7 +160,7 @@ __device__ __forceinline__ void tile_step(const SimConst &k, const TiledGeom &g,
         if constexpr (D == 3) {
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u)
-                if (hok[u]) *reinterpret_cast<V *>(L + hlds[u]) = hcur[u];
+                if (hok[u]) *reinterpret_cast<hvec *>(L + hlds[u]) = hcur[u];
         }
         lds_barrier();
 #pragma unroll
