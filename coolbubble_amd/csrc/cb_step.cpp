@@ -140,7 +140,7 @@ int default_variant(const cb_sim *s)
     if (s->variants.empty() || (D == 3 ? s->nsites < (1LL << 19) : N < 512)) return -1;
     struct Want { int fuse, nthr, TR, NI, pf, nt; } w;
     if (s->lagged) w = s->fp32 ? Want{1, 256, 4, 16, 1, 1} : Want{1, 256, 2, 32, 1, 1};
-    else if (D == 3 && N >= 512) w = s->fp32 ? Want{3, 1024, 4, 32, 2, 1} : Want{2, 1024, 4, 64, 2, 1};
+    else if (D == 3 && N >= 512) w = s->fp32 ? Want{3, 1024, 4, 64, 2, 1} : Want{2, 1024, 4, 64, 2, 1};
     else if (D == 3 && N >= 192) w = {3, 512, 4, 32, 1, 0};
     else if (D == 3) w = {1, 256, 8, 8, 1, 1};
     else if (N >= 2048) w = {3, 512, 1, N / 256, 1, 0};
