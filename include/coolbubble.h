@@ -167,14 +167,15 @@ cb_status cb_batch_size(cb_batch *b, int *n, long long *sites);
 cb_status cb_batch_stream(cb_batch *b, void **stream);
 /* Autotune the batched launch: time `steps` steps of the whole batch for a
  * set of launch shapes (vectors per thread, prefetch depth, non-temporal
- * stores, preferred chunk length, two steps per launch for static members;
- * each member then takes its tallest tile of that shape), state saved and
- * restored around the trials; keep the fastest.  shape (optional, 5 ints)
- * receives it, best_ms its time per step. */
+ * stores, preferred chunk length, several steps per launch for static
+ * members: three on 3D lattices from N = 192 and 2D from N = 2048, two on
+ * smaller ones; each member then takes its tallest tile of that shape),
+ * state saved and restored around the trials; keep the fastest.  shape
+ * (optional, 5 ints) receives it, best_ms its time per step. */
 cb_status cb_batch_autotune(cb_batch *b, int steps, int shape[5], double *best_ms);
 /* Set the batched launch shape directly (the five values cb_batch_autotune
  * reports: vectors per thread 1/2/4, prefetch 1/2, non-temporal 0/1,
- * chunk planes >= 1, two steps per launch 0/1). */
+ * chunk planes >= 1, several steps per launch for static members 0/1). */
 cb_status cb_batch_set_shape(cb_batch *b, const int shape[5]);
 cb_status cb_batch_destroy(cb_batch *b);
 
