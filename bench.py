@@ -31,6 +31,9 @@ HBM_PEAK = 8.0e12  # MI355X datasheet, bytes/s
 CONFIG = dict(D=3, nGrids=512, L=64.0, c=4.0, readxns=1, nBubbles=32, expansion=0)
 CONFIG["dt"] = 0.2 * CONFIG["L"] / CONFIG["nGrids"]
 ENERGY_EVERY = 40
+# the expanding-background workload: d=3 N=384 L=48 c=2, thermal start (mode -3,
+# Trad=0.1, seeded), Mpl=10, FP32 field with FP64 scale factor and sums
+CONFIG_EXP = dict(D=3, nGrids=384, L=48.0, c=2.0, readxns=-3, Trad=0.1, expansion=1, Mpl=10.0, dt=0.025)
 
 
 class HipEvents:
@@ -98,6 +101,39 @@ def run_precision(cb, hip, precision, steps, warmup, log, variant=None):
     row = sim.energy_row()
     log("%s: %d steps in %.3f ms (%.4f ms/step)" % (precision, steps, ms_total, ms_total / steps))
     return sim, ms_total, row
+
+
+def run_expanding(cb, hip, steps, log):
+    """FP32 steps of the expanding-background thermal workload: every step
+    needs the previous step's global energy sum (the Friedmann update), so
+    each step is one pass of the lagged-drift kernel with its energies."""
+    t0 = time.time()
+    sim = cb.Bubble(precision="fp32", **CONFIG_EXP)
+    sites = CONFIG_EXP["nGrids"] ** 3
+    log("expanding: thermal start %d^3 in %.1f s" % (CONFIG_EXP["nGrids"], time.time() - t0))
+    sim.autotune(steps=ENERGY_EVERY)
+    sel = sim.kernel_variants()[1]
+    sim.step(ENERGY_EVERY)
+    stream = sim.stream()
+    a, b = hip.event(), hip.event()
+    hip.sync()
+    hip.record(a, stream)
+    done = 0
+    while done < steps:
+        n = min(ENERGY_EVERY, steps - done)
+        sim.step(n)
+        done += n
+    hip.record(b, stream)
+    ms = hip.elapsed_ms(a, b)
+    su = sites * steps / (ms * 1e-3)
+    log("expanding: %d steps in %.3f ms (%.4f ms/step), a = %.6f" % (steps, ms, ms / steps, sim.state().a))
+    out = {"config": "d=3 N=384^3 L=48 c=2 mode -3 thermal Trad=0.1 expansion Mpl=10 dt=0.025, FP32 field, "
+                     "FP64 scale factor and sums, energies every step (lagged drift)",
+           "steps": steps, "ms_per_step": ms / steps, "site_updates_per_s": su,
+           "achieved_GBps": 16.0 * su / 1e9, "frac": 16.0 * su / HBM_PEAK,
+           "variant": sim.variant_info(sel) if sel >= 0 else "general kernel", "final_a": sim.state().a}
+    sim.close()
+    return out
 
 
 def launch_time_live(sim, hip, fuse, m=13, reps=3):
@@ -292,6 +328,7 @@ def main():
                     help="profiling aid: fixed kernel variant instead of autotuning")
     ap.add_argument("--no-spectrum", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-expanding", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
@@ -382,6 +419,7 @@ def main():
                "sample": "%d oracle steps (FP64, energies every step) of the same 512^3 lattice "
                          "from the GPU-seeded state" % args.cpu_steps}
     sim64.close()
+    expd = None if args.no_expanding else run_expanding(cb, hip, K, log)
     ens = None if args.no_ensemble else run_ensemble(cb, hip, args.ensemble_steps, log)
     srv = None if args.no_serving else run_serving(cb, args.serve_requests, args.serve_rate, log)
     out = {
@@ -421,6 +459,7 @@ def main():
         "fp64": {"value": su64, "ms_per_step": ms64 / K, "achieved_GBps": ach64 / 1e9,
                  "frac": ach64 / HBM_PEAK},
         "fp32_over_fp64": su32 / su64,
+        "expanding": expd,
         "ensemble": ens,
         "serving": srv,
         "measured_copy_GBps": copy_gbps,
