@@ -26,6 +26,8 @@
 // VGPRs per vector item).
 #pragma once
 
+#include <type_traits>
+
 #include "cb_device.hpp"
 #include "cb_tile_common.hpp"
 #include "cb_tiled.hpp"
@@ -146,11 +148,24 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
         hlds[u] = (side ? TRv + 5 : 0) * N + c0;
     }
 
-    V p0[ITEMS], c0[ITEMS], n0[ITEMS];      // phi_n   at planes p-1, p,   p+1
-    V p1[ITEMS], c1[ITEMS], n1[ITEMS];      // phi_n+1 at planes p-2, p-1, p
-    V p2[ITEMS], c2[ITEMS], n2[ITEMS];      // phi_n+2 at planes p-3, p-2, p-1
+    // Each level's three planes sit in a ring indexed by the plane counter
+    // t = p - (i0 - 2) modulo 3, so moving to the next plane moves no
+    // registers for them: the loop body is instantiated for t % 3 = 0, 1, 2
+    // and runs three planes per trip (a plane loop with barriers is not
+    // unrolled by the compiler, and the copies p = c, c = n were a third of
+    // its VALU instructions; 5% faster at 3D N = 512 FP32, while unrolling
+    // six times to drop the pi copies as well ran slower).
+    //   f0[t % 3] phi_n at plane p   (f0[(t+2) % 3] at p-1, f0[(t+1) % 3] at p+1)
+    //   f1[t % 3] phi_n+1 at plane p-1 (likewise at p-2 and p)
+    //   f2[t % 3] phi_n+2 at plane p-2 (likewise at p-3 and p-1)
+    // PF == 2: phi_n at p+1, pi_n-1/2 and the outer L0 rows at p arrive in
+    // nx / pr / hr (native vectors: arrays of Vec carried around the loop
+    // went to scratch), loaded during plane p-1.
+    typedef T nvec __attribute__((ext_vector_type(VW)));
+    V f0[3][ITEMS], f1[3][ITEMS], f2[3][ITEMS];
     V q1c[ITEMS], q1n[ITEMS];               // pi_n+1/2 at planes p-1, p
     V q2c[ITEMS], q2n[ITEMS];               // pi_n+3/2 at planes p-2, p-1
+    nvec nx[ITEMS], pr[ITEMS], hr[ITEMS];
 
     double acc[NQ];
 #pragma unroll
@@ -162,53 +177,52 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
 #pragma unroll
         for (int u = 0; u < ITEMS; ++u)
             if (ok[u]) {
-                p0[u] = *reinterpret_cast<const V *>(phi_in + bm3 + off[u]);
-                c0[u] = *reinterpret_cast<const V *>(phi_in + bm2 + off[u]);
+                f0[2][u] = *reinterpret_cast<const V *>(phi_in + bm3 + off[u]);
+                f0[0][u] = *reinterpret_cast<const V *>(phi_in + bm2 + off[u]);
             }
     }
-
-    // PF == 2: loads for the next plane, in flight during this one
-    typedef T nvec __attribute__((ext_vector_type(VW)));
-    nvec nx_n0[ITEMS], nx_pi[ITEMS], nx_h[ITEMS];
     if constexpr (PF == 2) {
         const long long b0 = (long long)wrapN(i0 - 2) * NN, b1 = (long long)wrapN(i0 - 1) * NN;
 #pragma unroll
         for (int u = 0; u < ITEMS; ++u) {
-            nx_n0[u] = nx_pi[u] = nx_h[u] = (nvec)(T)0;
+            nx[u] = pr[u] = hr[u] = (nvec)(T)0;
             if (ok[u]) {
-                nx_n0[u] = *reinterpret_cast<const nvec *>(phi_in + b1 + off[u]);
-                nx_pi[u] = *reinterpret_cast<const nvec *>(pi_in + b0 + off[u]);
+                nx[u] = *reinterpret_cast<const nvec *>(phi_in + b1 + off[u]);
+                pr[u] = *reinterpret_cast<const nvec *>(pi_in + b0 + off[u]);
             }
-            if (D == 3 && hok[u]) nx_h[u] = *reinterpret_cast<const nvec *>(phi_in + b0 + hoff[u]);
+            if (D == 3 && hok[u]) hr[u] = *reinterpret_cast<const nvec *>(phi_in + b0 + hoff[u]);
         }
     }
 
-    for (int p = i0 - 2; p <= i1 + 1; ++p) {
+    // one plane p, with S3 = t % 3
+    auto plane = [&](auto s3, int p) __attribute__((always_inline)) {
+        constexpr int S3 = decltype(s3)::value;
+        constexpr int PV = (S3 + 2) % 3, NX = (S3 + 1) % 3;
         const long long base = (long long)wrapN(p) * NN;
         const long long baseN = (long long)wrapN(p + 1) * NN;
         V pi0[ITEMS], hv[ITEMS];
         if constexpr (PF == 2) {
-            // (p + 2 <= i1 + 3 <= N + 3: wrapN keeps every plane in the box)
-            const long long bq = baseN, bq1 = (long long)wrapN(p + 2) * NN;
+            // plane p+1's loads (p + 2 <= i1 + 3 <= N + 3: wrapN keeps every plane in the box)
+            const long long bq1 = (long long)wrapN(p + 2) * NN;
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u) {
 #pragma unroll
                 for (int e = 0; e < VW; ++e) {
-                    n0[u].v[e] = nx_n0[u][e];
-                    pi0[u].v[e] = nx_pi[u][e];
-                    hv[u].v[e] = nx_h[u][e];
+                    f0[NX][u].v[e] = nx[u][e];
+                    pi0[u].v[e] = pr[u][e];
+                    hv[u].v[e] = hr[u][e];
                 }
                 if (ok[u]) {
-                    nx_n0[u] = *reinterpret_cast<const nvec *>(phi_in + bq1 + off[u]);
-                    nx_pi[u] = *reinterpret_cast<const nvec *>(pi_in + bq + off[u]);
+                    nx[u] = *reinterpret_cast<const nvec *>(phi_in + bq1 + off[u]);
+                    pr[u] = *reinterpret_cast<const nvec *>(pi_in + baseN + off[u]);
                 }
-                if (D == 3 && hok[u]) nx_h[u] = *reinterpret_cast<const nvec *>(phi_in + bq + hoff[u]);
+                if (D == 3 && hok[u]) hr[u] = *reinterpret_cast<const nvec *>(phi_in + baseN + hoff[u]);
             }
         } else {
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u)
                 if (ok[u]) {
-                    n0[u] = *reinterpret_cast<const V *>(phi_in + baseN + off[u]);
+                    f0[NX][u] = *reinterpret_cast<const V *>(phi_in + baseN + off[u]);
                     pi0[u] = *reinterpret_cast<const V *>(pi_in + base + off[u]);
                 }
         }
@@ -217,7 +231,7 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
         // stage phi_n(p), phi_n+1(p-1), phi_n+2(p-2)
 #pragma unroll
         for (int u = 0; u < ITEMS; ++u)
-            if (ok[u]) *reinterpret_cast<V *>(L0 + (rowA[u] + H) * N + col[u]) = c0[u];
+            if (ok[u]) *reinterpret_cast<V *>(L0 + (rowA[u] + H) * N + col[u]) = f0[S3][u];
         if constexpr (D == 3) {
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u)
@@ -229,27 +243,27 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
         if (doB) {
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u)
-                if (ok[u]) *reinterpret_cast<V *>(L1 + rowA[u] * N + col[u]) = c1[u];
+                if (ok[u]) *reinterpret_cast<V *>(L1 + rowA[u] * N + col[u]) = f1[S3][u];
         }
         if (doC) {
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u)
-                if (inB[u]) *reinterpret_cast<V *>(L2 + (rowA[u] - H) * N + col[u]) = c2[u];
+                if (inB[u]) *reinterpret_cast<V *>(L2 + (rowA[u] - H) * N + col[u]) = f2[S3][u];
         }
         lds_barrier();
         // step A on plane p, all TR + 4H rows
 #pragma unroll
         for (int u = 0; u < ITEMS; ++u)
             if (ok[u])
-                leap_vec<T, D, VW, false>(k, c0[u], p0[u], n0[u], L0 + (rowA[u] + H) * N, col[u], pi0[u], fac_lap,
-                                          fac_pot, sf, dt, q1n[u], n1[u], sc, nacc);
+                leap_vec<T, D, VW, false>(k, f0[S3][u], f0[PV][u], f0[NX][u], L0 + (rowA[u] + H) * N, col[u], pi0[u],
+                                          fac_lap, fac_pot, sf, dt, q1n[u], f1[NX][u], sc, nacc);
         // step B on plane p-1, TR + 2H rows
         if (doB) {
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u)
                 if (inB[u])
-                    leap_vec<T, D, VW, false>(k, c1[u], p1[u], n1[u], L1 + rowA[u] * N, col[u], q1c[u], fac_lap,
-                                              fac_pot, sf, dt, q2n[u], n2[u], sc, nacc);
+                    leap_vec<T, D, VW, false>(k, f1[S3][u], f1[PV][u], f1[NX][u], L1 + rowA[u] * N, col[u], q1c[u],
+                                              fac_lap, fac_pot, sf, dt, q2n[u], f2[NX][u], sc, nacc);
         }
         // step C on plane p-2, owned rows: to HBM
         if (doC) {
@@ -258,8 +272,8 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
             for (int u = 0; u < ITEMS; ++u) {
                 if (!inC[u]) continue;
                 V pn, phn;
-                leap_vec<T, D, VW, ENERGY>(k, c2[u], p2[u], n2[u], L2 + (rowA[u] - H) * N, col[u], q2c[u], fac_lap,
-                                           fac_pot, sf, dt, pn, phn, sc, acc);
+                leap_vec<T, D, VW, ENERGY>(k, f2[S3][u], f2[PV][u], f2[NX][u], L2 + (rowA[u] - H) * N, col[u],
+                                           q2c[u], fac_lap, fac_pot, sf, dt, pn, phn, sc, acc);
                 store_vec<T, VW>(pi_out + baseq + off[u], pn, nt);
                 store_vec<T, VW>(phi_out + baseq + off[u], phn, nt);
             }
@@ -267,15 +281,20 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
         lds_barrier();
 #pragma unroll
         for (int u = 0; u < ITEMS; ++u) {
-            p0[u] = c0[u];
-            c0[u] = n0[u];
-            p1[u] = c1[u];
-            c1[u] = n1[u];
-            p2[u] = c2[u];
-            c2[u] = n2[u];
             q1c[u] = q1n[u];
             q2c[u] = q2n[u];
         }
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    // (p is uniform over the block: every wave leaves at the same plane)
+    for (int p = i0 - 2; p <= i1 + 1; p += 3) {
+        plane(I0{}, p);
+        if (p + 1 > i1 + 1) break;
+        plane(I1{}, p + 1);
+        if (p + 2 > i1 + 1) break;
+        plane(I2{}, p + 2);
     }
     if constexpr (ENERGY) block_partials<NQ>(acc, partials + (long long)lt * NQ);
 }
