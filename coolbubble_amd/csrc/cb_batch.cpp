@@ -57,21 +57,17 @@ int batch_variant(const cb_sim *s) { return pick_variant(s, kBatchItems, 1, 1, 3
 // Multi-step tile of a static member for the batched multi-step launch:
 // 256 threads, at most kBatchItems vectors per thread, prefetch 1, its
 // one-step companion fitting the batched launches too.  Three steps per
-// launch on the lattices where the lone stepper takes three (3D from
-// N = 192, 2D from N = 2048) and such a tile of 2 rows or more exists, else
-// two; the tallest tile of at most 4 rows, chunks nearest 16 planes (8 in
-// 2D), non-temporal stores preferred.  On smaller lattices the three-step
-// tile's overlap costs more than the HBM pass it saves: an ensemble of
-// 64 x 3D N = 128 and 16 x 2D N = 1024 members runs at 378 us per step
-// with three steps per launch, 305 us with two.  At most max_fuse steps.
-// -1 if there is none.
+// launch where such a tile of 2 rows or more exists (at most max_fuse),
+// else two; the tallest tile of at most 4 rows, chunks nearest 16 planes
+// (8 in 2D), non-temporal stores preferred.  An ensemble of 64 x 3D
+// N = 128 and 16 x 2D N = 1024 members steps 1000 times in 249 ms with
+// three steps per launch, 335 ms with two.  -1 if there is none.
 int pick_variant_multi(const cb_sim *s, int max_fuse)
 {
     if (s->lagged || s->p.D < 2) return -1;
     const double ni_pref = s->p.D == 3 ? 16.0 : 8.0;
-    const bool large = s->p.D == 3 ? s->p.nGrids >= 192 : s->p.nGrids >= 2048;
     for (int fuse : {3, 2}) {
-        if (fuse > max_fuse || (fuse == 3 && !large)) continue;
+        if (fuse > max_fuse) continue;
         int best = -1;
         double best_score = -1e300;
         for (size_t v = 0; v < s->variants.size(); ++v) {
@@ -90,10 +86,10 @@ int pick_variant_multi(const cb_sim *s, int max_fuse)
     return -1;
 }
 
-// Two steps per launch: with three allowed for the large requests (3D from
-// N = 192) a serving mix of 200 Poisson-arriving requests of every size ran
-// at p50 2.9-3.0 s against 1.9 s (a server's table mixes both tile kinds,
-// and its short runs to the next event leave steps over for the companion).
+// Two steps per launch: with three, a serving mix of 200 Poisson-arriving
+// requests of every size ran at p50 2.6 s against 1.8 s (three for the
+// large requests only: 2.9-3.0 s); a request's short runs to its next
+// event leave steps over for the one-step companion.
 int server_variant(const cb_sim *s)
 {
     const int vm = pick_variant_multi(s, 2);

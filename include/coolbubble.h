@@ -168,8 +168,8 @@ cb_status cb_batch_stream(cb_batch *b, void **stream);
 /* Autotune the batched launch: time `steps` steps of the whole batch for a
  * set of launch shapes (vectors per thread, prefetch depth, non-temporal
  * stores, preferred chunk length, several steps per launch for static
- * members: three on 3D lattices from N = 192 and 2D from N = 2048, two on
- * smaller ones; each member then takes its tallest tile of that shape),
+ * members: three where a three-step tile fits the batched launch, else
+ * two; each member then takes its tallest tile of that shape),
  * state saved and restored around the trials; keep the fastest.  shape
  * (optional, 5 ints) receives it, best_ms its time per step. */
 cb_status cb_batch_autotune(cb_batch *b, int steps, int shape[5], double *best_ms);
