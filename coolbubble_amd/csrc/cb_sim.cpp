@@ -210,7 +210,7 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
     s->cfg = simple_cfg(s->nsites);
 
     {   // step-kernel variants and the energy-partials capacity they need
-        static const int kMaxVariants = 384;
+        static const int kMaxVariants = 512;
         std::vector<cb::TiledCfg> cand((size_t)kMaxVariants);
         int nc = cb::tiled_candidates(k, s->fp32, cand.data(), kMaxVariants);
         s->variants.assign(cand.begin(), cand.begin() + nc);

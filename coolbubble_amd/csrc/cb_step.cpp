@@ -129,23 +129,27 @@ int companion_variant(const cb_sim *s, const TiledCfg &f)
 }
 
 // The variant a simulation starts with (cb_autotune can do better): the
-// tiled shape nearest to what the autotuner picks on MI355X for lattices of
-// this kind, measured over 2D N = 256-2048, 3D N = 64-512, static and
-// expanding, FP32 and FP64.  Small lattices (3D below 2^19 sites, 2D below
-// 512^2) fill the GPU only with the one-site-per-thread kernel; large static
-// ones take two or three steps per pass.
+// tiled shape nearest to what the autotuner picks on MI355X (trials of 40
+// steps) for lattices of this kind, measured over 2D N = 256-2048, 3D
+// N = 64-512, static and expanding, FP32 and FP64.  Small lattices (3D below
+// 2^19 sites, 2D below 512^2) fill the GPU only with the one-site-per-thread
+// kernel; static ones take three steps per pass (FP64 at 3D N >= 512 two:
+// no three-step tile of its 1024-wide rows fits a block), with the plane
+// prefetch; expanding ones one step.
 int default_variant(const cb_sim *s)
 {
     const int D = s->p.D, N = s->p.nGrids;
     if (s->variants.empty() || (D == 3 ? s->nsites < (1LL << 19) : N < 512)) return -1;
     struct Want { int fuse, nthr, TR, NI, pf, nt; } w;
-    if (s->lagged) w = s->fp32 ? Want{1, 256, 4, 16, 1, 1} : Want{1, 256, 2, 32, 1, 1};
-    else if (D == 3 && N >= 512) w = s->fp32 ? Want{3, 1024, 4, 64, 2, 1} : Want{2, 1024, 4, 64, 2, 1};
-    else if (D == 3 && N >= 192) w = {3, 512, 4, 32, 1, 0};
-    else if (D == 3) w = {1, 256, 8, 8, 1, 1};
-    else if (N >= 2048) w = {3, 512, 1, N / 256, 1, 0};
-    else if (N >= 1024) w = {2, 256, 1, 8, 1, 1};
-    else w = {1, 256, 1, 4, 2, 1};
+    const bool f32 = s->fp32 != 0;
+    if (s->lagged) w = f32 ? Want{1, 256, 4, 16, 1, 1} : Want{1, 256, 2, 32, 1, 1};
+    else if (D == 3 && N >= 512) w = f32 ? Want{3, 1024, 4, 64, 2, 1} : Want{2, 1024, 4, 64, 2, 1};
+    else if (D == 3 && N >= 320) w = f32 ? Want{3, 1024, 6, 32, 2, 1} : Want{3, 1024, 4, 32, 2, 1};
+    else if (D == 3 && N >= 224) w = f32 ? Want{3, 1024, 8, 32, 2, 0} : Want{3, 1024, 4, 64, 2, 1};
+    else if (D == 3 && N >= 160) w = f32 ? Want{3, 512, 6, 16, 2, 0} : Want{3, 512, 4, 16, 2, 1};
+    else if (D == 3) w = f32 ? Want{3, 256, 4, 16, 2, 1} : Want{3, 512, 4, 16, 2, 1};
+    else if (N >= 2048) w = {3, 512, 1, 8, 2, 0};
+    else w = {3, 256, 1, 8, 2, 1};
     int best = -1;
     double best_score = -1e300;
     for (size_t v = 0; v < s->variants.size(); ++v) {

@@ -319,7 +319,7 @@ def test_multi_step_variants_bit_identical_with_odd_counts(name, fuse):
 
 def test_default_variant_follows_the_lattice():
     """cb_create starts small lattices on the general kernel, large static
-    boxes on a two-step tile, expanding ones on a one-step tile."""
+    boxes on a three-step tile, expanding ones on a one-step tile."""
     small = cb.Bubble(**CASES["2d_mode1"])
     assert small.kernel_variants()[1] == -1
     big = cb.Bubble(precision="fp32", **params(D=3, readxns=-2, nGrids=256, L=32.0))
