@@ -129,8 +129,20 @@ k_rows_r2c(const T *__restrict__ field, cpx<R> *__restrict__ H, long long rows, 
     const long long r0 = (long long)blockIdx.x * RB;
     const int nr = (int)((rows - r0) < RB ? (rows - r0) : RB);
     const T *src = field + r0 * N;
-    for (int idx = threadIdx.x; idx < RB * N; idx += blockDim.x)
-        A[idx] = (idx < nr * N) ? cpx<R>{(R)src[idx], (R)0} : cpx<R>{(R)0, (R)0};
+    // loads in batches of 8 per thread, all issued before any is waited for
+    for (int i0 = threadIdx.x; i0 < RB * N; i0 += 8 * blockDim.x) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = i0 + u * (int)blockDim.x;
+            v[u] = idx < nr * N ? src[idx] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = i0 + u * (int)blockDim.x;
+            if (idx < RB * N) A[idx] = cpx<R>{(R)v[u], (R)0};
+        }
+    }
     __syncthreads();
     cpx<R> *res = lds_fft<R>(A, B, RB, N, 1, st, tw, -1);
     cpx<R> *dst = H + r0 * nc;
@@ -187,9 +199,21 @@ k_cols(cpx<R> *H, long long inner, int CB, Stages st, const cpx<R> *__restrict__
     const long long c0 = (blockIdx.x - o * tiles) * CB;
     const int nc = (int)((inner - c0) < CB ? (inner - c0) : CB);
     cpx<R> *base = H + o * (long long)len * inner + c0;
-    for (int idx = threadIdx.x; idx < len * CB; idx += blockDim.x) {
-        const int e = idx / CB, cc = idx - e * CB;
-        A[cc * len + e] = (cc < nc) ? base[(long long)e * inner + cc] : cpx<R>{(R)0, (R)0};
+    // loads in batches of 8 per thread, all issued before any is waited for
+    for (int i0 = threadIdx.x; i0 < len * CB; i0 += 8 * blockDim.x) {
+        cpx<R> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = i0 + u * (int)blockDim.x;
+            const int e = idx / CB, cc = idx - e * CB;
+            v[u] = (idx < len * CB && cc < nc) ? base[(long long)e * inner + cc] : cpx<R>{(R)0, (R)0};
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = i0 + u * (int)blockDim.x;
+            const int e = idx / CB, cc = idx - e * CB;
+            if (idx < len * CB) A[cc * len + e] = v[u];
+        }
     }
     __syncthreads();
     cpx<R> *res = lds_fft<R>(A, B, CB, len, 1, st, tw, sign);

@@ -342,9 +342,23 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
         const int ncol = (int)((inner - c0) < CB ? (inner - c0) : CB);
         cpx<R> *base = H + o * (long long)N * inner + c0;
         __syncthreads();
-        for (int idx = threadIdx.x; idx < N * CB; idx += 256) {
-            const int e = idx / CB, cc = idx - e * CB;
-            buf[cc * BS + pidx(e)] = (cc < ncol) ? base[(long long)e * inner + cc] : cpx<R>{(R)0, (R)0};
+        // the loads of a batch are all issued before any is waited for (a
+        // plain load-store loop keeps one load per thread in flight)
+        constexpr int LIT = (N * CB + 255) / 256, LB = LIT < 8 ? LIT : 8;
+        for (int r0 = 0; r0 < LIT; r0 += LB) {
+            cpx<R> v[LB];
+#pragma unroll
+            for (int u = 0; u < LB; ++u) {
+                const int idx = threadIdx.x + (r0 + u) * 256;
+                const int e = idx / CB, cc = idx - e * CB;
+                v[u] = (idx < N * CB && cc < ncol) ? base[(long long)e * inner + cc] : cpx<R>{(R)0, (R)0};
+            }
+#pragma unroll
+            for (int u = 0; u < LB; ++u) {
+                const int idx = threadIdx.x + (r0 + u) * 256;
+                const int e = idx / CB, cc = idx - e * CB;
+                if (idx < N * CB) buf[cc * BS + pidx(e)] = v[u];
+            }
         }
         __syncthreads();
         p2_fft<R, N, RND>(buf, BS, twl, sign);
