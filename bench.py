@@ -159,6 +159,31 @@ def launch_time_live(sim, hip, fuse, m=13, reps=3):
     return (timed(fuse * m) - timed(fuse)) / (m - 1) * 1e3
 
 
+def slab_share(sim, info, log, planes=64):
+    """One GPU's slab of the same lattice sharded 8 ways along the slowest
+    axis (512 x 512 x 64 planes): the selected three-step tile and its
+    variants with other chunk lengths, restricted to the slab's planes, and
+    the device copies standing in for the halo exchange."""
+    if not info or info.get("fuse", 1) < 2:
+        return None
+    n, _ = sim.kernel_variants()
+    out = {"planes": planes, "shape": "%d x %d x %d" % (CONFIG["nGrids"], CONFIG["nGrids"], planes), "tiles": []}
+    for ni in (64, 32, 16):
+        want = dict(info, NI=ni)
+        v = next((v for v in range(n) if sim.variant_info(v) == want), None)
+        if v is None or planes % ni:
+            continue
+        sim.select_kernel(v)
+        ml, mx = sim.slab_probe(planes, 20)
+        su = CONFIG["nGrids"] ** 2 * planes * info["fuse"] / (ml * 1e-3)
+        log("slab %s, NI %d: %.1f us per launch (%d steps), %.3e site-updates/s; exchange copies %.1f us"
+            % (out["shape"], ni, ml * 1e3, info["fuse"], su, mx * 1e3))
+        out["tiles"].append({"NI": ni, "us_per_launch": ml * 1e3, "site_updates_per_s": su,
+                             "exchange_copies_us": mx * 1e3})
+    out["exchange_bytes_per_launch"] = 2 * 2 * info["fuse"] * CONFIG["nGrids"] ** 2 * 4
+    return out
+
+
 def spectrum_time(sim, hip, reps=3):
     stream = sim.stream()
     sim.spectrum()   # plan + warm
@@ -380,6 +405,7 @@ def main():
     log("fp32: one plain launch (%d steps) %.1f us, live" % (fuse32, launch_us32))
     spec32 = spectrum_time(sim32, hip)
     log("fp32 spectrum: %.3f ms" % spec32)
+    slab = slab_share(sim32, info32, log)   # last: only the slab advances
     sim32.close()
     sim64, ms64, row64 = run_precision(cb, hip, "fp64", K, W, log, args.variant)
     spec64 = spectrum_time(sim64, hip)
@@ -460,6 +486,7 @@ def main():
                  "frac": ach64 / HBM_PEAK},
         "fp32_over_fp64": su32 / su64,
         "expanding": expd,
+        "slab_share_8_gpus": slab,
         "ensemble": ens,
         "serving": srv,
         "measured_copy_GBps": copy_gbps,

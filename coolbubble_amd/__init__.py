@@ -101,6 +101,8 @@ def lib():
         L.cb_autotune.argtypes = [vp, ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.cb_bandwidth_probe.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp]
         L.cb_bandwidth_probe.restype = ctypes.c_int
+        L.cb_slab_probe.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.cb_slab_probe.restype = ctypes.c_int
         L.cb_batch_create.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(vp)]
         L.cb_batch_step.argtypes = [vp, ctypes.c_int]
         L.cb_batch_member.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
@@ -348,6 +350,14 @@ class Bubble:
         check(lib().cb_power_spectrum(self._h, _dp(k), _dp(kp), _dp(d), cap, ctypes.byref(nb)))
         n = nb.value
         return k[:n].copy(), kp[:n].copy(), d[:n].copy()
+
+    def slab_probe(self, planes, launches=20):
+        """(ms per launch, ms per exchange) of this simulation's multi-step
+        variant on its first `planes` planes, as one rank's slab of a run
+        sharded along the slowest axis; the fields are scratch afterwards."""
+        ml, mx = ctypes.c_double(), ctypes.c_double()
+        check(lib().cb_slab_probe(self._h, int(planes), int(launches), ctypes.byref(ml), ctypes.byref(mx)))
+        return ml.value, mx.value
 
     def kernel_variants(self):
         """(number of tiled variants, selected variant; -1 = general kernel)."""

@@ -63,9 +63,10 @@ bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c)
 
 template <typename T, int D, int ITEMS, int NTHR>
 static void launch3_t(const SimConst &k, int energy, const TiledCfg &c, const StepPtrs &p, const DevScalars *sc,
-                      double *partials, hipStream_t st, size_t shm)
+                      double *partials, hipStream_t st, size_t shm, int nic)
 {
     TiledGeom g = geom3(k, c);
+    g.nic = nic;
     dim3 grid(g.ntr * g.nic), block(NTHR);
     // (the energy launches do not prefetch: their FP64 sums take the registers)
     if (energy)
@@ -85,9 +86,9 @@ static void launch3_t(const SimConst &k, int energy, const TiledCfg &c, const St
 
 template <typename T, int D>
 static void launch3_d(const SimConst &k, int energy, const TiledCfg &c, const StepPtrs &p, const DevScalars *sc,
-                      double *partials, hipStream_t st, size_t shm)
+                      double *partials, hipStream_t st, size_t shm, int nic)
 {
-#define CB_L3(I, NT) launch3_t<T, D, I, NT>(k, energy, c, p, sc, partials, st, shm)
+#define CB_L3(I, NT) launch3_t<T, D, I, NT>(k, energy, c, p, sc, partials, st, shm, nic)
     if (c.nthr == 256) {
         if (c.items == 1) CB_L3(1, 256);
         else if (c.items == 2) CB_L3(2, 256);
@@ -105,18 +106,19 @@ static void launch3_d(const SimConst &k, int energy, const TiledCfg &c, const St
 }
 
 hipError_t launch_step3(const SimConst &k, int fp32, int energy, const TiledCfg &c, const StepPtrs &p,
-                        const DevScalars *sc, double *partials, hipStream_t st, int *nblocks)
+                        const DevScalars *sc, double *partials, hipStream_t st, int *nblocks, int nchunks)
 {
     if (!step3_valid(k, fp32, c)) return hipErrorInvalidValue;
     const size_t shm = step3_lds_bytes(k, fp32, c);
     TiledGeom g = geom3(k, c);
+    if (nchunks > 0 && nchunks < g.nic) g.nic = nchunks;   // the first nchunks chunks of planes only
     if (nblocks) *nblocks = g.ntr * g.nic;
     if (fp32) {
-        if (k.D == 3) launch3_d<float, 3>(k, energy, c, p, sc, partials, st, shm);
-        else launch3_d<float, 2>(k, energy, c, p, sc, partials, st, shm);
+        if (k.D == 3) launch3_d<float, 3>(k, energy, c, p, sc, partials, st, shm, g.nic);
+        else launch3_d<float, 2>(k, energy, c, p, sc, partials, st, shm, g.nic);
     } else {
-        if (k.D == 3) launch3_d<double, 3>(k, energy, c, p, sc, partials, st, shm);
-        else launch3_d<double, 2>(k, energy, c, p, sc, partials, st, shm);
+        if (k.D == 3) launch3_d<double, 3>(k, energy, c, p, sc, partials, st, shm, g.nic);
+        else launch3_d<double, 2>(k, energy, c, p, sc, partials, st, shm, g.nic);
     }
     return hipGetLastError();
 }

@@ -111,14 +111,16 @@ bool step2_valid(const SimConst &k, int fp32, const TiledCfg &c);
 int step2_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap);
 size_t step2_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
 hipError_t launch_step2(const SimConst &k, int fp32, int energy, const TiledCfg &c, const StepPtrs &p,
-                        const DevScalars *sc, double *partials, hipStream_t st, int *nblocks);
+                        const DevScalars *sc, double *partials, hipStream_t st, int *nblocks, int nchunks = 0);
 
 // Three steps per launch (cb_step3.hip): static box only; pi must ping-pong.
+// nchunks > 0: only the tiles of the first nchunks chunks of NI planes (one
+// GPU's slab of a run sharded along the slowest axis); the same for two.
 bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c);
 int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap);
 size_t step3_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c);
 hipError_t launch_step3(const SimConst &k, int fp32, int energy, const TiledCfg &c, const StepPtrs &p,
-                        const DevScalars *sc, double *partials, hipStream_t st, int *nblocks);
+                        const DevScalars *sc, double *partials, hipStream_t st, int *nblocks, int nchunks = 0);
 
 // Registry of chosen configurations (the autotuner writes, the step reads).
 int tiled_register(const TiledCfg &c);

@@ -296,6 +296,18 @@ cb_status cb_write_snapshot_h5(const char *path, int D, int nGrids, double L, do
  * `reps` times, counting read + write bytes, in GB/s. */
 cb_status cb_bandwidth_probe(int device, long long bytes, int reps, double *gbps);
 
+/* Measurement helper (no reference counterpart): one GPU's share of a run
+ * sharded into slabs along the slowest axis, at its real shape.  Times
+ * `launches` launches of s's multi-step tiled variant (static 3D box)
+ * restricted to the first `planes` planes of the lattice, which read their
+ * halo planes (steps-per-launch planes per face) where a rank would hold
+ * the planes its neighbours sent; then the device-to-device copies that
+ * stand in for that exchange (those planes of phi and pi, both faces).
+ * planes: a multiple of the variant's chunk length, leaving room for the
+ * halo planes.  Only the slab advances, so s's fields are scratch
+ * afterwards.  ms_launch / ms_exchange: average per launch. */
+cb_status cb_slab_probe(cb_sim *s, int planes, int launches, double *ms_launch, double *ms_exchange);
+
 /* Number of lattice sites N = nGrids^D (src/bubble.hpp:43). */
 long long cb_nsites(const cb_sim *s);
 

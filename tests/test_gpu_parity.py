@@ -344,3 +344,24 @@ def test_spectrum_with_more_bins_than_64k_of_lds_matches_oracle():
     np.testing.assert_allclose(kg, ko, rtol=1e-13, atol=0)
     floor = 1e-13 * np.max(do)
     assert np.all(np.abs(dg - do) <= TOL * np.abs(do) + floor), np.max(np.abs(dg - do) / (np.abs(do) + floor))
+
+
+def test_slab_probe_times_one_ranks_share():
+    """The slab probe runs the multi-step tile on the first planes of the
+    lattice only (one rank's share of a run sharded along the slowest axis)
+    and the halo-exchange copies; it refuses shapes without room for the
+    halo planes or off the chunk length."""
+    s = cb.Bubble(precision="fp32", **params(D=3, readxns=-2, nGrids=64, L=8.0))
+    n, _ = s.kernel_variants()
+    v = next(v for v in range(n) if s.variant_info(v)["fuse"] == 3 and s.variant_info(v)["NI"] == 16)
+    s.select_kernel(v)
+    ml, mx = s.slab_probe(32, 5)
+    assert ml > 0 and mx > 0
+    for bad in (8, 24, 64):
+        with pytest.raises(cb.CBError):
+            s.slab_probe(bad, 1)
+    s.close()
+    e = cb.Bubble(precision="fp32", **params(D=3, readxns=-2, nGrids=64, L=8.0, expansion=1, Mpl=10.0))
+    with pytest.raises(cb.CBError):
+        e.slab_probe(32, 1)
+    e.close()
