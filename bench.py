@@ -315,6 +315,17 @@ def run_driver(cb, log):
     return res
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sim64, nsteps, log):
     """Oracle (FP64 C/OpenMP restatement, 4 passes per step with energies
     every step, like the reference) stepping the same lattice from the GPU's
@@ -345,7 +356,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
                     help="profiling aid: run one precision only")
@@ -441,7 +452,7 @@ def main():
     if not args.no_cpu:
         cdt, threads = cpu_baseline(sim64, args.cpu_steps, log)
         cpu = {"value": sites * args.cpu_steps / cdt, "unit": "site-updates/s", "cores": threads,
-               "kind": "port",
+               "kind": "port", "cpu_model": cpu_model(),
                "sample": "%d oracle steps (FP64, energies every step) of the same 512^3 lattice "
                          "from the GPU-seeded state" % args.cpu_steps}
     sim64.close()
