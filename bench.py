@@ -369,7 +369,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--only", choices=["ensemble", "serving", "run"], default=None,
+    ap.add_argument("--only", choices=["ensemble", "serving", "run", "expanding"], default=None,
                     help="run one leg alone and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -395,6 +395,9 @@ def main():
         return 0
     if args.only == "run":
         print(json.dumps({"run": run_driver(cb, log)}), flush=True)
+        return 0
+    if args.only == "expanding":
+        print(json.dumps({"expanding": run_expanding(cb, hip, K, log)}), flush=True)
         return 0
     copy_gbps = cb.bandwidth_probe(1 << 31, 20)
     log("device-to-device copy: %.1f GB/s (read + write)" % copy_gbps)

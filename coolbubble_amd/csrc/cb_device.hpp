@@ -59,7 +59,23 @@ __device__ inline void sum_partials_block(const double *partials, int nblocks, d
     Neu n[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) n[q] = {0.0, 0.0};
-    for (int b = t; b < nblocks; b += blockDim.x)
+    // Each thread adds rows t, t + 256, t + 512, ... in that order; the rows
+    // are loaded four at a time so that their latencies overlap (one block
+    // alone on the GPU has nothing else to hide them behind).
+    const int stride = blockDim.x;
+    int b = t;
+    for (; b + 3 * stride < nblocks; b += 4 * stride) {
+        double v[4][NQ];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) v[u][q] = partials[(long long)(b + u * stride) * NQ + q];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) n[q].add(v[u][q]);
+    }
+    for (; b < nblocks; b += stride)
 #pragma unroll
         for (int q = 0; q < NQ; ++q) n[q].add(partials[(long long)b * NQ + q]);
 #pragma unroll
@@ -131,7 +147,7 @@ __device__ inline void energy_terms(double ph, const double (&fwd)[3], double pn
     // The potential is a difference of O(c) terms that cancel near phi = 0
     // and phi = 1, so the FP64 path keeps the reference's correctly rounded
     // powers; the FP32 path, whose fields carry 1e-7 errors, uses products.
-    const double iadx = sc->inv_adx, pk2 = sc->pk2, pk1 = sc->pk1, dxLD = k.dxLD;
+    const double iadx = sc->inv_adx, pk2 = sc->pk2, pk1 = sc->pk1;
     double gx = 0., gy = 0., gz = 0.;
     if constexpr (D == 1) {
         gx = (fwd[0] - ph) * iadx;
@@ -152,18 +168,42 @@ __device__ inline void energy_terms(double ph, const double (&fwd)[3], double pn
     double Ek = 0.5 * pk2 * pn * pn;
     double Et = Ek + Eg + Ep;
     double pp = pn * pk1;
-    acc[Q_KIN] += Ek * dxLD;
-    acc[Q_POT] += Ep * dxLD;
-    acc[Q_GRAD] += Eg * dxLD;
-    acc[Q_PHI] += ph * dxLD;
-    acc[Q_PHIP] += pp * dxLD;
-    acc[Q_VKIN] += Ek * dxLD * Ek * dxLD;
-    acc[Q_VPOT] += Ep * dxLD * Ep * dxLD;
-    acc[Q_VGRAD] += Eg * dxLD * Eg * dxLD;
-    acc[Q_VTOT] += Et * dxLD * Et * dxLD;
-    acc[Q_VPHI] += ph * dxLD * ph * dxLD;
-    acc[Q_VPHIP] += pn * pk1 * dxLD * pn * pk1 * dxLD;
-    acc[Q_KIN_OLD] += 0.5 * pk2 * po * po * dxLD;
+    // The reference weights every term by the site volume dxLD (dxLD^2 for
+    // the squares) and the old kinetic term by 0.5 pk2 as it adds it; these
+    // factors are the same for every site of a step, so the sums here are
+    // unweighted and energy_partials applies them once per thread.
+    acc[Q_KIN] += Ek;
+    acc[Q_POT] += Ep;
+    acc[Q_GRAD] += Eg;
+    acc[Q_PHI] += ph;
+    acc[Q_PHIP] += pp;
+    acc[Q_VKIN] += Ek * Ek;
+    acc[Q_VPOT] += Ep * Ep;
+    acc[Q_VGRAD] += Eg * Eg;
+    acc[Q_VTOT] += Et * Et;
+    acc[Q_VPHI] += ph * ph;
+    acc[Q_VPHIP] += pp * pp;
+    acc[Q_KIN_OLD] += po * po;
+}
+
+// Weight a thread's energy_terms sums (src/bubble.cpp:790-827 weights each
+// site's terms) and write the block's partials.
+__device__ inline void energy_partials(double (&acc)[NQ], const SimConst &k, const DevScalars *sc, double *out)
+{
+    const double v = k.dxLD, v2 = v * v;
+    acc[Q_KIN] *= v;
+    acc[Q_POT] *= v;
+    acc[Q_GRAD] *= v;
+    acc[Q_PHI] *= v;
+    acc[Q_PHIP] *= v;
+    acc[Q_VKIN] *= v2;
+    acc[Q_VPOT] *= v2;
+    acc[Q_VGRAD] *= v2;
+    acc[Q_VTOT] *= v2;
+    acc[Q_VPHI] *= v2;
+    acc[Q_VPHIP] *= v2;
+    acc[Q_KIN_OLD] *= 0.5 * sc->pk2 * v;
+    block_partials<NQ>(acc, out);
 }
 
 // Energies of the step and the scalar chain of oneStep, src/bubble.cpp:839-875.

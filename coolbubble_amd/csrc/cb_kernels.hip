@@ -77,7 +77,7 @@ k_step_simple(SimConst k, const T *__restrict__ phi_in, T *__restrict__ phi_out,
             energy_terms<D, sizeof(T) == 8>((double)ph, fwd, (double)pn, (double)po, k, sc, acc);
         }
     }
-    if constexpr (ENERGY) block_partials<NQ>(acc, partials + (long long)blockIdx.x * NQ);
+    if constexpr (ENERGY) energy_partials(acc, k, sc, partials + (long long)blockIdx.x * NQ);
 }
 
 // ---------------------------------------------------------------------------

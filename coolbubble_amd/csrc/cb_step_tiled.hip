@@ -224,7 +224,7 @@ __device__ __forceinline__ void tile_step(const SimConst &k, const TiledGeom &g,
             }
         }
     }
-    if constexpr (ENERGY) block_partials<NQ>(acc, partials + (long long)lt * NQ);
+    if constexpr (ENERGY) energy_partials(acc, k, sc, partials + (long long)lt * NQ);
     lds_barrier();
 }
 

@@ -270,7 +270,7 @@ __device__ __forceinline__ void tile_step2(const SimConst &k, const TiledGeom &g
             q1c[u] = q1n[u];
         }
     }
-    if constexpr (ENERGY) block_partials<NQ>(acc, partials + (long long)lt * NQ);
+    if constexpr (ENERGY) energy_partials(acc, k, sc, partials + (long long)lt * NQ);
 }
 
 } // namespace cb

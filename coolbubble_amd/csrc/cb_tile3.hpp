@@ -296,7 +296,7 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
         if (p + 2 > i1 + 1) break;
         plane(I2{}, p + 2);
     }
-    if constexpr (ENERGY) block_partials<NQ>(acc, partials + (long long)lt * NQ);
+    if constexpr (ENERGY) energy_partials(acc, k, sc, partials + (long long)lt * NQ);
 }
 
 } // namespace cb
