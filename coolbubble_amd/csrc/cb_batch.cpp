@@ -283,7 +283,7 @@ cb_status enqueue_steps(cb_batch *b, int n)
         }
     }
     for (int i : b->general) {
-        cb_status r = cb::step_n(b->mem[(size_t)i], n);
+        cb_status r = cb::step_n(b->mem[(size_t)i], n, false);   // one buffer flip per step, as replays assume
         if (r != CB_OK) return r;
     }
     for (int i : b->tiled) cb::advance_host(b->mem[(size_t)i], n);

@@ -280,6 +280,96 @@ hipError_t launch_step_simple(const SimConst &k, int fp32, int lagged, int energ
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Several 1D steps per launch (temporal blocking; force, kick and drift of
+// src/bubble.cpp:716-743, :854-856, :868-870 per step).  A 1D lattice of 2^16
+// sites is 0.5 MiB a field: one launch per step
+// leaves the GPU idle between launches.  Block b owns sites
+// [b kSeg1d, b kSeg1d + seg) and loads them with `steps` sites either side
+// (periodic) into LDS; step t updates window positions [t, W - t), whose
+// neighbours are still exact after t - 1 steps; the owned sites are written
+// after the last step.  The site update is the expression of k_step_simple's
+// non-lagged 1D case in the same translation unit, so the fields equal
+// `steps` one-step launches bit for bit.  No energies: a call's last step
+// runs on k_step_simple with them.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads1d)
+k_step1d_multi(SimConst k, const T *__restrict__ phi_in, T *__restrict__ phi_out, const T *__restrict__ pi_in,
+               T *__restrict__ pi_out, const DevScalars *__restrict__ sc, int steps)
+{
+    using C = T;
+    constexpr int WMAX = kSeg1d + 2 * kMaxSteps1d;
+    constexpr int R = (WMAX + kThreads1d - 1) / kThreads1d;
+    __shared__ T sphi[WMAX], spi[WMAX];
+    const C dt = (C)k.dt;
+    const C fac_lap = (C)sc->fac_lap, fac_pot = (C)sc->fac_pot;
+    const StepF32 sf = step_f32(k, sc);
+    const long long N = k.N;
+    const long long base = (long long)blockIdx.x * kSeg1d;
+    const int seg = (int)(N - base < kSeg1d ? N - base : kSeg1d);
+    const int W = seg + 2 * steps;
+    for (int j = threadIdx.x; j < W; j += kThreads1d) {
+        long long g = (base - steps + j) % N;
+        if (g < 0) g += N;
+        sphi[j] = phi_in[g];
+        spi[j] = pi_in[g];
+    }
+    __syncthreads();
+    for (int t = 1; t <= steps; ++t) {
+        T nphi[R], npi[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = t + threadIdx.x + r * kThreads1d;
+            if (j < W - t) {
+                const C ph = sphi[j], P0 = sphi[j + 1], M0 = sphi[j - 1], po = spi[j];
+                C pn;
+                if constexpr (sizeof(T) == 8) {
+                    const C dd = P0 - (C)2.0 * ph + M0;
+                    const C F = lap_term(fac_lap * dd, k) - fac_pot * vprime(ph, k);
+                    pn = po + dt * F;
+                    npi[r] = (T)pn;
+                    nphi[r] = (T)(ph + dt * pn);
+                } else {
+                    const C F = force_f32<1>(ph, P0, M0, P0, M0, P0, M0, sf);
+                    pn = __builtin_fmaf(sf.dt, F, po);
+                    npi[r] = pn;
+                    nphi[r] = __builtin_fmaf(sf.dt, pn, ph);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = t + threadIdx.x + r * kThreads1d;
+            if (j < W - t) {
+                sphi[j] = nphi[r];
+                spi[j] = npi[r];
+            }
+        }
+        __syncthreads();
+    }
+    for (int j = threadIdx.x; j < seg; j += kThreads1d) {
+        phi_out[base + j] = sphi[steps + j];
+        pi_out[base + j] = spi[steps + j];
+    }
+}
+
+hipError_t launch_step1d_multi(const SimConst &k, int fp32, const StepPtrs &p, const DevScalars *sc, int steps,
+                               hipStream_t st)
+{
+    if (k.D != 1 || steps < 1 || steps > kMaxSteps1d || p.pi_in == p.pi_out || p.phi_in == p.phi_out)
+        return hipErrorInvalidValue;
+    const unsigned grid = (unsigned)((k.nsites + kSeg1d - 1) / kSeg1d);
+    if (fp32)
+        k_step1d_multi<float><<<grid, kThreads1d, 0, st>>>(k, (const float *)p.phi_in, (float *)p.phi_out,
+                                                          (const float *)p.pi_in, (float *)p.pi_out, sc, steps);
+    else
+        k_step1d_multi<double><<<grid, kThreads1d, 0, st>>>(k, (const double *)p.phi_in, (double *)p.phi_out,
+                                                           (const double *)p.pi_in, (double *)p.pi_out, sc, steps);
+    return hipGetLastError();
+}
+
 hipError_t launch_ctor_energy(const SimConst &k, int fp32, const void *phi, const void *pi,
                               double *partials, SimpleCfg cfg, hipStream_t st)
 {

@@ -27,6 +27,13 @@ hipError_t launch_step_simple(const SimConst &k, int fp32, int lagged, int energ
                               const StepPtrs &p, const DevScalars *sc, double *partials,
                               SimpleCfg cfg, hipStream_t st);
 
+// Up to kMaxSteps1d non-lagged 1D steps in one launch, no energies; blocks
+// of kThreads1d threads own kSeg1d sites each.  phi and pi are read from the
+// *_in buffers and written to distinct *_out buffers.
+constexpr int kSeg1d = 1024, kMaxSteps1d = 32, kThreads1d = 256;
+hipError_t launch_step1d_multi(const SimConst &k, int fp32, const StepPtrs &p, const DevScalars *sc, int steps,
+                               hipStream_t st);
+
 // Constructor pieces (src/bubble.cpp:239-291).
 hipError_t launch_ctor_energy(const SimConst &k, int fp32, const void *phi, const void *pi,
                               double *partials, SimpleCfg cfg, hipStream_t st);

@@ -78,8 +78,10 @@ long long footprint_of(const cb_params &p)
     for (int d = 0; d < p.D; ++d) sites *= N;
     const long long esz = p.precision == CB_FP32 ? 4 : 8;
     const bool lagged = p.expansion && p.D == 3;
-    // phi and pi double-buffered (expanding box, or a two-step variant)
-    const long long fields = sites * esz * (lagged || p.D >= 2 ? 4 : 3);
+    // phi and pi double-buffered (expanding box, a multi-step variant, or
+    // the 1D launches of several steps)
+    (void)lagged;
+    const long long fields = sites * esz * 4;
     const long long rows = sites / N, nc = N / 2 + 1;
     const long long ncp = (nc + 7) / 8 * 8;   // padded rows of the fused power-of-two spectrum
     const long long fft = rows * ncp * 2 * esz + N * 2 * esz + 256 * 3 * 2048 * 8;

@@ -223,7 +223,7 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
     const size_t bytes = (size_t)s->nsites * s->esz;
     for (int i = 0; i < 2; ++i) {
         HIP_TRY(hipMalloc(&s->phi[i], bytes));
-        if (i == 0 || s->lagged) HIP_TRY(hipMalloc(&s->pi[i], bytes));
+        if (i == 0 || s->lagged || s->p.D == 1) HIP_TRY(hipMalloc(&s->pi[i], bytes));
     }
     HIP_TRY(hipMalloc((void **)&s->partials, sizeof(double) * cb::NQ * (size_t)s->max_blocks));
     HIP_TRY(hipMalloc((void **)&s->dtot, sizeof(double) * cb::NQ));

@@ -55,7 +55,11 @@ struct cb_sim {
 namespace cb {
 // spectrum of the current phi (cb_fft.hip); bins filled with the reference's
 // non-empty-shell convention
-cb_status step_n(cb_sim *s, int nsteps);
+// nsteps leapfrog steps.  multi1d: a 1D simulation takes all but the last
+// step in launches of several steps, which flip its buffers once per launch;
+// a caller that replays captured steps and expects one flip of phi per step
+// and pi in place (the batch's general members) passes false.
+cb_status step_n(cb_sim *s, int nsteps, bool multi1d = true);
 // cb_create, optionally keeping pi at t = 0 (the initial snapshot's kinetic map)
 cb_status create_sim(const cb_params *p, const double *xns, int n_xns, bool keep_pi0, cb_sim **out);
 // apply a lagged drift still pending on phi (expanding runs)

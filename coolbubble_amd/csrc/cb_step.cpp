@@ -65,7 +65,7 @@ static hipError_t launch_multi(cb_sim *s, int fuse, bool energy)
     return e;
 }
 
-cb_status step_n(cb_sim *s, int nsteps)
+cb_status step_n(cb_sim *s, int nsteps, bool multi1d)
 {
     const int fuse = s->variant >= 0 ? s->variants[(size_t)s->variant].fuse : 1;
     int it = 0;
@@ -90,6 +90,30 @@ cb_status step_n(cb_sim *s, int nsteps)
             s->step += 1;
         }
         return CB_OK;
+    }
+    if (multi1d && s->p.D == 1 && !s->lagged && nsteps > 1) {
+        // every step but the last (which measures the energies) in runs of
+        // up to kMaxSteps1d per launch, into the second pi buffer that a 1D
+        // simulation allocates at construction (no allocation here: calls
+        // may be captured into a graph)
+        if (!s->pi[1]) {
+            set_error("1D simulation without its second pi buffer");
+            return CB_UNSUPPORTED;
+        }
+        while (nsteps - 1 - it > 0) {
+            const int m = std::min(kMaxSteps1d, nsteps - 1 - it);
+            StepPtrs p;
+            p.phi_in = s->phi[s->iphi];
+            p.phi_out = s->phi[s->iphi ^ 1];
+            p.pi_in = s->pi[s->ipi];
+            p.pi_out = s->pi[s->ipi ^ 1];
+            HIP_TRY(launch_step1d_multi(s->k, s->fp32, p, s->dsc, m, s->st));
+            s->iphi ^= 1;
+            s->ipi ^= 1;
+            for (int q = 0; q < m; ++q) s->t += s->p.dt;
+            s->step += m;
+            it += m;
+        }
     }
     for (; it < nsteps; ++it) {
         const bool energy = s->lagged || it == nsteps - 1;

@@ -289,6 +289,31 @@ def test_long_1d_spectrum_matches_oracle(N):
     np.testing.assert_allclose(d32[big], do[big], rtol=1e-3)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("N", [100, 3000, 65536])
+def test_1d_multi_step_launches_bit_identical(N, precision):
+    """A 1D call runs its steps up to 32 per launch (each block's sites and
+    their light cone in LDS), the last one on the one-step kernel with the
+    energies.  Fields and energy row equal those of one-step calls bit for
+    bit: a lattice shorter than a block's window (100 sites, 100 + 2 x 32
+    loaded) and one that is not a multiple of a block's 1024 sites included.
+    75 steps = launches of 32, 32 and 10 steps, then one."""
+    kw = params(D=1, nGrids=N, L=N * 0.0625, dt=0.0125)
+    a = cb.Bubble(precision=precision, **kw)
+    b = cb.Bubble(precision=precision, **kw)
+    a.step(75)
+    for _ in range(75):
+        b.step(1)
+    (pa, qa), (pb, qb) = a.fields(), b.fields()
+    assert np.any(pa != pa.flat[0])
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(qa, qb)
+    np.testing.assert_array_equal(a.energy_row(), b.energy_row())
+    assert a.state().t == b.state().t
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("fuse", [2, 3])
 @pytest.mark.parametrize("name", ["2d_mode1", "3d_mode-2", "3d_mode1"])
 def test_multi_step_variants_bit_identical_with_odd_counts(name, fuse):
