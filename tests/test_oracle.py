@@ -367,7 +367,7 @@ def test_spectrum_of_a_single_mode(D, N, m):
 # --------------------------------------------------------------------------
 # thermal initial conditions (src/bubble.cpp:459-523)
 # --------------------------------------------------------------------------
-@pytest.mark.parametrize("D,N", [(2, 32), (3, 16), (2, 15), (3, 9)])
+@pytest.mark.parametrize("D,N", [(2, 32), (3, 16), (2, 15), (3, 9), (2, 33), (3, 7)])
 def test_thermal_modes_hermitian_and_c2r_matches_numpy(D, N):
     L, T, m = 4.0, 0.6, 2.0
     X = oracle.thermal_modes(D, N, L, T, m, velocity=False)
@@ -381,17 +381,18 @@ def test_thermal_modes_hermitian_and_c2r_matches_numpy(D, N):
             else:
                 assert rows[r, j] == np.conj(rows[pr, j])
     field = oracle.thermal_field(D, N, L, T, m, velocity=False)
+    # numpy's irfftn takes the half array as the stored half of a Hermitian
+    # array, as FFTW's c2r does; for odd N the reference's "fix" of plane
+    # j = N/2 (a regular plane there) only changes the data, not the rule
     want = np.fft.irfftn(X, s=(N,) * D, axes=tuple(range(D))) * N ** D / math.sqrt(L ** D)
-    if N % 2 == 0:
-        np.testing.assert_allclose(field, want, atol=1e-13 * np.abs(want).max())
+    np.testing.assert_allclose(field, want, atol=1e-13 * np.abs(want).max())
     # Parseval: sum_x field^2 = N^D L^-D sum over the Hermitian-extended array
     w = np.full(N // 2 + 1, 2.0)
     w[0] = 1.0
     if N % 2 == 0:
         w[N // 2] = 1.0
     want_sq = N ** D / L ** D * np.sum(w * np.abs(rows) ** 2)
-    if N % 2 == 0:
-        assert abs(np.sum(field ** 2) - want_sq) <= 1e-12 * want_sq
+    assert abs(np.sum(field ** 2) - want_sq) <= 1e-12 * want_sq
 
 
 def test_thermal_mode_variance_is_the_bose_einstein_spectrum():
@@ -546,3 +547,233 @@ def test_wall_tracker_schedule_and_line_format():
     assert w.track(200, 2.5, phi).startswith("2.500 1.0000 ")
     assert w.track(700, 8.75, phi) == ""                           # t >= L / 2
     assert oracle.WallTracker(2, 64, 16.0, 4.0, 1, dt=0.05).track(14, 0.7, np.zeros((64, 64))) == ""
+
+
+# --------------------------------------------------------------------------
+# critical-bubble fits and seeded profiles (src/bubble.cpp:81-108,324-393)
+# --------------------------------------------------------------------------
+# The fits as src/bubble.cpp:81-99 states them, evaluated once at 40 digits
+# with Python's decimal module (independent of glibc's exp/pow):
+#   phiC = 0.997 (1 - exp(-0.688 c)), R = 0.823 c^0.203 (x 1.3 in 3D),
+#   lw = 0.496 c^-0.531.
+FIT_KAT = {   # c: (phiC, R in 1D/2D, R in 3D, lw)
+    1.0: (0.49592751564964788, 0.82299999999999995, 1.0699000000000001, 0.496),
+    2.0: (0.74517087806114957, 0.94734665136274865, 1.2315506467715733, 0.34326912958076705),
+    4.0: (0.93339126714494247, 1.0904807750282057, 1.4176250075366674, 0.23756793411922872),
+    7.5: (0.9912755254134028, 1.2389029061226764, 1.6105737779594793, 0.17014694104868963),
+}
+
+
+@pytest.mark.parametrize("c", sorted(FIT_KAT))
+def test_bubble_fit_known_answers(c):
+    phiC, R2, R3, lw = FIT_KAT[c]
+    for D, R in ((1, R2), (2, R2), (3, R3)):
+        got = oracle.bubble_fit(D, c)
+        np.testing.assert_allclose(got, [phiC, R, lw], rtol=4e-16, atol=0)
+    # nucleated 1.3 times larger in 3D (src/bubble.cpp:87-94)
+    assert abs(oracle.bubble_fit(3, c)[1] / oracle.bubble_fit(2, c)[1] - 1.3) <= 4e-16
+
+
+def _o3_bounce_centre(c):
+    """Centre value of the O(3)-symmetric critical bubble of V (the thermal
+    bounce: phi'' + (2/r) phi' = V'(phi), phi'(0) = 0, phi -> 0 far away),
+    found by overshoot/undershoot shooting on phi(0)."""
+    from scipy.integrate import solve_ivp
+
+    def vp(f):
+        return 2 * c * f - 3 * (2 * c + 4) * f * f + 4 * (c + 3) * f ** 3
+
+    def overshoots(f0):
+        r0 = 1e-6
+        y0 = [f0 + vp(f0) * r0 * r0 / 6, vp(f0) * r0 / 3]
+        crossed = lambda r, y: y[0]
+        crossed.terminal = True
+        turned = lambda r, y: y[1]
+        turned.terminal, turned.direction = True, 1
+        sol = solve_ivp(lambda r, y: [y[1], vp(y[0]) - 2 / r * y[1]], [r0, 200], y0,
+                        events=[crossed, turned], rtol=1e-9, atol=1e-12)
+        return len(sol.t_events[0]) > 0
+
+    lo, hi = c / (2 * c + 6), 1.0 - 1e-15     # barrier top .. true vacuum
+    while hi - lo > 1e-9:
+        mid = 0.5 * (lo + hi)
+        lo, hi = (lo, mid) if overshoots(mid) else (mid, hi)
+    return 0.5 * (lo + hi)
+
+
+@pytest.mark.parametrize("c", [1.0, 2.0, 4.0, 8.0])
+def test_bubble_centre_fit_is_the_thermal_critical_bubble(c):
+    """phizero (src/bubble.cpp:81-84) fits the centre value of the critical
+    bubble: the O(3) bounce of this potential reproduces it within 0.5 %
+    over c = 1..8 (a changed constant, sign or exponent misses by far more)."""
+    got = oracle.bubble_fit(3, c)[0]
+    bounce = _o3_bounce_centre(c)
+    assert abs(got - bounce) <= 5e-3 * bounce, (got, bounce)
+
+
+def _profile(r, c, D):
+    phiC, R2, R3, lw = FIT_KAT[c]
+    R = R3 if D == 3 else R2
+    return (phiC / 2.0) * (np.tanh((R - r) / lw) + 1.0)
+
+
+def _coords(D, N, L):
+    """Cell-centred coordinates x = L (i + 1/2) / N (src/gridfuncs.cpp:77-85)."""
+    x = (np.arange(N) + 0.5) * L / N
+    return np.meshgrid(*([x] * D), indexing="ij")
+
+
+@pytest.mark.parametrize("D,N,L,c,mode,DR", [(2, 40, 8.0, 4.0, -2, 0.0), (3, 24, 6.0, 2.0, -2, 0.0),
+                                            (2, 48, 6.0, 4.0, -1, 0.3), (3, 20, 5.0, 7.5, -1, 0.25),
+                                            (1, 96, 6.0, 4.0, 1, 0.0), (1, 64, 1.5, 1.0, 1, 0.0)])
+def test_seeded_profiles_at_known_distances(D, N, L, c, mode, DR):
+    """phi after seeding = the tanh profile of the fits at each site's
+    distance from the centres, added in order and capped at phiC
+    (NucleateOneBubble :356-369, NucleateTwoDiagonalBubbles :373-393,
+    Initialize1D :324-335: the 1D bubble sits on the box edge, profiles
+    around x = 0 and x = L).  In the 1D box with L = 1.5 the two profiles
+    overlap and the cap phiC is reached."""
+    s = oracle.Sim(D=D, nGrids=N, L=L, c=c, readxns=mode, DeltaR=DR, dt=0.2 * L / N)
+    phiC = FIT_KAT[c][0]
+    X = _coords(D, N, L)
+    if D == 1:
+        centres = [(0.0,), (L,)]
+    elif mode == -2:
+        centres = [(L / 2,) * D]
+    else:
+        centres = [(L / 4 + DR, L / 4 + DR, L / 2)[:D], (3 * L / 4 - DR, 3 * L / 4 - DR, L / 2)[:D]]
+    want = np.zeros((N,) * D)
+    for ctr in centres:
+        r = np.sqrt(sum((X[a] - ctr[a]) ** 2 for a in range(D)))
+        want = np.minimum(want + _profile(r, c, D), phiC)
+    np.testing.assert_allclose(s.phi(), want, rtol=0, atol=4e-15)   # ~ulp of 1 + tanh, times phiC
+    assert s.phi().max() <= phiC
+    if L == 1.5:
+        assert np.sum(s.phi() == phiC) > 0
+    assert s.phi().max() > 0.9 * phiC     # the lattice crosses the bubble interior
+
+
+@pytest.mark.parametrize("D,c", [(2, 4.0), (3, 2.0), (3, 7.5)])
+def test_profile_is_half_the_centre_value_at_the_radius(D, c):
+    """A centre placed exactly R from a site gives phi = phiC / 2 there."""
+    phiC, R2, R3, _ = FIT_KAT[c]
+    R = R3 if D == 3 else R2
+    N, L = 16, 4.0
+    site = (3, 5, 7)[:D]
+    x = [(i + 0.5) * L / N for i in site]
+    ctr = np.array([[x[0] + R] + x[1:] + [0.0] * (3 - D)])
+    idx = np.ravel_multi_index(site, (N,) * D)
+    got = oracle.seed_sites(ctr, [idx], D=D, nGrids=N, L=L, c=c)[0]
+    # (x + R) - x is R to an ulp of x + R; the profile's slope there is phiC / (2 lw)
+    tol = 2 * np.spacing(x[0] + R) * phiC / (2 * FIT_KAT[c][3]) + 2 * np.spacing(phiC)
+    assert abs(got - phiC / 2) <= tol
+    # one wall width further out the profile is (phiC / 2)(1 - tanh 1)
+    ctr[0, 0] = x[0] + R + FIT_KAT[c][3]
+    got = oracle.seed_sites(ctr, [idx], D=D, nGrids=N, L=L, c=c)[0]
+    assert abs(got - phiC / 2 * (1 - math.tanh(1.0))) <= 2 * tol
+
+
+# --------------------------------------------------------------------------
+# ctor back-step and initial scale factor (src/bubble.cpp:239-291)
+# --------------------------------------------------------------------------
+def _numpy_ctor(phi, pi0, D, N, L, c, dt, expanding, Mpl, alpha):
+    """The ctor's sequence evaluated independently: energies with forward
+    differences / dx, H0 from the Friedmann equation, a' = a H0 - dt/2 a'',
+    pi_{-1/2} = pi_0 - dt/2 F(phi_0), and the kinetic energy of pi_{-1/2}."""
+    dx = L / N
+    dxLD = (1.0 / N) ** D
+    ax = tuple(range(D))
+    lap = sum(np.roll(phi, -1, a) - 2 * phi + np.roll(phi, 1, a) for a in ax)
+    grad = 0.5 * sum(((np.roll(phi, -1, a) - phi) / dx) ** 2 for a in ax)
+    V = c * phi ** 2 - (2 * c + 4) * phi ** 3 + (c + 3) * phi ** 4 + 1.0
+    K, G, P = (0.5 * pi0 ** 2).sum() * dxLD, grad.sum() * dxLD, V.sum() * dxLD
+    E = K + G + P
+    H0 = math.sqrt(E / (3 * Mpl ** 2)) if expanding else 0.0
+    if not expanding:
+        add = 0.0
+    elif alpha == 0:
+        add = -(2 * K - P) / (3 * Mpl ** 2)
+    else:
+        add = (2 * P - K + G) / (3 * Mpl ** 2)
+    ap = H0 - 0.5 * dt * add
+    F = lap / dx ** 2 - _vprime(phi, c)
+    pim = pi0 - 0.5 * dt * F
+    a_mh = 1.0 - 0.5 * dt * ap
+    ekh = (0.5 * a_mh ** (-2 * D) * pim ** 2).sum() * dxLD
+    return dict(row=(E, K, G, P), H0=H0, ap=ap, pim=pim, ekh=ekh, F=F)
+
+
+@pytest.mark.parametrize("D,N,expanding,alpha", [(2, 20, 0, 0), (3, 10, 0, 0), (3, 10, 1, 0), (3, 12, 1, 1)])
+def test_ctor_back_step_and_scale_factor_on_a_random_field(D, N, expanding, alpha):
+    L, c, dt, Mpl = 5.0, 3.0, 0.05, 0.8
+    kw = dict(D=D, nGrids=N, L=L, c=c, dt=dt, expansion=expanding, Mpl=Mpl, alpha=alpha)
+    rng = np.random.default_rng(100 * D + N + alpha)
+    phi = 0.5 + 0.3 * rng.standard_normal((N,) * D)
+    pi0 = 0.4 * rng.standard_normal((N,) * D)
+    s = oracle.Sim(blank=True, **kw)
+    s.phi()[...] = phi
+    s.pi()[...] = pi0
+    s.ctor()
+    want = _numpy_ctor(phi, pi0, D, N, L, c, dt, expanding, Mpl, alpha)
+    row, sc = s.energy_row(), s.scalars()
+    np.testing.assert_allclose(row[2:6], want["row"], rtol=1e-13)
+    assert abs(sc[1] - want["ap"]) <= 1e-13 * max(abs(want["ap"]), 1e-300) + (0 if expanding else 0.0)
+    assert row[14] == sc[2] and abs(sc[2] - want["H0"]) <= 1e-14 * max(want["H0"], 1e-300)
+    assert sc[0] == 1.0 and sc[4] == 0.0
+    scale = np.abs(pi0).max() + 0.5 * dt * np.abs(want["F"]).max()
+    assert np.max(np.abs(s.pi() - want["pim"])) <= 1e-14 * scale
+    assert abs(sc[5] - want["ekh"]) <= 1e-13 * want["ekh"]
+    # the back-step moves pi by half a step of the force; EkinHalfPrev is the
+    # kinetic energy of pi_{-1/2} with a_{-1/2}, not of pi_0
+    assert abs(sc[5] - row[3]) > 1e-6 * row[3]
+    if expanding:
+        assert want["ap"] != want["H0"]
+
+
+@pytest.mark.parametrize("alpha", [0, 1])
+def test_whole_thermal_ctor_back_step(alpha):
+    """The whole ctor of a thermal start (mode -3: pi_0 drawn as well) leaves
+    pi_{-1/2}, the scalars and EkinHalfPrev of the independent evaluation of
+    its own phi_0 and pi_0 (thermal fields, pinned against numpy above)."""
+    D, N, L, c, dt, Mpl, Trad = 3, 16, 8.0, 2.0, 0.04, 1.5, 0.5
+    s = oracle.Sim(D=D, nGrids=N, L=L, c=c, dt=dt, readxns=-3, Trad=Trad, expansion=1, Mpl=Mpl, alpha=alpha)
+    T, m = Trad * math.sqrt(2 * (c + 6)), math.sqrt(2 * c)
+    phi = oracle.thermal_field(D, N, L, T, m, velocity=False)
+    pi0 = oracle.thermal_field(D, N, L, T, m, velocity=True)
+    np.testing.assert_array_equal(s.phi(), phi)
+    want = _numpy_ctor(phi, pi0, D, N, L, c, dt, True, Mpl, alpha)
+    row, sc = s.energy_row(), s.scalars()
+    np.testing.assert_allclose(row[2:6], want["row"], rtol=1e-13)
+    assert abs(sc[1] - want["ap"]) <= 1e-13 * abs(want["ap"])
+    scale = np.abs(pi0).max() + 0.5 * dt * np.abs(want["F"]).max()
+    assert np.max(np.abs(s.pi() - want["pim"])) <= 1e-14 * scale
+    assert abs(sc[5] - want["ekh"]) <= 1e-13 * want["ekh"]
+
+
+# --------------------------------------------------------------------------
+# Friedmann update (src/bubble.cpp:747-757,858-875)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("alpha,Mpl,T", [(0, 1.0, 2.0), (1, 2.0, 1.0)])
+def test_hubble_constraint_converges_at_second_order(alpha, Mpl, T):
+    """With a'' from the second Friedmann equation, the lattice system
+    conserves the first one, H^2 = a^(2 alpha) Etot / (3 Mpl^2), exactly in
+    continuous time (the leapfrog forces are the gradients of the lattice
+    energies).  So the constraint C written in column 15 (:873-875) is the
+    integrator's O(dt^2) error: C / H^2 at a fixed time falls 4x when dt
+    halves.  The start has K ~ V, so a wrong kinetic term in a'' (cosmic
+    time -a (2K - V), conformal a^3 (2V - K + G)) would leave an O(1)
+    constraint violation that does not converge."""
+    vals, a_end = [], []
+    for dt in (0.05, 0.025, 0.0125):
+        s = oracle.Sim(D=3, nGrids=16, L=8.0, c=2.0, readxns=-3, Trad=0.5, dt=dt, expansion=1, Mpl=Mpl,
+                       alpha=alpha)
+        r0 = s.energy_row()
+        assert r0[3] > 0.5 * r0[5] and r0[15] == 0.0           # kinetic ~ potential at the start
+        s.step(int(round(T / dt)))
+        r = s.energy_row()
+        vals.append(r[15] / r[14] ** 2)
+        a_end.append(r[1])
+    assert a_end[-1] > 1.5                                      # the background has expanded
+    assert abs(vals[-1]) < 1e-3
+    for coarse, fine in zip(vals, vals[1:]):
+        assert 3.6 < coarse / fine < 4.6, vals
