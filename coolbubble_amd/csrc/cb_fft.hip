@@ -16,6 +16,7 @@
 //            coalesced load
 // each as a radix-4/2/3 (+ any prime) Stockham sweep between two LDS
 // buffers with twiddles from a table computed in long double on the host.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -224,8 +225,12 @@ k_cols(cpx<R> *H, long long inner, int CB, Stages st, const cpx<R> *__restrict__
 }
 
 // Radial binning of the half-complex spectrum, ComputePowerSpectrum
-// (src/bubble.cpp:668-709): per block an LDS histogram of (power, weight,
-// weighted k) over a run of rows, written out as one partial per block.
+// (src/bubble.cpp:668-709): per block one row of (power, weight, weighted
+// k) partials over a contiguous run of modes.  Thread t owns the bins
+// [b0, b1): along a row the bin never decreases with j, so the thread starts
+// at the row's first j of bin b0 (an integer square root) and walks up to
+// bin b1; only the owner writes a bin's LDS slots, so the sums have a fixed
+// order (no atomics).
 // GEOM: weights and weighted k only (they do not depend on the field; H is
 // not read).
 template <typename R, bool GEOM = false>
@@ -237,37 +242,47 @@ k_bin(const cpx<R> *__restrict__ H, long long rows, int N, int D, int nBins, dou
     double *hp = reinterpret_cast<double *>(smem_raw);
     double *hw = hp + nBins;
     double *hk = hw + nBins;
-    for (int b = threadIdx.x; b < 3 * nBins; b += blockDim.x) hp[b] = 0.0;
-    __syncthreads();
+    const int bpt = bins_per_thread(nBins, blockDim.x);
+    const int b0 = threadIdx.x * bpt;
+    const int b1 = b0 + bpt < nBins ? b0 + bpt : nBins;
+    for (int b = b0; b < b1; ++b) hp[b] = hw[b] = hk[b] = 0.0;
     const int nc = N / 2 + 1;
     const long long total = rows * nc;
     const long long per = (total + gridDim.x - 1) / gridDim.x;
     const long long m0 = per * blockIdx.x;
     const long long m1 = (m0 + per < total) ? m0 + per : total;
-    for (long long m = m0 + threadIdx.x; m < m1; m += blockDim.x) {
-        const long long r = m / nc;
-        const int j = (int)(m - r * nc);
-        double n2rows = 0.;
-        if (D == 2) {
-            const int s = (r <= N / 2) ? (int)r : (int)r - N;
-            n2rows = (double)s * (double)s;
-        } else if (D == 3) {
-            const int a = (int)(r / N), c = (int)(r % N);
-            const int s1 = (a <= N / 2) ? a : a - N, s2 = (c <= N / 2) ? c : c - N;
-            n2rows = (double)s1 * (double)s1 + (double)s2 * (double)s2;
+    if (b0 < b1 && m0 < m1) {
+        for (long long r = m0 / nc; r * nc < m1; ++r) {
+            long long S = 0;   // n2rows, an integer
+            if (D == 2) {
+                const long long s = (r <= N / 2) ? r : r - N;
+                S = s * s;
+            } else if (D == 3) {
+                const long long a = r / N, c = r % N;
+                const long long s1 = (a <= N / 2) ? a : a - N, s2 = (c <= N / 2) ? c : c - N;
+                S = s1 * s1 + s2 * s2;
+            }
+            const long long jlo = (r * nc < m0) ? m0 - r * nc : 0;
+            const long long jhi = ((r + 1) * nc > m1) ? m1 - 1 - r * nc : nc - 1;
+            if ((long long)b1 * b1 <= S + jlo * jlo) continue;     // every bin >= b1
+            if ((long long)b0 * b0 > S + jhi * jhi) continue;      // every bin < b0
+            long long j = isqrt_ceil((long long)b0 * b0 - S);
+            if (j < jlo) j = jlo;
+            for (; j <= jhi; ++j) {
+                const double n = sqrt((double)S + (double)j * j);
+                const int bin = (int)n;
+                if (bin >= b1) break;
+                const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
+                const double kc = n * dk;
+                if (!GEOM) {
+                    const cpx<R> f = H[r * nc + j];
+                    const double fx = (double)f.x, fy = (double)f.y;
+                    hp[bin] += w * pref * ipow_rn(kc, D) * Pnorm * (fx * fx + fy * fy);
+                }
+                hw[bin] += w;
+                hk[bin] += w * kc;
+            }
         }
-        const double n = sqrt(n2rows + (double)j * j);
-        const int bin = (int)n;
-        if (bin >= nBins) continue;
-        const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
-        const double kc = n * dk;
-        if (!GEOM) {
-            const cpx<R> f = H[m];
-            const double fx = (double)f.x, fy = (double)f.y;
-            atomicAdd(&hp[bin], w * pref * ipow_rn(kc, D) * Pnorm * (fx * fx + fy * fy));
-        }
-        atomicAdd(&hw[bin], w);
-        atomicAdd(&hk[bin], w * kc);
     }
     __syncthreads();
     double *dst = binpart + (long long)blockIdx.x * 3 * nBins;
@@ -481,14 +496,34 @@ struct FftPlan {
     double *binpart = nullptr, *bins = nullptr;
     // power-of-two D = 2 / 3: binning fused into the last column pass, the
     // field-independent weights and weighted k of each bin computed once
+    // into bins[nBins, 3 nBins) (the pass itself writes bins[0, nBins))
     bool fused = false;
     int fused_blocks = 0;
-    std::vector<double> wsks;   // [nBins] weights, [nBins] weighted k
     // long 1D rows (four-step): N = N1 N2
     int N1 = 0, N2 = 0;
     Stages st1{}, st2{};
     void *tw1 = nullptr, *tw2 = nullptr;
+    // device buffers from hipMalloc (a simulation's own plan, freed by
+    // fft_destroy), or carved from the no-allocation scratch
+    bool owned = true;
 };
+
+// Device memory for a plan: hipMalloc when arena is null, else the next
+// 256-byte-aligned piece of the arena.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
+static hipError_t dev_get(Arena *a, void **out, size_t bytes)
+{
+    if (!a) return hipMalloc(out, bytes);
+    const size_t off = (a->used + 255) & ~(size_t)255;
+    if (off > a->cap || bytes > a->cap - off) return hipErrorOutOfMemory;
+    *out = a->base + off;
+    a->used = off + bytes;
+    return hipSuccess;
+}
 
 static bool factorise(int n, Stages &st)
 {
@@ -504,18 +539,26 @@ static bool factorise(int n, Stages &st)
     return st.ns <= kMaxStages;
 }
 
+// Twiddles exp(-2 pi i e / n), e < n, computed in long double on the host
+// and copied to device memory from dev_get.  stage: a host buffer of at
+// least n entries for the table (null: a temporary vector).
 template <typename R>
-static hipError_t make_twiddles(int n, void **out, hipStream_t s)
+static hipError_t make_twiddles(int n, void **out, hipStream_t s, Arena *arena = nullptr, void *stage = nullptr)
 {
-    std::vector<cpx<R>> h((size_t)n);
+    std::vector<cpx<R>> tmp;
+    cpx<R> *h = reinterpret_cast<cpx<R> *>(stage);
+    if (!h) {
+        tmp.resize((size_t)n);
+        h = tmp.data();
+    }
     const long double twopi = 6.283185307179586476925286766559005768L;
     for (int e = 0; e < n; ++e) {
         long double a = twopi * (long double)e / (long double)n;
-        h[(size_t)e] = {(R)cosl(a), (R)-sinl(a)};
+        h[e] = {(R)cosl(a), (R)-sinl(a)};
     }
-    hipError_t err = hipMalloc(out, sizeof(cpx<R>) * (size_t)n);
+    hipError_t err = dev_get(arena, out, sizeof(cpx<R>) * (size_t)n);
     if (err != hipSuccess) return err;
-    err = hipMemcpyAsync(*out, h.data(), sizeof(cpx<R>) * (size_t)n, hipMemcpyHostToDevice, s);
+    err = hipMemcpyAsync(*out, h, sizeof(cpx<R>) * (size_t)n, hipMemcpyHostToDevice, s);
     if (err == hipSuccess) err = hipStreamSynchronize(s);
     return err;
 }
@@ -598,6 +641,10 @@ static hipError_t long_row(FftPlan *p, const R *field, double pref, double dk, d
 void fft_destroy(FftPlan *p)
 {
     if (!p) return;
+    if (!p->owned) {
+        delete p;
+        return;
+    }
     if (p->tw) hipFree(p->tw);
     if (p->tw1) hipFree(p->tw1);
     if (p->tw2) hipFree(p->tw2);
@@ -616,11 +663,28 @@ void fft_destroy(FftPlan *p)
         }                                                                                \
     } while (0)
 
-static cb_status plan_for(cb_sim *s)
+// Clears n 8-byte words (a kernel rather than hipMemsetAsync, so a caller
+// may capture the launch into a graph).
+__global__ void __launch_bounds__(256) k_zero8(unsigned long long *p, long long n)
 {
-    if (s->fft) return CB_OK;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ULL;
+}
+
+static hipError_t zero_words(void *p, size_t bytes, hipStream_t st)
+{
+    const long long n = (long long)(bytes / 8);
+    if (n <= 0) return hipSuccess;
+    const long long blocks = std::min<long long>((n + 255) / 256, 8192);
+    k_zero8<<<(unsigned)blocks, 256, 0, st>>>((unsigned long long *)p, n);
+    return hipGetLastError();
+}
+
+// Shape of the transforms for s (no device work): factorisation, four-step
+// split of long 1D rows, fused binning, bin and block counts.
+static cb_status plan_shape(const cb_sim *s, FftPlan *p)
+{
     const int N = s->p.nGrids, D = s->p.D;
-    FftPlan *p = new FftPlan();
     p->N = N;
     p->D = D;
     p->fp32 = s->fp32;
@@ -644,54 +708,73 @@ static cb_status plan_for(cb_sim *s)
         }
     }
     if (!ok) {
-        delete p;
         set_error("FFT length not supported by the in-LDS transform");
         return CB_UNSUPPORTED;
     }
     p->rows = (D == 3) ? (long long)N * N : (D == 2 ? N : 1);
-    const int nc = N / 2 + 1;
-    const size_t csz = s->fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
-    hipError_t e = s->fp32 ? make_twiddles<float>(N, &p->tw, s->st) : make_twiddles<double>(N, &p->tw, s->st);
-    if (e == hipSuccess && p->N1) {
-        e = s->fp32 ? make_twiddles<float>(p->N1, &p->tw1, s->st) : make_twiddles<double>(p->N1, &p->tw1, s->st);
-        if (e == hipSuccess)
-            e = s->fp32 ? make_twiddles<float>(p->N2, &p->tw2, s->st) : make_twiddles<double>(p->N2, &p->tw2, s->st);
-    }
     p->fused = D >= 2 && !p->N1 && p2_supported(N);
-    // the fused path pads rows to p2_row_stride(N); the padding stays zero
-    const size_t ldh = p->fused ? (size_t)p2_row_stride(N) : (size_t)nc;
-    if (e == hipSuccess) e = hipMalloc(&p->H, csz * (p->N1 ? (size_t)N : (size_t)p->rows * ldh));
-    if (e == hipSuccess && p->fused) e = hipMemsetAsync(p->H, 0, csz * (size_t)p->rows * ldh, s->st);
     p->nBins = s->nkmax + 1;
-    long long modes = p->rows * nc;
-    p->binblocks = (int)((modes + 4095) / 4096);
-    if (p->binblocks > 256) p->binblocks = 256;
-    if (p->binblocks < 1) p->binblocks = 1;
-    size_t part = 3 * (size_t)p->nBins * p->binblocks;
-    if (p->fused) {
-        p->fused_blocks = p2_bin_blocks_max();
-        part = std::max(part, (size_t)p->nBins * 64);   // the fused pass's partial rows
-    }
-    if (e == hipSuccess) e = hipMalloc((void **)&p->binpart, sizeof(double) * part);
-    if (e == hipSuccess) e = hipMalloc((void **)&p->bins, sizeof(double) * 3 * (size_t)p->nBins);
-    if (e == hipSuccess && p->fused) {
-        // weights and weighted k per bin, once
-        const int nB = p->nBins;
-        const size_t shm = sizeof(double) * 3 * (size_t)nB;
-        if (shm > kLdsBlockMax || lds_opt_in(k_bin<double, true>, shm) != hipSuccess) {
-            fft_destroy(p);
+    const long long modes = p->rows * (N / 2 + 1);
+    p->binblocks = (int)std::max<long long>(1, std::min<long long>(256, (modes + 4095) / 4096));
+    if (p->fused) p->fused_blocks = p2_bin_blocks_max();
+    if (p->fused || (!p->N1 && D >= 2)) {
+        const size_t shm = sizeof(double) * 3 * (size_t)p->nBins;
+        if (shm > kLdsBlockMax) {
             set_error("too many spectrum bins for LDS");
             return CB_UNSUPPORTED;
         }
-        const double dk = 2.0 * 3.14159265358979323846 / s->p.L;
-        k_bin<double, true><<<p->binblocks, 256, shm, s->st>>>(nullptr, p->rows, N, D, nB, dk, 0.0, 0.0, p->binpart);
-        k_bin_reduce<<<(3 * nB + 63) / 64, 256, 0, s->st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
-        std::vector<double> h(3 * (size_t)nB);
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), p->bins, sizeof(double) * 3 * nB, hipMemcpyDeviceToHost, s->st);
-        if (e == hipSuccess) e = hipStreamSynchronize(s->st);
-        p->wsks.assign(h.begin() + nB, h.end());
     }
+    return CB_OK;
+}
+
+// Device buffers of a shaped plan (hipMalloc, or carved from arena), its
+// twiddles (host staging in stage, or a temporary), the padding of the fused
+// path's array cleared, and the fused path's per-bin weights and weighted k.
+static hipError_t plan_buffers(FftPlan *p, Arena *arena, void *stage, double L, hipStream_t st)
+{
+    const int N = p->N, D = p->D, nc = N / 2 + 1;
+    const size_t csz = p->fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+    const size_t ldh = p->fused ? (size_t)p2_row_stride(N) : (size_t)nc;
+    const size_t hbytes = csz * (p->N1 ? (size_t)N : (size_t)p->rows * ldh);
+    size_t part = 3 * (size_t)p->nBins * p->binblocks;
+    if (p->fused) part = std::max(part, (size_t)p->nBins * p->fused_blocks);   // the fused pass's partial rows
+    // every buffer first, so that a scratch too small fails before any launch
+    hipError_t e = dev_get(arena, &p->H, hbytes);
+    if (e == hipSuccess) e = dev_get(arena, (void **)&p->binpart, sizeof(double) * part);
+    if (e == hipSuccess) e = dev_get(arena, (void **)&p->bins, sizeof(double) * 3 * (size_t)p->nBins);
+    if (e != hipSuccess) return e;
+    auto tw = [&](int n, void **out) {
+        return p->fp32 ? make_twiddles<float>(n, out, st, arena, stage) : make_twiddles<double>(n, out, st, arena, stage);
+    };
+    e = tw(N, &p->tw);
+    if (e == hipSuccess && p->N1) e = tw(p->N1, &p->tw1);
+    if (e == hipSuccess && p->N1) e = tw(p->N2, &p->tw2);
+    // the fused path pads rows to p2_row_stride(N); the padding stays zero
+    if (e == hipSuccess && p->fused) e = zero_words(p->H, hbytes, st);
+    if (e == hipSuccess && p->fused) {
+        const int nB = p->nBins;
+        const size_t shm = sizeof(double) * 3 * (size_t)nB;
+        e = lds_opt_in(k_bin<double, true>, shm);
+        if (e != hipSuccess) return e;
+        const double dk = 2.0 * 3.14159265358979323846 / L;
+        k_bin<double, true><<<p->binblocks, 256, shm, st>>>(nullptr, p->rows, N, D, nB, dk, 0.0, 0.0, p->binpart);
+        k_bin_reduce<<<(3 * nB + 63) / 64, 256, 0, st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+static cb_status plan_for(cb_sim *s)
+{
+    if (s->fft) return CB_OK;
+    FftPlan *p = new FftPlan();
+    cb_status r = plan_shape(s, p);
+    if (r != CB_OK) {
+        delete p;
+        return r;
+    }
+    hipError_t e = plan_buffers(p, nullptr, nullptr, s->p.L, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
     if (e != hipSuccess) {
         fft_destroy(p);
         set_error(std::string("FFT plan: ") + hipGetErrorString(e));
@@ -701,11 +784,11 @@ static cb_status plan_for(cb_sim *s)
     return CB_OK;
 }
 
-cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int cap, int *nbins)
+// The spectrum of s's current phi with the plan p; h: host buffer of
+// 3 nBins doubles.
+static cb_status spectrum_run(cb_sim *s, FftPlan *p, double *h, double *kmean, double *kphys, double *delta,
+                              int cap, int *nbins)
 {
-    cb_status r = plan_for(s);
-    if (r != CB_OK) return r;
-    FftPlan *p = s->fft;
     const int D = s->p.D, N = s->p.nGrids, nB = p->nBins;
     const double L = s->p.L;
     const void *field = s->phi[s->iphi];
@@ -713,7 +796,6 @@ cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int c
     const double dk = 2.0 * PI / L;
     const double pref = (D == 1) ? 1.0 / PI : (D == 2) ? 1.0 / (2.0 * PI) : 1.0 / (2.0 * PI * PI);
     const double Pnorm = s->dxD * s->dxD / std::pow(L, D);
-    std::vector<double> h(3 * (size_t)nB);
     if (p->fused) {
         // rows r2c, (3D) axis 1, then axis 0 fused with the binning
         const int f32 = s->fp32 ? 1 : 0, ncp = p2_row_stride(N);
@@ -724,8 +806,6 @@ cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int c
                             p->fused_blocks, &nblk, s->st));
         k_bin_reduce<<<(nB + 63) / 64, 256, 0, s->st>>>(p->binpart, nblk, nB, p->bins);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(h.data(), p->bins, sizeof(double) * nB, hipMemcpyDeviceToHost, s->st));
-        std::copy(p->wsks.begin(), p->wsks.end(), h.begin() + nB);
     } else if (p->N1) {
         HIP_TRY(s->fp32 ? long_row<float>(p, (const float *)field, pref, dk, Pnorm, s->st)
                         : long_row<double>(p, (const double *)field, pref, dk, Pnorm, s->st));
@@ -745,7 +825,7 @@ cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int c
         k_bin_reduce<<<(3 * nB + 63) / 64, 256, 0, s->st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
         HIP_TRY(hipGetLastError());
     }
-    if (!p->fused) HIP_TRY(hipMemcpyAsync(h.data(), p->bins, sizeof(double) * 3 * nB, hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipMemcpyAsync(h, p->bins, sizeof(double) * 3 * nB, hipMemcpyDeviceToHost, s->st));
     cb::DevScalars sc;
     HIP_TRY(hipMemcpyAsync(&sc, s->dsc, sizeof sc, hipMemcpyDeviceToHost, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
@@ -764,6 +844,14 @@ cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int c
     }
     *nbins = nout;
     return CB_OK;
+}
+
+cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int cap, int *nbins)
+{
+    cb_status r = plan_for(s);
+    if (r != CB_OK) return r;
+    std::vector<double> h(3 * (size_t)s->fft->nBins);
+    return spectrum_run(s, s->fft, h.data(), kmean, kphys, delta, cap, nbins);
 }
 
 // Mode -3 field (GenerateThermalFluc, src/bubble.cpp:459-523), computed in

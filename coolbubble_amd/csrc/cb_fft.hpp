@@ -28,6 +28,33 @@ __device__ inline cpx<R> cmul(cpx<R> a, cpx<R> b)
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
+// Integer square roots of m < 2^52: floor(sqrt(m)) and ceil(sqrt(m)), 0 for
+// m <= 0.  A mode's bin is (int)sqrt((double)n2) with n2 an integer below
+// 2^52, which is exactly isqrt_floor(n2): the double square root is correctly
+// rounded and cannot reach k + 1 from below (k + 1)^2.  So the modes of bin b
+// on a line of squared offset C are those with b^2 <= C + a^2 < (b + 1)^2.
+__device__ inline long long isqrt_floor(long long m)
+{
+    if (m <= 0) return 0;
+    long long r = (long long)sqrt((double)m);
+    while (r * r > m) --r;
+    while ((r + 1) * (r + 1) <= m) ++r;
+    return r;
+}
+__device__ inline long long isqrt_ceil(long long m)
+{
+    if (m <= 0) return 0;
+    const long long r = isqrt_floor(m);
+    return r * r == m ? r : r + 1;
+}
+
+// Bins per thread when a block's threads own contiguous bin ranges (each
+// bin's LDS slot is written by its owner only: no atomics, a fixed order).
+__host__ __device__ constexpr int bins_per_thread(int nBins, int threads)
+{
+    return (nBins + threads - 1) / threads;
+}
+
 
 // Power-of-two lengths 8..2048: compile-time radix-8/4/2 transforms
 // (cb_fft_p2.hip).  tw = exp(-2 pi i e / N), e < N, in the transform's
@@ -45,7 +72,7 @@ hipError_t p2_cols(int fp32, void *H, long long outer, long long inner, int N, c
 // last forward pass of a D = 2 / 3 spectrum: transform axis 0 of [N][inner]
 // (inner = ncp, or N ncp, with ncp = p2_row_stride(N)) and bin the modes
 // instead of storing them; writes *nblk rows of nBins power partials to
-// binpart (which holds at least 64 rows).
+// binpart, one per block of the launch (at most cap_blocks rows).
 hipError_t p2_cols_bin(int fp32, const void *H, long long inner, int N, const void *tw, int D, int nBins, double dk,
                        double pref, double Pnorm, double *binpart, int cap_blocks, int *nblk, hipStream_t st);
 // largest block count p2_cols_bin uses

@@ -307,11 +307,10 @@ k_p2_rows_c2r(const cpx<R> *__restrict__ H, T *__restrict__ field, long long row
 // [N][ncp]; D = 3: the columns are (i1, j) of [N][N][ncp] (axis 0
 // transformed), rows padded to ncp = kNcp<N> >= nc; columns j >= nc are
 // padding and skipped.
-constexpr int kBinRows = 64;   // partial histograms, block b adds into row b % kBinRows
 struct BinArgs {
     int D, nBins;
     double dk, pref, Pnorm;
-    double *binpart;   // [kBinRows][nBins] power partials, zeroed before the launch
+    double *binpart;   // [grid][nBins] power partials, one row per block
 };
 
 // Columns per block: RND rounds of kSeq<N> sequences (about 32 KiB of data
@@ -368,70 +367,63 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
                 if (cc < ncol) base[(long long)e * inner + cc] = buf[cc * BS + pidx(e)];
             }
         } else {
-            // runs of up to 8 adjacent columns per thread: the terms of a run
-            // are independent (computed first), then along j the bin index
-            // never decreases, so the run accumulates in a register and adds
-            // to the LDS histogram when its bin changes.  kc^D is a product
-            // of D factors (within 2 ulp of the reference's pow(kc, D)).
-            constexpr int CL = CB < 8 ? CB : 8;
-            for (int q = threadIdx.x; q < N * (CB / CL); q += 256) {
-                const int e = q & (N - 1);
-                const int cc0 = (q / N) * CL;
-                const int se = (e <= N / 2) ? e : e - N;
-                const int c = (int)c0 + cc0;
-                int j = ba.D == 3 ? c % kNcp<N> : c;
-                int i1 = ba.D == 3 ? c / kNcp<N> : 0;
-                double pv[CL];
-                int bv[CL];
-#pragma unroll
-                for (int u = 0; u < CL; ++u) {
-                    const int s1 = (ba.D == 3) ? ((i1 <= N / 2) ? i1 : i1 - N) : 0;
-                    const double n2rows = (double)se * (double)se + (double)s1 * (double)s1;
-                    const double n = sqrt(n2rows + (double)j * j);
-                    const int bin = (int)n;
-                    const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
-                    const double kc = n * ba.dk;
-                    const double kd = (ba.D == 3) ? kc * kc * kc : kc * kc;
-                    const cpx<R> f = buf[(cc0 + u) * BS + pidx(e)];
-                    const double fx = (double)f.x, fy = (double)f.y;
-                    pv[u] = w * ba.pref * kd * ba.Pnorm * (fx * fx + fy * fy);
-                    bv[u] = (cc0 + u < ncol && j < nc && bin < ba.nBins) ? bin : -1;
-                    if (++j == kNcp<N>) {
-                        j = 0;
-                        ++i1;
+            // Binning with no atomics, in a fixed order: thread t owns the
+            // bins [b0, b1) and only it touches their LDS slots.  For each
+            // column (j, and i1 in 3D: squared offset C = s1^2 + j^2) and each
+            // half of axis 0 (s_e = e for e <= N/2, e - N above), the bin of
+            // a mode never decreases with |s_e|, so the thread starts at the
+            // first |s_e| of bin b0 (an integer square root) and walks up to
+            // bin b1, summing each bin's run in a register.  kc^D is a
+            // product of D factors (within 2 ulp of the reference's
+            // pow(kc, D)).
+            const int bpt = bins_per_thread(ba.nBins, 256);
+            const int b0 = threadIdx.x * bpt;
+            const int b1 = b0 + bpt < ba.nBins ? b0 + bpt : ba.nBins;
+            for (int cc = 0; cc < ncol && b0 < b1; ++cc) {
+                const int c = (int)c0 + cc;
+                const int j = ba.D == 3 ? c % kNcp<N> : c;
+                if (j >= nc) continue;
+                const int i1 = ba.D == 3 ? c / kNcp<N> : 0;
+                const int s1 = (i1 <= N / 2) ? i1 : i1 - N;
+                const long long C = (long long)s1 * s1 + (long long)j * j;
+                const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
+                for (int half = 0; half < 2; ++half) {
+                    const long long amin = half, amax = half ? N / 2 - 1 : N / 2;
+                    if ((long long)b1 * b1 <= C + amin * amin) continue;     // every bin >= b1
+                    if ((long long)b0 * b0 > C + amax * amax) continue;      // every bin < b0
+                    long long a = isqrt_ceil((long long)b0 * b0 - C);
+                    if (a < amin) a = amin;
+                    int cur = -1;
+                    double acc = 0.0;
+                    for (; a <= amax; ++a) {
+                        const double n = sqrt((double)(C + a * a));
+                        const int bin = (int)n;
+                        if (bin >= b1) break;
+                        const int e = half ? N - (int)a : (int)a;
+                        const double kc = n * ba.dk;
+                        const double kd = (ba.D == 3) ? kc * kc * kc : kc * kc;
+                        const cpx<R> f = buf[cc * BS + pidx(e)];
+                        const double fx = (double)f.x, fy = (double)f.y;
+                        const double v = w * ba.pref * kd * ba.Pnorm * (fx * fx + fy * fy);
+                        if (bin == cur) {
+                            acc += v;
+                        } else {
+                            if (cur >= 0) hist[cur] += acc;
+                            cur = bin;
+                            acc = v;
+                        }
                     }
+                    if (cur >= 0) hist[cur] += acc;
                 }
-                int cur = -1;
-                double acc = 0.0;
-#pragma unroll
-                for (int u = 0; u < CL; ++u) {
-                    if (bv[u] < 0) continue;
-                    if (bv[u] == cur) {
-                        acc += pv[u];
-                    } else {
-                        if (cur >= 0) atomicAdd(&hist[cur], acc);
-                        cur = bv[u];
-                        acc = pv[u];
-                    }
-                }
-                if (cur >= 0) atomicAdd(&hist[cur], acc);
             }
         }
     }
     if constexpr (BIN) {
+        // one row of partials per block, summed in block order afterwards
         __syncthreads();
-        double *dst = ba.binpart + (long long)(blockIdx.x % kBinRows) * ba.nBins;
-        for (int b = threadIdx.x; b < ba.nBins; b += 256) {
-            const double v = hist[b];
-            if (v != 0.0) atomicAdd(&dst[b], v);
-        }
+        double *dst = ba.binpart + (long long)blockIdx.x * ba.nBins;
+        for (int b = threadIdx.x; b < ba.nBins; b += 256) dst[b] = hist[b];
     }
-}
-
-__global__ void __launch_bounds__(256) k_zero(double *p, int n)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = 0.0;
 }
 
 int current_cus()
@@ -523,9 +515,7 @@ hipError_t cols_bin_n(const void *H, long long inner, const void *tw, BinArgs ba
     if (e != hipSuccess) return e;
     const long long items = (inner + CB - 1) / CB;
     const unsigned grid = persistent_grid(k_p2_cols<R, N, true>, shm, items, cap_blocks);
-    const int nz = kBinRows * ba.nBins;
-    k_zero<<<(nz + 255) / 256, 256, 0, st>>>(ba.binpart, nz);
-    *nblk = kBinRows;
+    *nblk = (int)grid;
     k_p2_cols<R, N, true><<<grid, 256, shm, st>>>((cpx<R> *)H, 1, inner, (const cpx<R> *)tw, -1, ba);
     return hipGetLastError();
 }

@@ -1,7 +1,12 @@
 """Write the full-length oracle fixtures of the two large measured workloads.
 
-    python tests/golden/make_fullsize.py c2 [threads]   # ~2 h on 8 cores
-    python tests/golden/make_fullsize.py c3 [threads]   # ~1.5 h on 8 cores
+    python tests/golden/make_fullsize.py c2 [threads] [checkpoint dir]   # ~2 h on 8 cores
+    python tests/golden/make_fullsize.py c3 [threads] [checkpoint dir]   # ~1.5 h on 8 cores
+
+With a checkpoint directory the run saves its state there every 200 steps
+(phi, pi, the scalars and the outputs so far) and resumes from it when
+started again; the oracle's step is a function of that state alone, so a
+resumed run writes the same numbers.
 
 Each runs the oracle (oracle/cb_oracle.c, the CPU restatement of the
 reference's Bubble ctor and oneStep, src/bubble.cpp:156-316,849-876) for the
@@ -67,21 +72,47 @@ def path_of(name):
     return os.path.join(HERE, "%s_d%d_n%d_nt%d.npz" % (name, kw["D"], kw["nGrids"], nt))
 
 
-def main(name, nthreads=0):
+def _save(ckpt, o, done, rows, row_steps, spec_steps, spec_k, spec_d, sums):
+    tmp = os.path.join(ckpt, "state.tmp.npz")
+    np.savez(tmp, done=done, phi=o.phi(), pi=o.pi(), scalars=o.scalars(), rows=np.array(rows),
+             row_steps=np.array(row_steps), spec_steps=np.array(spec_steps),
+             spec_k=np.concatenate(spec_k),
+             spec_d=np.concatenate(spec_d), spec_len=np.array([len(k) for k in spec_k]),
+             sums=np.array(sums))
+    os.replace(tmp, os.path.join(ckpt, "state.npz"))
+
+
+def main(name, nthreads=0, ckpt=None):
     kw, nt = CONFIGS[name]
     snaps = snapshot_steps(nt, kw["dt"], 20)
     events = sorted(set(range(40, nt + 1, 40)) | set(snaps))
     t0 = time.time()
-    o = oracle.Sim(nthreads=nthreads, **kw)
-    print("%s: created in %.0f s" % (name, time.time() - t0), flush=True)
-    rows, row_steps = [o.energy_row()], [0]
-    spec_steps, spec_k, spec_d, sums = [0], [], [], []
-    k, _, d = o.spectrum()
-    spec_k.append(k)
-    spec_d.append(d)
-    sums.append(checksums(o.phi()) + checksums(o.pi()))
-    done = 0
+    state = os.path.join(ckpt, "state.npz") if ckpt else None
+    if state and os.path.exists(state):
+        z = np.load(state)
+        o = oracle.Sim(nthreads=nthreads, blank=True, **kw)
+        o.phi()[...] = z["phi"]
+        o.pi()[...] = z["pi"]
+        o.set_scalars(z["scalars"])
+        done = int(z["done"])
+        rows, row_steps = list(z["rows"]), list(z["row_steps"])
+        spec_steps, sums = list(z["spec_steps"]), [list(x) for x in z["sums"]]
+        cut = np.cumsum(z["spec_len"])[:-1]
+        spec_k, spec_d = np.split(z["spec_k"], cut), np.split(z["spec_d"], cut)
+        print("%s: resumed at step %d" % (name, done), flush=True)
+    else:
+        o = oracle.Sim(nthreads=nthreads, **kw)
+        print("%s: created in %.0f s" % (name, time.time() - t0), flush=True)
+        rows, row_steps = [o.energy_row()], [0]
+        spec_steps, spec_k, spec_d, sums = [0], [], [], []
+        k, _, d = o.spectrum()
+        spec_k.append(k)
+        spec_d.append(d)
+        sums.append(checksums(o.phi()) + checksums(o.pi()))
+        done = 0
     for ev in events:
+        if ev <= done:
+            continue
         o.step(ev - done)
         done = ev
         if ev % 40 == 0:
@@ -94,9 +125,10 @@ def main(name, nthreads=0):
             spec_d.append(d)
             sums.append(checksums(o.phi()) + checksums(o.pi()))
             print("%s: step %d, %.0f s" % (name, ev, time.time() - t0), flush=True)
+        if ckpt and ev % 200 == 0 and ev < nt:
+            _save(ckpt, o, done, rows, row_steps, spec_steps, spec_k, spec_d, sums)
     sites = fixed_sites(o.n)
-    meta = dict(params=kw, nt=nt, snapshots=20, threads=nthreads, generator="tests/golden/make_fullsize.py",
-                seconds=round(time.time() - t0))
+    meta = dict(params=kw, nt=nt, snapshots=20, generator="tests/golden/make_fullsize.py")
     np.savez_compressed(path_of(name), meta=np.array(json.dumps(meta)), row_steps=np.array(row_steps),
                         rows=np.array(rows), spec_steps=np.array(spec_steps), spec_k=np.array(spec_k),
                         spec_delta=np.array(spec_d), checksums=np.array(sums), sites=sites,
@@ -105,4 +137,4 @@ def main(name, nthreads=0):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1], int(sys.argv[2]) if len(sys.argv) > 2 else 0)
+    main(sys.argv[1], int(sys.argv[2]) if len(sys.argv) > 2 else 0, sys.argv[3] if len(sys.argv) > 3 else None)

@@ -198,6 +198,8 @@ def test_spectrum_matches_oracle(name):
     # bins: relative 1e-12, with an absolute floor at the FFT's rounding level
     floor = 1e-13 * np.max(do)
     assert np.all(np.abs(dg - do) <= TOL * np.abs(do) + floor), np.max(np.abs(dg - do) / (np.abs(do) + floor))
+    _, _, dg2 = g.spectrum()
+    np.testing.assert_array_equal(dg2, dg)
 
 
 def test_fp32_energies_within_1e5_of_fp64():
@@ -265,9 +267,12 @@ def test_power_of_two_sizes_thermal_fields_and_spectra(D, N):
     np.testing.assert_allclose(k32, ko, rtol=1e-13)
     big = do > 1e-6 * np.max(do)
     np.testing.assert_allclose(d32[big], do[big], rtol=2e-3)
-    # a second spectrum of the same state gives the same bins (to rounding)
+    # a second spectrum of the same state gives the same bins bit for bit
+    # (fixed summation order, no atomics)
     _, _, dg2 = g.spectrum()
-    np.testing.assert_allclose(dg2, dg, rtol=1e-13, atol=1e-13 * np.max(dg))
+    np.testing.assert_array_equal(dg2, dg)
+    _, _, d32b = g32.spectrum()
+    np.testing.assert_array_equal(d32b, d32)
 
 
 @pytest.mark.parametrize("N", [8192, 65536])

@@ -71,10 +71,9 @@ def test_results_identical_alone_and_in_the_mix(precision):
         assert r.n_spectra == len(spectra)
         for w, (k, _, d) in enumerate(spectra):
             ks, ds = sv.spectrum(r.id, w)
-            # spectrum bins are summed with LDS atomics (order not fixed): the
-            # stated tolerance is 1e-12 relative; energy rows are bit-identical
-            np.testing.assert_allclose(ks, k, rtol=1e-13)
-            np.testing.assert_allclose(ds, d, rtol=1e-12, atol=1e-14 * max(np.max(d), 1e-300))
+            # bins are summed in a fixed order (no atomics): bit-identical too
+            np.testing.assert_array_equal(ks, k)
+            np.testing.assert_array_equal(ds, d)
         sv.release(r.id)
         alone.close()
     sv.close()
