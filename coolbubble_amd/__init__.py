@@ -89,6 +89,7 @@ def lib():
         L.cb_step.argtypes = [vp, ctypes.c_int]
         L.cb_energies_row.argtypes = [vp, ctypes.POINTER(Energies)]
         L.cb_power_spectrum.argtypes = [vp, dp, dp, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.cb_power_spectrum_noalloc.argtypes = [vp, dp, dp, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.cb_get_fields.argtypes = [vp, dp, dp]
         L.cb_get_state.argtypes = [vp, ctypes.POINTER(State)]
         L.cb_set_state.argtypes = [vp, ctypes.POINTER(State), dp, dp]
@@ -150,7 +151,8 @@ def lib():
         L.cb_status_string.restype = ctypes.c_char_p
         L.cb_last_error.restype = ctypes.c_char_p
         for name in ("cb_default_params", "cb_check_stability", "cb_create", "cb_step",
-                     "cb_energies_row", "cb_power_spectrum", "cb_get_fields", "cb_get_state",
+                     "cb_energies_row", "cb_power_spectrum", "cb_power_spectrum_noalloc", "cb_get_fields",
+                     "cb_get_state",
                      "cb_set_state", "cb_get_xns", "cb_stream", "cb_destroy",
                      "cb_kernel_variants", "cb_select_kernel", "cb_autotune"):
             getattr(L, name).restype = ctypes.c_int

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -852,6 +853,77 @@ cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int c
     if (r != CB_OK) return r;
     std::vector<double> h(3 * (size_t)s->fft->nBins);
     return spectrum_run(s, s->fft, h.data(), kmean, kphys, delta, cap, nbins);
+}
+
+// ---------------------------------------------------------------------------
+// The spectrum with no allocation (cb_power_spectrum_noalloc).  The plan's
+// device buffers (half-complex array, twiddles, bin partials and bins) are
+// carved from `scratch`, a static device array of kNoAllocScratchBytes
+// that each GPU holds for the life of the process; the host stages the
+// twiddles and reads the bins back through static host buffers.  The device
+// address of each GPU's copy comes from hipGetSymbolAddress on that GPU's
+// first call (the array's name is only a host-side handle).  The scratch
+// keeps the last call's contents; every buffer that must start at zero is
+// cleared by a kernel on each call.  Two calls on one GPU whose work
+// overlapped would race on it, so calls in one process run one at a time
+// under a lock held until the call's stream has drained (also on the error
+// paths).
+// ---------------------------------------------------------------------------
+static constexpr size_t kNoAllocScratchBytes = CB_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+constexpr int kMaxStageTw = 65536;          // longest twiddle table (1D N = 2^16)
+constexpr int kMaxStageBins = 3 * 32769;    // 3 x bins of 1D N = 2^16
+std::mutex g_noalloc_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+cpx<double> g_stage_tw[kMaxStageTw];
+double g_stage_bins[kMaxStageBins];
+} // namespace
+
+cb_status spectrum_noalloc(cb_sim *s, double *kmean, double *kphys, double *delta, int cap, int *nbins)
+{
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    const int dev = s->p.device;
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device ordinal out of range for the no-allocation scratch");
+        return CB_INVALID_ARG;
+    }
+    HIP_TRY(hipSetDevice(dev));
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) {
+            set_error("no device address for the no-allocation scratch");
+            return CB_HIP_ERROR;
+        }
+        g_scratch_dev[dev] = ptr;
+    }
+    FftPlan p;
+    p.owned = false;
+    cb_status r = plan_shape(s, &p);
+    if (r != CB_OK) return r;
+    if (p.N > kMaxStageTw || 3 * p.nBins > kMaxStageBins) {
+        set_error("lattice too large for the no-allocation spectrum");
+        return CB_UNSUPPORTED;
+    }
+    Arena arena{(char *)g_scratch_dev[dev], kNoAllocScratchBytes, 0};
+    hipError_t e = plan_buffers(&p, &arena, g_stage_tw, s->p.L, s->st);
+    if (e == hipErrorOutOfMemory) {   // raised before any launch
+        set_error("lattice too large for the no-allocation spectrum: its work arrays exceed the " +
+                  std::to_string(kNoAllocScratchBytes >> 20) + " MiB scratch");
+        return CB_UNSUPPORTED;
+    }
+    if (e == hipSuccess) {
+        r = spectrum_run(s, &p, g_stage_bins, kmean, kphys, delta, cap, nbins);
+        if (r == CB_OK) return r;
+    } else {
+        set_error(std::string("no-allocation spectrum: ") + hipGetErrorString(e));
+        r = CB_HIP_ERROR;
+    }
+    (void)hipStreamSynchronize(s->st);   // nothing of this call may still run on the scratch
+    return r;
 }
 
 // Mode -3 field (GenerateThermalFluc, src/bubble.cpp:459-523), computed in

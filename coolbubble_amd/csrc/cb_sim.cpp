@@ -542,6 +542,14 @@ cb_status cb_power_spectrum(cb_sim *s, double *kmean, double *kphys, double *del
     return cb::spectrum(s, kmean, kphys, delta, cap, nbins);
 }
 
+cb_status cb_power_spectrum_noalloc(cb_sim *s, double *kmean, double *kphys, double *delta, int cap, int *nbins)
+{
+    if (!s || !nbins) { set_error("invalid argument"); return CB_INVALID_ARG; }
+    cb_status r = materialise(s);
+    if (r != CB_OK) return r;
+    return cb::spectrum_noalloc(s, kmean, kphys, delta, cap, nbins);
+}
+
 cb_status cb_kernel_variants(cb_sim *s, int *count, int *selected)
 {
     if (!s) { set_error("invalid handle"); return CB_INVALID_ARG; }

@@ -97,6 +97,8 @@ cb_status select_variant(cb_sim *s, int variant);
 // the variant cb_create selects (-1 = the general kernel)
 int default_variant(const cb_sim *s);
 cb_status spectrum(cb_sim *s, double *kmean, double *kphys, double *delta, int cap, int *nbins);
+// the same spectrum with the work arrays in the static no-allocation scratch
+cb_status spectrum_noalloc(cb_sim *s, double *kmean, double *kphys, double *delta, int cap, int *nbins);
 // thermal initial condition of one field (cb_fft.hip)
 cb_status thermal_field(cb_sim *s, void *field, double T, double m, int velocity);
 void fft_destroy(FftPlan *p);

@@ -107,6 +107,22 @@ cb_status cb_energies_row(cb_sim *s, cb_energies *out);
 cb_status cb_power_spectrum(cb_sim *s, double *kmean, double *kphys, double *delta,
                             int cap, int *nbins);
 
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): cb_power_spectrum with no allocation.  cb_power_spectrum
+ * allocates its work arrays (the half-complex array, as large as the
+ * lattice, twiddles, bin partials) on its first call on a handle; this
+ * variant carves them from a static device array of
+ * CB_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends, and
+ * calls no hipMalloc, hipFree or other allocating routine.  Lattices whose
+ * work arrays exceed it (e.g. 3D above about 192^3 in FP64, 2D above 2048^2)
+ * return CB_UNSUPPORTED.  Results agree with cb_power_spectrum's.  Calls run
+ * one at a time in a process (a lock held until the call's stream drains);
+ * calls on one GPU that overlapped would race on the shared scratch. */
+#define CB_NOALLOC_SCRATCH_BYTES (64u << 20)
+cb_status cb_power_spectrum_noalloc(cb_sim *s, double *kmean, double *kphys, double *delta,
+                                    int cap, int *nbins);
+
 /* The fields a snapshot writes (OutField, src/bubble.cpp:887-917): phi at
  * the current whole step and pi half a step behind, as double (FP32 runs are
  * widened).  Either pointer may be NULL.  Each holds nGrids^D values in
