@@ -27,6 +27,7 @@ import time
 import numpy as np
 
 HBM_PEAK = 8.0e12  # MI355X datasheet, bytes/s
+CPU_SHARE = 16     # host cores that go with one GPU
 
 CONFIG = dict(D=3, nGrids=512, L=64.0, c=4.0, readxns=1, nBubbles=32, expansion=0)
 CONFIG["dt"] = 0.2 * CONFIG["L"] / CONFIG["nGrids"]
@@ -316,14 +317,21 @@ def run_driver(cb, log):
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it (model name, family, model,
+    stepping) and the logical CPUs the machine shows."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" in line:
+                    k, v = (x.strip() for x in line.split(":", 1))
+                    if k in ("model name", "cpu family", "model", "stepping", "vendor_id") and k not in info:
+                        info[k] = v
     except OSError:
         pass
-    return "unknown"
+    return "%s (vendor %s, family %s, model %s, stepping %s; %d logical CPUs on the host)" % (
+        info.get("model name", "unknown"), info.get("vendor_id", "?"), info.get("cpu family", "?"),
+        info.get("model", "?"), info.get("stepping", "?"), os.cpu_count() or 0)
 
 
 def cpu_baseline(sim64, nsteps, log):
@@ -331,7 +339,9 @@ def cpu_baseline(sim64, nsteps, log):
     every step, like the reference) stepping the same lattice from the GPU's
     seeded state, on the host cores."""
     import oracle
-    threads = min(16, os.cpu_count() or 1)
+    # the host share of one GPU on the MI355X boxes is 16 cores (the machine
+    # shows more, shared with the other GPUs' jobs)
+    threads = min(CPU_SHARE, os.cpu_count() or 1)
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
     phi, pi = sim64.fields()
@@ -412,6 +422,9 @@ def main():
                           "achieved_GBps": 4 * w * sites / (ms * 1e-3 / K) / 1e9}), flush=True)
         return 0
     sim32, ms32, row32 = run_precision(cb, hip, "fp32", K, W, log, args.variant)
+    # FP32 state after the same W + K steps, for its errors against FP64
+    phi32, _ = sim32.fields()
+    _, _, d32 = sim32.spectrum()
     sel32 = sim32.kernel_variants()[1]
     info32 = sim32.variant_info(sel32) if sel32 >= 0 else None
     fuse32 = info32.get("fuse", 1) if info32 else 1
@@ -422,6 +435,12 @@ def main():
     slab = slab_share(sim32, info32, log)   # last: only the slab advances
     sim32.close()
     sim64, ms64, row64 = run_precision(cb, hip, "fp64", K, W, log, args.variant)
+    phi64, _ = sim64.fields()
+    _, _, d64 = sim64.spectrum()
+    field_err32 = float(np.max(np.abs(phi32 - phi64)) / np.max(np.abs(phi64)))
+    big = d64 > 1e-6 * np.max(d64)
+    spec_err32 = float(np.max(np.abs(d32[big] - d64[big]) / d64[big])) if len(d32) == len(d64) else None
+    del phi32, phi64
     spec64 = spectrum_time(sim64, hip)
     log("fp64 spectrum: %.3f ms" % spec64)
 
@@ -441,15 +460,18 @@ def main():
     except (OSError, KeyError, ValueError, TypeError):
         pass
     su64 = sites * K / (ms64 * 1e-3)
-    t_step32 = ms32 * 1e-3 / K
-    bytes32 = 16.0 * sites          # read phi, read pi, write pi, write phi (FP32)
+    bytes32 = 16.0 * sites          # compulsory bytes of one pass: read phi, pi; write pi, phi (FP32)
     bytes64 = 32.0 * sites
-    ach32_steps = bytes32 / t_step32                             # over whole steps
-    ach32 = bytes32 * fuse32 / (launch_us32 * 1e-6)              # the dominant kernel, per launch
-    ach64 = bytes64 / (ms64 * 1e-3 / K)
+    # the dominant kernel moves the lattice's compulsory bytes once per
+    # launch, whatever the steps it takes in that pass
+    ach32 = bytes32 / (launch_us32 * 1e-6)
+    one_pass_su32 = HBM_PEAK / 16.0      # site-updates/s of one HBM pass per step at peak (FP32)
+    one_pass_su64 = HBM_PEAK / 32.0
     # energies after the same W+K steps from the same seeded lattice
     err = {name: abs(row32[i] - row64[i]) / abs(row64[i])
            for i, name in ((2, "Etot"), (3, "Ekin"), (4, "Egrad"), (5, "Epot"))}
+    err["field_max_abs_over_max"] = field_err32
+    err["spectrum_bins_max_rel"] = spec_err32
     fft_bytes = lambda w: (2 * 3 - 1) * w * sites   # (2d-1) w N^d per spectrum
     cpu = None
     if not args.no_cpu:
@@ -482,29 +504,29 @@ def main():
                           "frac": ach32 / HBM_PEAK, "traffic": traffic32,
                           "steps_per_launch": steps_per_launch32,
                           "kernel_avg_launch_us_live": launch_us32,
-                          "achieved_over_whole_steps": ach32_steps / 1e9,
-                          "frac_over_whole_steps": ach32_steps / HBM_PEAK,
-                          "note": "achieved = algorithmic bytes per launch of the dominant kernel (16 B per "
-                                  "site-update: read phi, pi; write pi, phi; x sites x steps per launch) / its "
-                                  "average plain launch, timed live with HIP events on the simulation's stream "
-                                  "(kernel_avg_launch_us_live); achieved_over_whole_steps = 16 B x site-updates/s "
-                                  "over the timed steps (energy launches and finalisation included). A two- or "
-                                  "three-step variant reads and writes each byte once per two or three "
-                                  "site-updates, so frac can exceed 1 while HBM delivers less than its peak. "
-                                  "traffic = measured HBM bytes per launch of the selected variant "
-                                  "(profiles/hbm_traffic.json, rocprofv3 FETCH_SIZE + WRITE_SIZE); "
-                                  "delivered_GBps_profiled = traffic / the kernel's profiled average launch"},
+                          "compulsory_bytes_per_launch": bytes32,
+                          "one_pass_roofline_site_updates_per_s": one_pass_su32,
+                          "site_updates_over_one_pass_roofline": su32 / one_pass_su32,
+                          "note": "achieved = compulsory HBM bytes of one launch of the dominant kernel (16 B per "
+                                  "site: read phi, pi; write pi, phi; once per launch, whatever the steps it "
+                                  "takes) / its average plain launch, timed live with HIP events on the "
+                                  "simulation's stream (kernel_avg_launch_us_live); frac = achieved / 8 TB/s. "
+                                  "The kernel takes steps_per_launch leapfrog steps per pass over HBM, so the "
+                                  "site-update rate can exceed the one-pass roofline (8 TB/s / 16 B): "
+                                  "site_updates_over_one_pass_roofline. traffic = measured HBM bytes per launch "
+                                  "(profiles/hbm_traffic.json, rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy "
+                                  "kernel); delivered_GBps_profiled = traffic / the profiled average launch"},
                          **prof32),
         "cpu_baseline": cpu,
-        "fp64": {"value": su64, "ms_per_step": ms64 / K, "achieved_GBps": ach64 / 1e9,
-                 "frac": ach64 / HBM_PEAK},
+        "fp64": {"value": su64, "ms_per_step": ms64 / K,
+                 "site_updates_over_one_pass_roofline": su64 / one_pass_su64},
         "fp32_over_fp64": su32 / su64,
         "expanding": expd,
         "slab_share_8_gpus": slab,
         "ensemble": ens,
         "serving": srv,
         "measured_copy_GBps": copy_gbps,
-        "frac_of_measured_copy": ach32_steps / (copy_gbps * 1e9),
+        "frac_of_measured_copy": ach32 / (copy_gbps * 1e9),
         "fp32_energy_rel_err_vs_fp64": err,
         "spectrum": {"fp32_ms": spec32, "fp64_ms": spec64,
                      "fp32_GBps_algorithmic": fft_bytes(4) / (spec32 * 1e-3) / 1e9,

@@ -86,12 +86,27 @@ int pick_variant_multi(const cb_sim *s, int max_fuse)
     return -1;
 }
 
-// Two steps per launch: with three, a serving mix of 200 Poisson-arriving
-// requests of every size ran at p50 2.6 s against 1.8 s (three for the
-// large requests only: 2.9-3.0 s); a request's short runs to its next
-// event leave steps over for the one-step companion.
+// Static lattices of 2^23 sites or more (3D N >= 208), and those whose rows
+// no batched tile holds, step on their own kernels, with the variant a lone
+// run starts on (the three-step tiles of
+// 512-1024 threads with the plane prefetch): the batched launch's
+// 256-thread tiles run them 1.3x slower or more (8 x 3D N = 256: 0.234 ms
+// per step batched, 0.175 ms as lone runs).  The rest take two steps per
+// batched launch: with three, a serving mix of 200 Poisson-arriving
+// requests of every size ran at p50 2.6 s against 1.8 s; a request's short
+// runs to its next event leave steps over for the one-step companion.
+bool server_solo(const cb_sim *s)
+{
+    if (s->lagged || s->p.D < 2) return false;
+    if (s->nsites >= (1LL << 23)) return true;
+    // rows too wide for a batched tile (e.g. FP64 2D N = 2048): a lone
+    // run's tiles rather than the general kernel
+    return pick_variant_multi(s, 2) < 0 && batch_variant(s) < 0 && default_variant(s) >= 0;
+}
+
 int server_variant(const cb_sim *s)
 {
+    if (server_solo(s)) return default_variant(s);
     const int vm = pick_variant_multi(s, 2);
     return vm >= 0 ? vm : batch_variant(s);
 }

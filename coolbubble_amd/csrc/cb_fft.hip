@@ -226,68 +226,176 @@ k_cols(cpx<R> *H, long long inner, int CB, Stages st, const cpx<R> *__restrict__
 }
 
 // Radial binning of the half-complex spectrum, ComputePowerSpectrum
-// (src/bubble.cpp:668-709): per block one row of (power, weight, weighted
-// k) partials over a contiguous run of modes.  Thread t owns the bins
-// [b0, b1): along a row the bin never decreases with j, so the thread starts
-// at the row's first j of bin b0 (an integer square root) and walks up to
-// bin b1; only the owner writes a bin's LDS slots, so the sums have a fixed
-// order (no atomics).
+// (src/bubble.cpp:668-709), in a fixed order with no atomics.  A block
+// takes a contiguous run of rows, RI rows at a time; each row is cut into K
+// chunks of E consecutive modes, one chunk per thread (K = ceil(nc / E),
+// RI K <= 256).  Along a row the bin never decreases with j, so the modes
+// of one bin form one run (a segment).  A thread sums the runs in its
+// chunk; a run that crosses into the next chunks is completed by the
+// thread where it starts, from those chunks' first partial sums, in order.
+// Each (row, bin) gets one total, written by one thread into the row's
+// slot; the RI slots of each bin are added in a fixed order at the end, one
+// row of (power, weight, weighted k) partials per block.
 // GEOM: weights and weighted k only (they do not depend on the field; H is
 // not read).
 template <typename R, bool GEOM = false>
 __global__ void __launch_bounds__(256)
 k_bin(const cpx<R> *__restrict__ H, long long rows, int N, int D, int nBins, double dk, double pref,
-      double Pnorm, double *__restrict__ binpart)
+      double Pnorm, int E, int K, int RI, double *__restrict__ binpart)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    double *hp = reinterpret_cast<double *>(smem_raw);
-    double *hw = hp + nBins;
-    double *hk = hw + nBins;
-    const int bpt = bins_per_thread(nBins, blockDim.x);
-    const int b0 = threadIdx.x * bpt;
-    const int b1 = b0 + bpt < nBins ? b0 + bpt : nBins;
-    for (int b = b0; b < b1; ++b) hp[b] = hw[b] = hk[b] = 0.0;
+    double *hp = reinterpret_cast<double *>(smem_raw);   // [RI][nBins] power
+    double *hw = hp + (size_t)RI * nBins;                  // weights
+    double *hk = hw + (size_t)RI * nBins;                  // weighted k
+    double *rfs = hk + (size_t)RI * nBins;                 // [3][256] first-run sums
+    int *rfk = reinterpret_cast<int *>(rfs + 3 * 256);     // first-run key (-1: none at the chunk start)
+    int *rlk = rfk + 256;                                  // last-run key (-1: none at the chunk end)
+    int *rone = rlk + 256;                                 // one run over the whole chunk
+    for (int i = threadIdx.x; i < 3 * RI * nBins; i += blockDim.x) hp[i] = 0.0;
     const int nc = N / 2 + 1;
-    const long long total = rows * nc;
-    const long long per = (total + gridDim.x - 1) / gridDim.x;
-    const long long m0 = per * blockIdx.x;
-    const long long m1 = (m0 + per < total) ? m0 + per : total;
-    if (b0 < b1 && m0 < m1) {
-        for (long long r = m0 / nc; r * nc < m1; ++r) {
-            long long S = 0;   // n2rows, an integer
-            if (D == 2) {
-                const long long s = (r <= N / 2) ? r : r - N;
-                S = s * s;
-            } else if (D == 3) {
-                const long long a = r / N, c = r % N;
-                const long long s1 = (a <= N / 2) ? a : a - N, s2 = (c <= N / 2) ? c : c - N;
-                S = s1 * s1 + s2 * s2;
-            }
-            const long long jlo = (r * nc < m0) ? m0 - r * nc : 0;
-            const long long jhi = ((r + 1) * nc > m1) ? m1 - 1 - r * nc : nc - 1;
-            if ((long long)b1 * b1 <= S + jlo * jlo) continue;     // every bin >= b1
-            if ((long long)b0 * b0 > S + jhi * jhi) continue;      // every bin < b0
-            long long j = isqrt_ceil((long long)b0 * b0 - S);
-            if (j < jlo) j = jlo;
-            for (; j <= jhi; ++j) {
+    const long long per = (rows + gridDim.x - 1) / gridDim.x;
+    const long long r0 = per * blockIdx.x;
+    const long long r1 = (r0 + per < rows) ? r0 + per : rows;
+    const int t = threadIdx.x, slot = t / K, chunk = t - slot * K;
+    for (long long rb = r0; rb < r1; rb += RI) {
+        __syncthreads();   // the previous rows' records are read
+        const long long r = rb + slot;
+        const bool rowok = slot < RI && r < r1;
+        long long S = 0;   // n2rows, an integer
+        if (rowok && D == 2) {
+            const long long s = (r <= N / 2) ? r : r - N;
+            S = s * s;
+        } else if (rowok && D == 3) {
+            const long long a = r / N, c = r % N;
+            const long long s1 = (a <= N / 2) ? a : a - N, s2 = (c <= N / 2) ? c : c - N;
+            S = s1 * s1 + s2 * s2;
+        }
+        double *slot_p = hp + (size_t)slot * nBins, *slot_w = hw + (size_t)slot * nBins, *slot_k = hk + (size_t)slot * nBins;
+        // runs of the chunk: A (first) and Z (last) kept, the others written
+        int nseg = 0, ak = -1, zk = -1;
+        bool a0 = false, zE = false;
+        double ap = 0, aw = 0, akk = 0, zp = 0, zw = 0, zkk = 0;
+        const int j0 = chunk * E;
+        for (int u = 0; u < E; ++u) {
+            const long long j = j0 + u;
+            int key = -1;
+            double p = 0.0, w = 0.0, wk = 0.0;
+            if (rowok && j < nc) {
                 const double n = sqrt((double)S + (double)j * j);
                 const int bin = (int)n;
-                if (bin >= b1) break;
-                const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
-                const double kc = n * dk;
-                if (!GEOM) {
-                    const cpx<R> f = H[r * nc + j];
-                    const double fx = (double)f.x, fy = (double)f.y;
-                    hp[bin] += w * pref * ipow_rn(kc, D) * Pnorm * (fx * fx + fy * fy);
+                if (bin < nBins) {
+                    key = bin;
+                    w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
+                    const double kc = n * dk;
+                    if (!GEOM) {
+                        const cpx<R> f = H[r * nc + j];
+                        const double fx = (double)f.x, fy = (double)f.y;
+                        p = w * pref * ipow_rn(kc, D) * Pnorm * (fx * fx + fy * fy);
+                    }
+                    wk = w * kc;
                 }
-                hw[bin] += w;
-                hk[bin] += w * kc;
             }
+            if (key >= 0 && key == zk && nseg > 0 && zE) {          // the current run goes on
+                zp += p;
+                zw += w;
+                zkk += wk;
+                continue;
+            }
+            // the current run (Z) closes: write it unless it is the first run from u = 0
+            if (nseg > 0 && zE && !(nseg == 1 && a0)) {
+                slot_p[zk] += zp;
+                slot_w[zk] += zw;
+                slot_k[zk] += zkk;
+            }
+            if (nseg == 1 && a0 && zE) {   // the first run from u = 0 ends inside the chunk
+                ap = zp;
+                aw = zw;
+                akk = zkk;
+            }
+            zE = false;
+            if (key >= 0) {                // a new run
+                if (nseg == 0) {
+                    ak = key;
+                    a0 = (u == 0);
+                }
+                ++nseg;
+                zk = key;
+                zp = p;
+                zw = w;
+                zkk = wk;
+                zE = true;                 // open; stays true while it runs to u = E - 1
+            }
+        }
+        // zE: the last run is open at u = E - 1 (it ends at the chunk end)
+        const bool one = nseg == 1 && a0 && zE;
+        if (nseg == 1 && a0 && zE) {
+            ap = zp;
+            aw = zw;
+            akk = zkk;
+        }
+        rfk[t] = (nseg > 0 && a0) ? ak : -1;
+        rfs[t] = ap;
+        rfs[256 + t] = aw;
+        rfs[512 + t] = akk;
+        rlk[t] = (nseg > 0 && zE) ? zk : -1;
+        rone[t] = one ? 1 : 0;
+        __syncthreads();
+        auto walk = [&](int key, double &p, double &w, double &k) {
+            for (int u2 = t + 1; u2 < 256 && u2 / K == slot && rfk[u2] == key; ++u2) {
+                p += rfs[u2];
+                w += rfs[256 + u2];
+                k += rfs[512 + u2];
+                if (!rone[u2]) break;
+            }
+        };
+        if (nseg > 0 && a0) {
+            const bool continued = chunk > 0 && rlk[t - 1] == ak;
+            if (!continued) {
+                double p = ap, w = aw, k = akk;
+                if (one) walk(ak, p, w, k);
+                slot_p[ak] += p;
+                slot_w[ak] += w;
+                slot_k[ak] += k;
+            }
+        }
+        if (nseg > 0 && zE && !one) {
+            double p = zp, w = zw, k = zkk;
+            walk(zk, p, w, k);
+            slot_p[zk] += p;
+            slot_w[zk] += w;
+            slot_k[zk] += k;
         }
     }
     __syncthreads();
     double *dst = binpart + (long long)blockIdx.x * 3 * nBins;
-    for (int b = threadIdx.x; b < 3 * nBins; b += blockDim.x) dst[b] = hp[b];
+    for (int b = threadIdx.x; b < nBins; b += blockDim.x) {
+        double p = 0.0, w = 0.0, k = 0.0;
+        for (int q = 0; q < RI; ++q) {
+            p += hp[(size_t)q * nBins + b];
+            w += hw[(size_t)q * nBins + b];
+            k += hk[(size_t)q * nBins + b];
+        }
+        dst[b] = p;
+        dst[nBins + b] = w;
+        dst[2 * nBins + b] = k;
+    }
+}
+
+// Chunk shape of k_bin for rows of N / 2 + 1 modes: E modes per thread (at
+// least 8), K chunks per row, RI rows per pass of 256 threads; LDS bytes.
+struct BinShape {
+    int E, K, RI;
+    size_t shm;
+};
+static BinShape bin_shape(int N, int nBins)
+{
+    const int nc = N / 2 + 1;
+    BinShape b;
+    b.E = std::max(8, (nc + 255) / 256);
+    b.K = (nc + b.E - 1) / b.E;
+    b.RI = std::max(1, 256 / b.K);
+    b.shm = sizeof(double) * (3 * (size_t)b.RI * nBins + 3 * 256) + sizeof(int) * 3 * 256;
+    return b;
 }
 
 // Fixed-order sum of the per-block bin partials: 64 bins per block, the
@@ -718,8 +826,8 @@ static cb_status plan_shape(const cb_sim *s, FftPlan *p)
     const long long modes = p->rows * (N / 2 + 1);
     p->binblocks = (int)std::max<long long>(1, std::min<long long>(256, (modes + 4095) / 4096));
     if (p->fused) p->fused_blocks = p2_bin_blocks_max();
-    if (p->fused || (!p->N1 && D >= 2)) {
-        const size_t shm = sizeof(double) * 3 * (size_t)p->nBins;
+    if (!p->N1) {   // every path but the four-step one bins with k_bin (the fused one its weights)
+        const size_t shm = bin_shape(N, p->nBins).shm;
         if (shm > kLdsBlockMax) {
             set_error("too many spectrum bins for LDS");
             return CB_UNSUPPORTED;
@@ -754,11 +862,12 @@ static hipError_t plan_buffers(FftPlan *p, Arena *arena, void *stage, double L, 
     if (e == hipSuccess && p->fused) e = zero_words(p->H, hbytes, st);
     if (e == hipSuccess && p->fused) {
         const int nB = p->nBins;
-        const size_t shm = sizeof(double) * 3 * (size_t)nB;
-        e = lds_opt_in(k_bin<double, true>, shm);
+        const BinShape bs = bin_shape(N, nB);
+        e = lds_opt_in(k_bin<double, true>, bs.shm);
         if (e != hipSuccess) return e;
         const double dk = 2.0 * 3.14159265358979323846 / L;
-        k_bin<double, true><<<p->binblocks, 256, shm, st>>>(nullptr, p->rows, N, D, nB, dk, 0.0, 0.0, p->binpart);
+        k_bin<double, true><<<p->binblocks, 256, bs.shm, st>>>(nullptr, p->rows, N, D, nB, dk, 0.0, 0.0, bs.E, bs.K,
+                                                               bs.RI, p->binpart);
         k_bin_reduce<<<(3 * nB + 63) / 64, 256, 0, st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
         e = hipGetLastError();
     }
@@ -815,13 +924,18 @@ static cb_status spectrum_run(cb_sim *s, FftPlan *p, double *h, double *kmean, d
         if (s->fp32) e = forward_r2c<float, float>(p->st, (const cpx<float> *)p->tw, (const float *)field, (cpx<float> *)p->H, D, s->st);
         else e = forward_r2c<double, double>(p->st, (const cpx<double> *)p->tw, (const double *)field, (cpx<double> *)p->H, D, s->st);
         HIP_TRY(e);
-        const size_t shm = sizeof(double) * 3 * (size_t)nB;
-        if (shm > kLdsBlockMax || (s->fp32 ? lds_opt_in(k_bin<float>, shm) : lds_opt_in(k_bin<double>, shm)) != hipSuccess) {
+        const BinShape bs = bin_shape(N, nB);
+        if (bs.shm > kLdsBlockMax ||
+            (s->fp32 ? lds_opt_in(k_bin<float>, bs.shm) : lds_opt_in(k_bin<double>, bs.shm)) != hipSuccess) {
             set_error("too many spectrum bins for LDS");
             return CB_UNSUPPORTED;
         }
-        if (s->fp32) k_bin<float><<<p->binblocks, 256, shm, s->st>>>((const cpx<float> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm, p->binpart);
-        else k_bin<double><<<p->binblocks, 256, shm, s->st>>>((const cpx<double> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm, p->binpart);
+        if (s->fp32)
+            k_bin<float><<<p->binblocks, 256, bs.shm, s->st>>>((const cpx<float> *)p->H, p->rows, N, D, nB, dk, pref, Pnorm,
+                                                                bs.E, bs.K, bs.RI, p->binpart);
+        else
+            k_bin<double><<<p->binblocks, 256, bs.shm, s->st>>>((const cpx<double> *)p->H, p->rows, N, D, nB, dk, pref,
+                                                                 Pnorm, bs.E, bs.K, bs.RI, p->binpart);
         HIP_TRY(hipGetLastError());
         k_bin_reduce<<<(3 * nB + 63) / 64, 256, 0, s->st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
         HIP_TRY(hipGetLastError());

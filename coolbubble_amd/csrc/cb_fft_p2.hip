@@ -321,6 +321,11 @@ template <typename R, int N, bool BIN>
 constexpr int kColRounds = (!BIN && sizeof(R) == 4) ? 2 : 1;
 constexpr size_t kLdsBlockMax = 160 * 1024;   // gfx950: LDS a block may use
 
+// the binning pass's run table fits the data area of its columns
+template <typename R, int N>
+constexpr bool kRunInData = (size_t)2 * kSeq<N> * (N / 2 + 1) * sizeof(double) <=
+                            (size_t)kSeq<N> * kColStride<R, N> * sizeof(cpx<R>);
+
 template <typename R, int N, bool BIN>
 __global__ void __launch_bounds__(256)
 k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict__ tw, int sign, BinArgs ba)
@@ -330,7 +335,20 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
     constexpr int CB = kSeq<N> * RND, BS = kColStride<R, N>, nc = N / 2 + 1;
     const cpx<R> *twl = stage_twiddles<R, N>(smem_raw, tw);
     cpx<R> *buf = data_area<R, N>(smem_raw);
+    // binning: the block's histogram, the per-thread run records and the
+    // bin range of each (column, half) line
     double *hist = reinterpret_cast<double *>(buf + CB * BS);
+    double *rec_fs = hist + ba.nBins;
+    int *rec_fk = reinterpret_cast<int *>(rec_fs + 256);
+    int *rec_lk = rec_fk + 256;
+    int *rec_one = rec_lk + 256;
+    int *line_lo = rec_one + 256;          // [2 CB] lowest bin of the line
+    int *line_hi = line_lo + 2 * CB;       // [2 CB] highest bin, -1: none
+    // run totals [2 CB][SPAN] by (line, bin - lowest bin): in the data area
+    // (free once an item's values are read) where it fits, else after it all
+    constexpr int SPAN = N / 2 + 1;        // bins a line can meet
+    double *runtab = kRunInData<R, N> ? reinterpret_cast<double *>(buf)
+                                      : reinterpret_cast<double *>(line_hi + 2 * CB + 2 * (CB & 1));
     if constexpr (BIN)
         for (int b = threadIdx.x; b < ba.nBins; b += 256) hist[b] = 0.0;
     const long long tiles = (inner + CB - 1) / CB;
@@ -367,60 +385,102 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
                 if (cc < ncol) base[(long long)e * inner + cc] = buf[cc * BS + pidx(e)];
             }
         } else {
-            // Binning with no atomics, in a fixed order: thread t owns the
-            // bins [b0, b1) and only it touches their LDS slots.  For each
-            // column (j, and i1 in 3D: squared offset C = s1^2 + j^2) and each
-            // half of axis 0 (s_e = e for e <= N/2, e - N above), the bin of
-            // a mode never decreases with |s_e|, so the thread starts at the
-            // first |s_e| of bin b0 (an integer square root) and walks up to
-            // bin b1, summing each bin's run in a register.  kc^D is a
-            // product of D factors (within 2 ulp of the reference's
-            // pow(kc, D)).
-            const int bpt = bins_per_thread(ba.nBins, 256);
-            const int b0 = threadIdx.x * bpt;
-            const int b1 = b0 + bpt < ba.nBins ? b0 + bpt : ba.nBins;
-            for (int cc = 0; cc < ncol && b0 < b1; ++cc) {
-                const int c = (int)c0 + cc;
-                const int j = ba.D == 3 ? c % kNcp<N> : c;
-                if (j >= nc) continue;
-                const int i1 = ba.D == 3 ? c / kNcp<N> : 0;
-                const int s1 = (i1 <= N / 2) ? i1 : i1 - N;
-                const long long C = (long long)s1 * s1 + (long long)j * j;
-                const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
-                for (int half = 0; half < 2; ++half) {
-                    const long long amin = half, amax = half ? N / 2 - 1 : N / 2;
-                    if ((long long)b1 * b1 <= C + amin * amin) continue;     // every bin >= b1
-                    if ((long long)b0 * b0 > C + amax * amax) continue;      // every bin < b0
-                    long long a = isqrt_ceil((long long)b0 * b0 - C);
-                    if (a < amin) a = amin;
-                    int cur = -1;
-                    double acc = 0.0;
-                    for (; a <= amax; ++a) {
-                        const double n = sqrt((double)(C + a * a));
-                        const int bin = (int)n;
-                        if (bin >= b1) break;
-                        const int e = half ? N - (int)a : (int)a;
-                        const double kc = n * ba.dk;
-                        const double kd = (ba.D == 3) ? kc * kc * kc : kc * kc;
-                        const cpx<R> f = buf[cc * BS + pidx(e)];
-                        const double fx = (double)f.x, fy = (double)f.y;
-                        const double v = w * ba.pref * kd * ba.Pnorm * (fx * fx + fy * fy);
-                        if (bin == cur) {
-                            acc += v;
-                        } else {
-                            if (cur >= 0) hist[cur] += acc;
-                            cur = bin;
-                            acc = v;
-                        }
-                    }
-                    if (cur >= 0) hist[cur] += acc;
+            // Binning in a fixed order, no atomics.  Thread t takes E = 8
+            // consecutive rows e of one column (CB N = 2048 modes per item).
+            // Along a column the bin never decreases with |s_e| in each half
+            // of axis 0 (s_e = e up to N/2, e - N above): such a line meets
+            // every bin from its lowest to its highest once, as one
+            // contiguous run of rows.  Each thread sums the runs in its
+            // rows; a run that crosses into the next threads' rows is
+            // completed by the thread where it starts, from the next
+            // threads' first partial sums (in row order).  The run totals go
+            // to a table indexed by (line, bin - lowest bin of the line), in
+            // the data area (its values are read by then); the thread that
+            // owns a bin in the block's histogram then adds that bin's
+            // totals line by line, in a fixed order.  kc^D is a product of D
+            // factors (within 2 ulp of the reference's pow(kc, D)).
+            constexpr int E = 8;
+            static_assert(CB * N == 256 * E, "eight modes per thread");
+            const int t = threadIdx.x;
+            const int cc = (t * E) / N, e0 = (t * E) % N;
+            const int c = (int)c0 + cc;
+            const int j = ba.D == 3 ? c % kNcp<N> : c;
+            const int i1 = ba.D == 3 ? c / kNcp<N> : 0;
+            const int s1 = (i1 <= N / 2) ? i1 : i1 - N;
+            const bool colok = cc < ncol && j < nc;
+            const long long C = (long long)s1 * s1 + (long long)j * j;
+            const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
+            int key[E];
+            double val[E];
+#pragma unroll
+            for (int u = 0; u < E; ++u) {
+                const int e = e0 + u;
+                const int half = e > N / 2 ? 1 : 0;
+                const long long a = half ? N - e : e;
+                const double n = sqrt((double)(C + a * a));
+                const int bin = (int)n;
+                const double kc = n * ba.dk;
+                const double kd = (ba.D == 3) ? kc * kc * kc : kc * kc;
+                const cpx<R> f = buf[cc * BS + pidx(e)];
+                const double fx = (double)f.x, fy = (double)f.y;
+                val[u] = w * ba.pref * kd * ba.Pnorm * (fx * fx + fy * fy);
+                key[u] = (colok && bin < ba.nBins) ? 2 * bin + half : -1;
+            }
+            if (e0 == 0) {   // the column's two lines: half 0 (|s_e| = 0..N/2), half 1 (1..N/2-1)
+                const int lo0 = (int)sqrt((double)C), hi0 = (int)sqrt((double)(C + (long long)(N / 2) * (N / 2)));
+                const int lo1 = (int)sqrt((double)(C + 1)), hi1 = (int)sqrt((double)(C + (long long)(N / 2 - 1) * (N / 2 - 1)));
+                line_lo[2 * cc] = lo0;
+                line_hi[2 * cc] = colok ? min(hi0, ba.nBins - 1) : -1;
+                line_lo[2 * cc + 1] = lo1;
+                line_hi[2 * cc + 1] = colok ? min(hi1, ba.nBins - 1) : -1;
+            }
+            // runs of equal keys in the thread's rows (keys are monotone in a
+            // half): val[u] becomes the sum of the run from row u to its end
+            // in these rows (static indices only: no register arrays indexed
+            // at run time)
+#pragma unroll
+            for (int u = E - 2; u >= 0; --u)
+                if (key[u] >= 0 && key[u + 1] == key[u]) val[u] += val[u + 1];
+            const bool starts = key[0] >= 0, ends = key[E - 1] >= 0;
+            bool one = starts;
+#pragma unroll
+            for (int u = 1; u < E; ++u) one = one && key[u] == key[0];
+            rec_fk[t] = starts ? key[0] : -1;
+            rec_fs[t] = starts ? val[0] : 0.0;
+            rec_lk[t] = ends ? key[E - 1] : -1;
+            rec_one[t] = one ? 1 : 0;
+            __syncthreads();   // records and line ranges written; the data area is free
+            const int tcol0 = (t / (N / E)) * (N / E);     // first thread of this column
+            // the first run is owned by a previous thread when it goes on from there
+            const bool cont = starts && t > tcol0 && rec_lk[t - 1] == key[0];
+            // the last run's part in the following threads' rows
+            double ext = 0.0;
+            if (ends && !(one && cont))
+                for (int u2 = t + 1; u2 < tcol0 + N / E && rec_fk[u2] == key[E - 1]; ++u2) {
+                    ext += rec_fs[u2];
+                    if (!rec_one[u2]) break;
                 }
+#pragma unroll
+            for (int u = 0; u < E; ++u) {
+                const bool head = key[u] >= 0 && (u == 0 || key[u - 1] != key[u]);
+                if (!head || (u == 0 && cont)) continue;
+                const int line = 2 * cc + (key[u] & 1);
+                const bool last = ends && key[u] == key[E - 1];
+                runtab[line * SPAN + (key[u] >> 1) - line_lo[line]] = last ? val[u] + ext : val[u];
+            }
+            __syncthreads();   // run table complete
+            const int bpt = bins_per_thread(ba.nBins, 256);
+            for (int b = t * bpt; b < t * bpt + bpt && b < ba.nBins; ++b) {
+                double acc = hist[b];
+                for (int line = 0; line < 2 * CB; ++line)
+                    if (b >= line_lo[line] && b <= line_hi[line]) acc += runtab[line * SPAN + b - line_lo[line]];
+                hist[b] = acc;
             }
         }
     }
     if constexpr (BIN) {
         // one row of partials per block, summed in block order afterwards
-        __syncthreads();
+        __syncthreads();   // a block may have had no item
         double *dst = ba.binpart + (long long)blockIdx.x * ba.nBins;
         for (int b = threadIdx.x; b < ba.nBins; b += 256) dst[b] = hist[b];
     }
@@ -510,7 +570,10 @@ hipError_t cols_bin_n(const void *H, long long inner, const void *tw, BinArgs ba
                       hipStream_t st)
 {
     constexpr int CB = kCols<R, N, true>;
-    const size_t shm = cols_shm<R, N, true>() + sizeof(double) * (size_t)ba.nBins;
+    constexpr size_t CBn = kCols<R, N, true>;
+    const size_t shm = cols_shm<R, N, true>() + sizeof(double) * ((size_t)ba.nBins + 256) +
+                       sizeof(int) * (3 * 256 + 4 * CBn + 2 * (CBn & 1)) +
+                       (kRunInData<R, N> ? 0 : sizeof(double) * 2 * CBn * (N / 2 + 1));
     hipError_t e = allow_lds(k_p2_cols<R, N, true>, shm);
     if (e != hipSuccess) return e;
     const long long items = (inner + CB - 1) / CB;

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <set>
 #include <mutex>
 #include <new>
 #include <string>
@@ -103,7 +104,8 @@ struct cb_server {
     std::vector<Req *> active;               // scheduler thread only
     std::vector<Req *> done;                 // completed, not yet polled
     std::map<unsigned long long, Req *> kept;   // polled, outputs stored until released
-    std::map<unsigned long long, bool> cancel_req;
+    std::map<unsigned long long, bool> cancel_req;   // cancels of admitted requests, taken at their next tick
+    std::set<unsigned long long> live;               // admitted and not finished
     long long used = 0, budget = 0;
     bool stop = false;
     int busy = 0;                            // requests admitted and not finished
@@ -152,6 +154,8 @@ void finish(cb_server *sv, Req *q, cb_status st)
     std::lock_guard<std::mutex> lk(sv->mu);
     sv->used -= q->footprint;
     sv->busy -= 1;
+    sv->live.erase(q->id);
+    sv->cancel_req.erase(q->id);
     sv->done.push_back(q);
     sv->idle_cv.notify_all();
 }
@@ -177,6 +181,7 @@ void admit(cb_server *sv)
                 sv->queue.pop_front();
                 sv->used += head->footprint;
                 sv->busy += 1;
+                sv->live.insert(head->id);
                 q = head;
             }
         }
@@ -231,11 +236,12 @@ cb_status tick(cb_server *sv)
         if (m[i] > L) L = m[i];
         if (q->sim->lagged) any_lagged = true;
     }
-    // batched tables: one-step tiled requests first, then multi-step ones
+    // batched tables: one-step tiled requests first, then multi-step ones;
+    // the general-kernel requests and the large ones step on their own
     std::vector<size_t> single, pair, general;
     for (size_t i = 0; i < act.size(); ++i) {
         const cb_sim *s = act[i]->sim;
-        if (s->variant < 0) general.push_back(i);
+        if (s->variant < 0 || cb::server_solo(s)) general.push_back(i);
         else (s->variants[(size_t)s->variant].fuse >= 2 ? pair : single).push_back(i);
     }
     const size_t ntab = single.size() + pair.size();
@@ -444,6 +450,10 @@ cb_status cb_server_cancel(cb_server *sv, unsigned long long id)
             sv->done.push_back(q);
             return CB_OK;
         }
+    }
+    if (!sv->live.count(id)) {
+        set_error("no queued or active request with this id");
+        return CB_INVALID_ARG;
     }
     sv->cancel_req[id] = true;   // active: retired at its next tick
     return CB_OK;

@@ -78,8 +78,10 @@ int pick_variant(const cb_sim *s, int items_max, int pf, int nt, int ni_pref);
 // a static member's two- or three-step tile (at most max_fuse steps) for
 // the batched multi-step launch, or -1
 int pick_variant_multi(const cb_sim *s, int max_fuse);
-// the variant the server gives a request: a two-step tile from
-// pick_variant_multi when there is one, else batch_variant
+// the variant the server gives a request: a lone run's for the large
+// static lattices that step on their own (server_solo), else a two-step
+// tile from pick_variant_multi when there is one, else batch_variant
+bool server_solo(const cb_sim *s);
 int server_variant(const cb_sim *s);
 int companion_variant(const cb_sim *s, const TiledCfg &f);
 // table entry of a member with its selected tiled variant (one- or two-step)

@@ -230,6 +230,9 @@ typedef struct cb_result {
 } cb_result;
 cb_status cb_server_create(const cb_server_opts *o, cb_server **out);
 cb_status cb_server_submit(cb_server *s, const cb_request *r, unsigned long long *id);
+/* A queued request is dropped at once, an active one at its next tick; its
+ * result has status CB_NOT_READY.  CB_INVALID_ARG for an id that is neither
+ * queued nor active (unknown, or finished). */
 cb_status cb_server_cancel(cb_server *s, unsigned long long id);
 /* completed requests since the last poll: up to cap results, *n written */
 cb_status cb_server_poll(cb_server *s, cb_result *out, int cap, int *n);
@@ -243,9 +246,11 @@ cb_status cb_server_stats(cb_server *s, int *queued, int *active, int *done, lon
 /* block until no request is queued or active */
 cb_status cb_server_drain(cb_server *s);
 /* The kernel variant the server gives a request with the parameters of s
- * (a two-step variant for a static box where one fits the batched launch,
- * else a one-step one; -1 = the general kernel).  Selecting it on a lone
- * simulation reproduces the server's results for that request bit for bit. */
+ * (for a static box of 2^23 sites or more the variant a lone run starts on:
+ * such requests step on their own kernels; else a two-step variant where one
+ * fits the batched launch, else a one-step one; -1 = the general kernel).
+ * Selecting it on a lone simulation reproduces the server's results for that
+ * request bit for bit. */
 cb_status cb_server_variant(const cb_sim *s, int *variant);
 cb_status cb_server_destroy(cb_server *s);
 

@@ -228,6 +228,30 @@ def test_autotune_preserves_state_and_selects_a_variant():
     assert (st0.t, st0.step, st0.a) == (st1.t, st1.step, st1.a)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["2d_mode1", "3d_mode-2"])
+def test_nan_injected_through_set_state_is_reported(name, precision):
+    """A non-finite value in the field reaches the energy sums; the row after
+    the next measured step reports CB_NAN_DETECTED (the reference has no
+    check and would print nan, src/bubble.cpp:977-989)."""
+    kw = CASES[name]
+    g = cb.Bubble(precision=precision, **kw)
+    g.step(3)
+    phi, pi = g.fields()
+    st = g.state()
+    phi.reshape(-1)[phi.size // 3] = np.nan
+    g.set_state(st, phi, pi)
+    g.step(2)
+    with pytest.raises(cb.CBError) as e:
+        g.energy_row()
+    assert e.value.status == 5                       # CB_NAN_DETECTED
+    # a clean state clears the flag
+    phi.reshape(-1)[phi.size // 3] = 0.0
+    g.set_state(st, phi, pi)
+    g.step(2)
+    assert np.all(np.isfinite(g.energy_row()))
+
+
 def test_set_state_roundtrip_and_stepping_from_it():
     kw = CASES["2d_mode1"]
     o = oracle.Sim(**kw)

@@ -46,8 +46,7 @@ def test_results_identical_alone_and_in_the_mix(precision):
         p, nt, snaps = REQS[r.tag]
         alone = cb.Bubble(precision=precision, **p)
         v = cb.server_variant(alone)
-        if v >= 0:
-            alone.select_kernel(v)
+        alone.select_kernel(v)            # -1: the general kernel, as the server runs it
         if p["nGrids"] == 192:
             assert alone.variant_info(v)["fuse"] == 2
         rows = [alone.energy_row()]
@@ -101,4 +100,44 @@ def test_cancel_queued_request():
     res = {r.id: r for r in sv.poll()}
     assert res[first].status == 0
     assert res[second].status == 8                    # CB_NOT_READY: cancelled
+    sv.close()
+
+
+def test_cancel_active_request_while_others_finish_unchanged():
+    """A request cancelled while it runs is retired at its next tick with
+    CB_NOT_READY; the requests beside it finish with the results they have
+    alone.  Cancelling an id that is neither queued nor active is an error."""
+    sv = cb.Server(precision="fp64", max_active=4)
+    long_id = sv.submit(100000000, 1, tag=99, D=3, nGrids=64, L=8.0, dt=0.025, readxns=-2)
+    others = {sv.submit(nt, snaps, tag=i, **p): i for i, (p, nt, snaps) in enumerate(REQS[:3])}
+    for _ in range(2000):                           # wait until the long one is active
+        if sv.stats()["active"] >= 1 and sv.stats()["queued"] <= 3:
+            break
+        time.sleep(0.005)
+    time.sleep(0.05)
+    sv.cancel(long_id)
+    sv.drain()
+    res = {r.id: r for r in sv.poll()}
+    assert res[long_id].status == 8                 # CB_NOT_READY: cancelled
+    assert res[long_id].site_updates == 0
+    for rid, i in others.items():
+        assert res[rid].status == 0
+        p, nt, snaps = REQS[i]
+        alone = cb.Bubble(precision="fp64", **p)
+        v = cb.server_variant(alone)
+        alone.select_kernel(v)            # -1: the general kernel, as the server runs it
+        done = 0
+        dt = p["dt"]
+        T = nt * dt
+        sched = sorted(set(min(nt, max(1, int(math.floor(T * k / snaps / dt + 0.5)))) for k in range(1, snaps + 1)))
+        for ev in sorted(set(list(range(40, nt + 1, 40)) + sched + [nt])):
+            alone.step(ev - done)
+            done = ev
+        np.testing.assert_array_equal(np.array(res[rid].final_row[:]), alone.energy_row())
+        alone.close()
+    with pytest.raises(cb.CBError) as e:
+        sv.cancel(long_id)                          # finished: neither queued nor active
+    assert e.value.status == 1
+    with pytest.raises(cb.CBError):
+        sv.cancel(123456789)
     sv.close()
