@@ -70,6 +70,7 @@ def lib():
         L.cbo_get_scalars.argtypes = [ctypes.c_void_p, dp]
         L.cbo_set_scalars.argtypes = [ctypes.c_void_p, dp]
         L.cbo_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.cbo_set_energy_cadence.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong]
         L.cbo_ctor.argtypes = [ctypes.c_void_p]
         L.cbo_seed_sites.argtypes = [ctypes.POINTER(Params), dp, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_longlong), ctypes.c_int, dp]
@@ -158,6 +159,11 @@ class Sim:
     def set_scalars(self, s):
         s = np.ascontiguousarray(s, dtype=np.float64)
         lib().cbo_set_scalars(self._h, _dp(s))
+
+    def set_energy_cadence(self, every, nstep=0):
+        """Static box: energies only on every `every`-th step counted from
+        nstep (cbo_set_energy_cadence); rows there are the every-step rows."""
+        lib().cbo_set_energy_cadence(self._h, int(every), int(nstep))
 
     def ctor(self):
         """Run the ctor's sequence (src/bubble.cpp:239-291) on the current

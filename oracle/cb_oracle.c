@@ -42,6 +42,11 @@ struct cbo_sim {
     double EtotVar, EkintotVar, EpottotVar, EgradtotVar, avphiPVar, avphiVar;
     double *xns;       /* centres read back from the __xns.dat text */
     int n_xns;
+    /* static box only: measure the energies on steps n with n % energy_every
+     * == 0 and the kinetic half-step sum on the steps before them (0: every
+     * step, as the reference does); nstep counts the steps taken */
+    int energy_every;
+    long long nstep;
 };
 
 /* ------------------------------------------------------------------ */
@@ -660,6 +665,33 @@ static void measure_energies(cbo_sim *s)                     /* src/bubble.cpp:7
     s->Etot = s->Ekintot + s->Epottot + s->Egradtot;
 }
 
+/* The kinetic half-step sum of measure_energies alone (the same per-site
+ * terms, thread split and compensated order as its E_KIN sum, so the same
+ * double), chained into EkinHalfPrev as measure_energies chains it. */
+static void measure_kinetic(cbo_sim *s)
+{
+    const int D = s->p.D, nth = s->nthreads;
+    const double *pi = s->pi;
+    const double dxLD = s->dxLD;
+    const double a_kin = s->a + 0.5 * s->p.dt * s->ap;
+    ksum *part = calloc((size_t)nth, sizeof(ksum));
+#pragma omp parallel num_threads(nth)
+    {
+        ksum acc = {0, 0};
+        int tid = omp_get_thread_num();
+#pragma omp for schedule(static)
+        for (long long i = 0; i < s->N; ++i) {
+            double Ek = 0.5 * pow(a_kin, -2 * D) * pi[i] * pi[i];
+            ks_add(&acc, Ek * dxLD);
+        }
+        part[tid] = acc;
+    }
+    ksum k = {0, 0};
+    for (int t = 0; t < nth; ++t) ks_merge(&k, part[t]);
+    free(part);
+    s->EkinHalfPrev = ks_val(k);
+}
+
 static void one_step(cbo_sim *s, double *dpi)                /* src/bubble.cpp:849-876 */
 {
     const double dt = s->p.dt;
@@ -668,7 +700,14 @@ static void one_step(cbo_sim *s, double *dpi)                /* src/bubble.cpp:8
     double *pi = s->pi, *phi = s->phi;
 #pragma omp parallel for schedule(static) num_threads(nth)
     for (long long i = 0; i < s->N; ++i) pi[i] += dt * dpi[i];
-    measure_energies(s);
+    s->nstep += 1;
+    if (s->energy_every > 0 && !expanding(s)) {
+        /* a static box: nothing feeds back from the energies (a'' = 0) */
+        if (s->nstep % s->energy_every == 0) measure_energies(s);
+        else if ((s->nstep + 1) % s->energy_every == 0) measure_kinetic(s);
+    } else {
+        measure_energies(s);
+    }
     double ap_prev = s->ap;
     s->ap += dt * addot(s);
     double ap_atn = (ap_prev + s->ap) / 2.0;
@@ -987,6 +1026,7 @@ long long cbo_nsites(const cbo_sim *s) { return s->N; }
 double *cbo_phi(cbo_sim *s) { return s->phi; }
 double *cbo_pi(cbo_sim *s) { return s->pi; }
 void cbo_set_threads(cbo_sim *s, int n) { s->nthreads = n > 0 ? n : omp_get_max_threads(); }
+void cbo_set_energy_cadence(cbo_sim *s, int every, long long nstep) { s->energy_every = every; s->nstep = nstep; }
 
 int cbo_xns(const cbo_sim *s, double *xyz, int cap)
 {

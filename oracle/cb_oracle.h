@@ -71,6 +71,12 @@ int cbo_xns(const cbo_sim *s, double *xyz, int cap);
 void cbo_get_scalars(const cbo_sim *s, double out[6]);
 void cbo_set_scalars(cbo_sim *s, const double in[6]);
 void cbo_set_threads(cbo_sim *s, int nthreads);
+/* Static box: measure the energies only on steps n (counting from nstep)
+ * with n % every == 0, and on the step before each the kinetic half-step
+ * sum they need (every = 0: every step).  The rows read after those steps
+ * are the every-step rows bit for bit; the fields do not depend on it.
+ * Expanding runs measure every step whatever the setting. */
+void cbo_set_energy_cadence(cbo_sim *s, int every, long long nstep);
 
 /* ---- pieces, exported for the pinning tests ---- */
 double cbo_V(double phi, double c, double DeltaV);

@@ -777,3 +777,30 @@ def test_hubble_constraint_converges_at_second_order(alpha, Mpl, T):
     assert abs(vals[-1]) < 1e-3
     for coarse, fine in zip(vals, vals[1:]):
         assert 3.6 < coarse / fine < 4.6, vals
+
+
+@pytest.mark.parametrize("kw", [dict(D=2, nGrids=40, L=5.0, c=4.0, readxns=1, nBubbles=4, dt=0.025),
+                                dict(D=3, nGrids=16, L=4.0, c=2.0, readxns=-3, Trad=0.4, dt=0.025)])
+def test_energy_cadence_gives_the_every_step_rows(kw):
+    """A static box measured only on every 40th step (and the kinetic sum on
+    the step before) prints the every-step oracle's rows bit for bit, on
+    the same fields (the full-length fixtures are generated this way)."""
+    a, b = oracle.Sim(**kw), oracle.Sim(**kw)
+    b.set_energy_cadence(40)
+    for _ in range(3):
+        a.step(40)
+        b.step(40)
+        np.testing.assert_array_equal(a.energy_row(), b.energy_row())
+    a.step(17)
+    b.step(17)
+    np.testing.assert_array_equal(a.phi(), b.phi())
+    np.testing.assert_array_equal(a.pi(), b.pi())
+    # resumed mid-way at step 137 with the step count
+    c = oracle.Sim(blank=True, **kw)
+    c.phi()[...] = b.phi()
+    c.pi()[...] = b.pi()
+    c.set_scalars(b.scalars())
+    c.set_energy_cadence(40, 137)
+    a.step(23)
+    c.step(23)
+    np.testing.assert_array_equal(a.energy_row()[2:], c.energy_row()[2:])
