@@ -58,6 +58,11 @@ struct SimConst {
     double c2, c3, c4;      // V' coefficients 2c, 3(2c+4), 4(c+3)
     double c5, c6;          // V coefficients 2c+4, c+3
     double Mpl;
+    // 1: this launch measures only what the Friedmann update needs (the
+    // kinetic and potential sums, the gradient sum in conformal time); an
+    // expanding run's steps before the last of a call, whose rows nobody
+    // reads, are measured so
+    int energy_lite;
 };
 
 // ---------------------------------------------------------------------------

@@ -50,41 +50,43 @@ struct Neu {
     __device__ double value() const { return s + c; }
 };
 
-// Sum nblocks x NQ partials (row-major) into tot[NQ]; one block of 256.
+// Sum nblocks x NQ partials (row-major) into tot[0, NV) (the first NV
+// quantities of each row); one block of 256.
+template <int NV = NQ>
 __device__ inline void sum_partials_block(const double *partials, int nblocks, double *tot)
 {
-    __shared__ double ss[NQ][256];
-    __shared__ double sc[NQ][256];
+    __shared__ double ss[NV][256];
+    __shared__ double sc[NV][256];
     const int t = threadIdx.x;
-    Neu n[NQ];
+    Neu n[NV];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) n[q] = {0.0, 0.0};
+    for (int q = 0; q < NV; ++q) n[q] = {0.0, 0.0};
     // Each thread adds rows t, t + 256, t + 512, ... in that order; the rows
     // are loaded four at a time so that their latencies overlap (one block
     // alone on the GPU has nothing else to hide them behind).
     const int stride = blockDim.x;
     int b = t;
     for (; b + 3 * stride < nblocks; b += 4 * stride) {
-        double v[4][NQ];
+        double v[4][NV];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) v[u][q] = partials[(long long)(b + u * stride) * NQ + q];
+            for (int q = 0; q < NV; ++q) v[u][q] = partials[(long long)(b + u * stride) * NQ + q];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) n[q].add(v[u][q]);
+            for (int q = 0; q < NV; ++q) n[q].add(v[u][q]);
     }
     for (; b < nblocks; b += stride)
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) n[q].add(partials[(long long)b * NQ + q]);
+        for (int q = 0; q < NV; ++q) n[q].add(partials[(long long)b * NQ + q]);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) { ss[q][t] = n[q].s; sc[q][t] = n[q].c; }
+    for (int q = 0; q < NV; ++q) { ss[q][t] = n[q].s; sc[q][t] = n[q].c; }
     __syncthreads();
     for (int half = blockDim.x >> 1; half > 0; half >>= 1) {
         if (t < half) {
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
+            for (int q = 0; q < NV; ++q) {
                 Neu a = {ss[q][t], sc[q][t]};
                 a.merge(Neu{ss[q][t + half], sc[q][t + half]});
                 ss[q][t] = a.s;
@@ -93,7 +95,7 @@ __device__ inline void sum_partials_block(const double *partials, int nblocks, d
         }
         __syncthreads();
     }
-    if (t < NQ) tot[t] = ss[t][0] + sc[t][0];
+    if (t < NV) tot[t] = ss[t][0] + sc[t][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -138,7 +140,9 @@ __device__ inline void neighbours(long long s, int N, long long (&nb)[6])
 // Energy terms of one site (measureEnergies, src/bubble.cpp:790-827), in
 // double.  gP = {forward neighbour along axis 1, 0, 2} per the reference's
 // gx, gy, gz naming (1D: gx only).  EXACT: the FP64 path.
-template <int D, bool EXACT>
+// LITE: only the sums the Friedmann update reads (src/bubble.cpp:747-757):
+// kinetic and potential, gradient in conformal time.
+template <int D, bool EXACT, bool LITE = false>
 __device__ inline void energy_terms(double ph, const double (&fwd)[3], double pn, double po,
                                     const SimConst &k, const DevScalars *sc, double (&acc)[NQ])
 {
@@ -148,6 +152,27 @@ __device__ inline void energy_terms(double ph, const double (&fwd)[3], double pn
     // and phi = 1, so the FP64 path keeps the reference's correctly rounded
     // powers; the FP32 path, whose fields carry 1e-7 errors, uses products.
     const double iadx = sc->inv_adx, pk2 = sc->pk2, pk1 = sc->pk1;
+    if constexpr (LITE) {
+        if constexpr (EXACT) {
+            acc[Q_POT] += vpot(ph, k);
+        } else {
+            const double ph2 = ph * ph;
+            acc[Q_POT] += ((k.c * ph2 - k.c5 * (ph2 * ph)) + k.c6 * (ph2 * ph2)) + k.DeltaV;
+        }
+        acc[Q_KIN] += 0.5 * pk2 * pn * pn;
+        if (k.alpha != 0) {
+            double g0 = 0., g1 = 0., g2 = 0.;
+            if constexpr (D == 1) {
+                g0 = (fwd[0] - ph) * iadx;
+            } else {
+                g0 = (fwd[1] - ph) * iadx;
+                g1 = (fwd[0] - ph) * iadx;
+                if constexpr (D == 3) g2 = (fwd[2] - ph) * iadx;
+            }
+            acc[Q_GRAD] += 0.5 * ((g0 * g0 + g1 * g1) + g2 * g2);
+        }
+        return;
+    }
     double gx = 0., gy = 0., gz = 0.;
     if constexpr (D == 1) {
         gx = (fwd[0] - ph) * iadx;
@@ -211,7 +236,8 @@ __device__ inline void finalize_member(const SimConst &k, const double *partials
                                        int use_old_kin)
 {
     __shared__ double tot[NQ];
-    sum_partials_block(partials, nblocks, tot);
+    if (k.energy_lite) sum_partials_block<3>(partials, nblocks, tot);   // Q_KIN, Q_POT, Q_GRAD
+    else sum_partials_block<NQ>(partials, nblocks, tot);
     __syncthreads();
     if (threadIdx.x != 0) return;
     DevScalars s = *sc;
@@ -221,14 +247,16 @@ __device__ inline void finalize_member(const SimConst &k, const double *partials
     s.EkinHalfPrev = EkinHalf;
     s.Epottot = tot[Q_POT];
     s.Egradtot = tot[Q_GRAD];
-    s.avphi = tot[Q_PHI];
-    s.avphiP = tot[Q_PHIP];
-    s.EkintotVar = tot[Q_VKIN];
-    s.EpottotVar = tot[Q_VPOT];
-    s.EgradtotVar = tot[Q_VGRAD];
-    s.EtotVar = tot[Q_VTOT];
-    s.avphiVar = tot[Q_VPHI];
-    s.avphiPVar = tot[Q_VPHIP];
+    if (!k.energy_lite) {   // a lite step's row is never read: its other sums are left as they were
+        s.avphi = tot[Q_PHI];
+        s.avphiP = tot[Q_PHIP];
+        s.EkintotVar = tot[Q_VKIN];
+        s.EpottotVar = tot[Q_VPOT];
+        s.EgradtotVar = tot[Q_VGRAD];
+        s.EtotVar = tot[Q_VTOT];
+        s.avphiVar = tot[Q_VPHI];
+        s.avphiPVar = tot[Q_VPHIP];
+    }
     s.Etot = s.Ekintot + s.Epottot + s.Egradtot;
     // addot, src/bubble.cpp:747-757
     double addot = 0.0;

@@ -20,7 +20,7 @@ namespace cb {
 //                drift of the previous step); writes phi_n and pi_{n+1/2};
 //                the drift of this step follows once a_half is known.
 // ---------------------------------------------------------------------------
-template <typename T, int D, bool LAGGED, bool ENERGY>
+template <typename T, int D, bool LAGGED, int ENERGY>
 __global__ void __launch_bounds__(256)
 k_step_simple(SimConst k, const T *__restrict__ phi_in, T *__restrict__ phi_out, const T *pi_in,
               T *pi_out, const DevScalars *__restrict__ sc, double *__restrict__ partials)
@@ -74,7 +74,7 @@ k_step_simple(SimConst k, const T *__restrict__ phi_in, T *__restrict__ phi_out,
         }
         if constexpr (ENERGY) {
             double fwd[3] = {(double)P[0], D >= 2 ? (double)P[1] : 0.0, D == 3 ? (double)P[2] : 0.0};
-            energy_terms<D, sizeof(T) == 8>((double)ph, fwd, (double)pn, (double)po, k, sc, acc);
+            energy_terms<D, sizeof(T) == 8, ENERGY == 2>((double)ph, fwd, (double)pn, (double)po, k, sc, acc);
         }
     }
     if constexpr (ENERGY) energy_partials(acc, k, sc, partials + (long long)blockIdx.x * NQ);
@@ -264,7 +264,8 @@ static void step_simple_t(const SimConst &k, int lagged, int energy, const StepP
     dim3 g(cfg.grid), b(cfg.block);
     CB_DISPATCH_D(k.D,
         if (lagged) {
-            if (energy) k_step_simple<T, DD, true, true><<<g, b, 0, st>>>(k, pin, pout, qin, qout, sc, partials);
+            if (energy && k.energy_lite) k_step_simple<T, DD, true, 2><<<g, b, 0, st>>>(k, pin, pout, qin, qout, sc, partials);
+            else if (energy) k_step_simple<T, DD, true, true><<<g, b, 0, st>>>(k, pin, pout, qin, qout, sc, partials);
             else k_step_simple<T, DD, true, false><<<g, b, 0, st>>>(k, pin, pout, qin, qout, sc, partials);
         } else {
             if (energy) k_step_simple<T, DD, false, true><<<g, b, 0, st>>>(k, pin, pout, qin, qout, sc, partials);

@@ -29,8 +29,12 @@ using cb::set_error;
 
 namespace cb {
 
-static hipError_t launch_one(cb_sim *s, bool energy)
+// lite: measure only the sums the Friedmann update needs (an expanding
+// run's steps before the last of a call)
+static hipError_t launch_one(cb_sim *s, bool energy, bool lite = false)
 {
+    SimConst k = s->k;
+    k.energy_lite = lite ? 1 : 0;
     StepPtrs p;
     p.phi_in = s->phi[s->iphi];
     p.phi_out = s->phi[s->iphi ^ 1];
@@ -39,13 +43,12 @@ static hipError_t launch_one(cb_sim *s, bool energy)
     int nblocks = s->cfg.grid;
     hipError_t e;
     if (s->tiled >= 0) {
-        e = launch_step_tiled(s->k, s->fp32, s->lagged, energy, s->tiled, p, s->dsc, s->partials, s->st,
-                              &nblocks);
+        e = launch_step_tiled(k, s->fp32, s->lagged, energy, s->tiled, p, s->dsc, s->partials, s->st, &nblocks);
     } else {
-        e = launch_step_simple(s->k, s->fp32, s->lagged, energy, p, s->dsc, s->partials, s->cfg, s->st);
+        e = launch_step_simple(k, s->fp32, s->lagged, energy, p, s->dsc, s->partials, s->cfg, s->st);
     }
     if (e != hipSuccess) return e;
-    if (energy) e = launch_finalize(s->k, s->partials, nblocks, s->dsc, s->lagged ? 0 : 1, s->st);
+    if (energy) e = launch_finalize(k, s->partials, nblocks, s->dsc, s->lagged ? 0 : 1, s->st);
     return e;
 }
 
@@ -116,8 +119,9 @@ cb_status step_n(cb_sim *s, int nsteps, bool multi1d)
         }
     }
     for (; it < nsteps; ++it) {
-        const bool energy = s->lagged || it == nsteps - 1;
-        HIP_TRY(launch_one(s, energy));
+        const bool last = it == nsteps - 1;
+        const bool energy = s->lagged || last;
+        HIP_TRY(launch_one(s, energy, s->lagged && !last));
         s->iphi ^= 1;
         if (s->lagged) s->ipi ^= 1;
         s->t += s->p.dt;

@@ -30,7 +30,7 @@ namespace cb {
 // One tile of one step: TR rows x all N columns of every plane i in
 // [lt's i-chunk].  lds holds two plane tiles (double buffer).  Ends with an
 // LDS barrier so a caller may run the next tile on the same LDS.
-template <typename T, int D, int VW, int ITEMS, int NTHR, bool LAGGED, bool ENERGY, int PF>
+template <typename T, int D, int VW, int ITEMS, int NTHR, bool LAGGED, int ENERGY, int PF>
 __device__ __forceinline__ void tile_step(const SimConst &k, const TiledGeom &g, int lt, const T *__restrict__ phi_in,
                                  T *__restrict__ phi_out, const T *pi_in, T *pi_out,
                                  const DevScalars *__restrict__ sc, double *__restrict__ partials, T *lds)
@@ -205,7 +205,7 @@ __device__ __forceinline__ void tile_step(const SimConst &k, const TiledGeom &g,
                     fwd[0] = (double)mP;
                     if constexpr (D == 3) { fwd[1] = (double)rp.v[e]; fwd[2] = (double)fP; }
                     else { fwd[1] = (double)fP; fwd[2] = 0.0; }
-                    energy_terms<D, sizeof(T) == 8>((double)c, fwd, (double)pnew, (double)po.v[e], k, sc, acc);
+                    energy_terms<D, sizeof(T) == 8, ENERGY == 2>((double)c, fwd, (double)pnew, (double)po.v[e], k, sc, acc);
                 }
             }
             store_vec<T, VW>(pi_out + base + off[u], pn, nt);
@@ -228,7 +228,7 @@ __device__ __forceinline__ void tile_step(const SimConst &k, const TiledGeom &g,
     lds_barrier();
 }
 
-template <typename T, int D, int VW, int ITEMS, int NTHR, bool LAGGED, bool ENERGY, int PF>
+template <typename T, int D, int VW, int ITEMS, int NTHR, bool LAGGED, int ENERGY, int PF>
 __global__ void __launch_bounds__(NTHR)
 k_step_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restrict__ phi_out,
              const T *pi_in, T *pi_out, const DevScalars *__restrict__ sc, double *__restrict__ partials)
@@ -571,7 +571,8 @@ static void launch_t(const SimConst &k, int lagged, int energy, const TiledCfg &
     T *qout = (T *)p.pi_out;
     if constexpr (D == 3) {
         if (lagged) {
-            if (energy) k_step_tiled<T, D, VW, ITEMS, NTHR, true, true, PF><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            if (energy && k.energy_lite) k_step_tiled<T, D, VW, ITEMS, NTHR, true, 2, PF><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            else if (energy) k_step_tiled<T, D, VW, ITEMS, NTHR, true, true, PF><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
             else k_step_tiled<T, D, VW, ITEMS, NTHR, true, false, PF><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
             return;
         }
