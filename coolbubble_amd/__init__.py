@@ -41,6 +41,7 @@ class Params(ctypes.Structure):
         ("test_wave", ctypes.c_int), ("expansion", ctypes.c_int), ("alpha", ctypes.c_int),
         ("Mpl", ctypes.c_double), ("rand_seed", ctypes.c_uint),
         ("thermal_base", ctypes.c_ulonglong), ("precision", ctypes.c_int), ("device", ctypes.c_int),
+        ("thermal_spectrum", ctypes.c_int),
     ]
 
 

@@ -474,7 +474,7 @@ struct MtRow {
 
 __global__ void __launch_bounds__(256)
 k_thermal_modes(cpx<double> *H, unsigned long long *state, long long rows, int N, int D, double L, double T,
-                double m, int velocity, unsigned long long seed)
+                double m, int velocity, unsigned long long seed, int rj)
 {
     const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
@@ -486,9 +486,20 @@ k_thermal_modes(cpx<double> *H, unsigned long long *state, long long rows, int N
         const long long s = (i <= N / 2) ? i : i - N;
         return 2.0 * PI * (double)s / L;
     };
-    double k2rows;
-    if (D == 2) { const double q = kphys(r); k2rows = q * q; }
-    else { const double q1 = kphys(r / N), q2 = kphys(r % N); k2rows = q1 * q1 + q2 * q2; }
+    // Rayleigh-Jeans option: the lattice momentum of mode s along an axis,
+    // (2 / dx) sin(pi s / N), the eigenvalue of the force's Laplacian
+    const double tdx = 2.0 * (double)N / L;
+    auto klat = [&](long long i) -> double { return tdx * sin(PI * (double)i / (double)N); };
+    double k2rows, kl2rows;
+    if (D == 2) {
+        const double q = kphys(r), ql = klat(r);
+        k2rows = q * q;
+        kl2rows = ql * ql;
+    } else {
+        const double q1 = kphys(r / N), q2 = kphys(r % N), l1 = klat(r / N), l2 = klat(r % N);
+        k2rows = q1 * q1 + q2 * q2;
+        kl2rows = l1 * l1 + l2 * l2;
+    }
     bool have = false;
     double saved = 0.0;
     auto normal = [&]() -> double {
@@ -505,11 +516,18 @@ k_thermal_modes(cpx<double> *H, unsigned long long *state, long long rows, int N
         return y * mult;
     };
     for (int j = 0; j < nc; ++j) {
-        const double kj = 2.0 * PI * j / L;
-        const double k2 = k2rows + kj * kj;
-        const double omega = sqrt(k2 + m * m);
-        const double n_k = 1.0 / (exp(omega / T) - 1.0);
-        const double P = velocity ? n_k * omega : n_k / omega;
+        double P;
+        if (rj) {
+            const double lj = klat(j);
+            const double w2 = kl2rows + lj * lj + m * m;
+            P = velocity ? T : T / w2;
+        } else {
+            const double kj = 2.0 * PI * j / L;
+            const double k2 = k2rows + kj * kj;
+            const double omega = sqrt(k2 + m * m);
+            const double n_k = 1.0 / (exp(omega / T) - 1.0);
+            P = velocity ? n_k * omega : n_k / omega;
+        }
         const double sigma = sqrt(P / 2.);
         const double re = sigma * (normal() * 1.0 + 0.0);
         const double im = sigma * (normal() * 1.0 + 0.0);
@@ -1074,7 +1092,7 @@ cb_status thermal_field(cb_sim *s, void *field, double T, double m, int velocity
     const unsigned long long seed = (unsigned long long)s->p.thermal_base + off;
     const unsigned nb = (unsigned)((rows + 255) / 256);
     k_thermal_modes<<<nb, 256, 0, s->st>>>((cpx<double> *)H, (unsigned long long *)state, rows, N, D, s->p.L, T, m,
-                                           velocity, seed);
+                                           velocity, seed, s->p.thermal_spectrum == 1 ? 1 : 0);
     k_hermitian<<<nb, 256, 0, s->st>>>((cpx<double> *)H, rows, N, D);
     const int CB = batch_for(N, sizeof(cpx<double>));
     const size_t shm = 2 * (size_t)CB * N * sizeof(cpx<double>);

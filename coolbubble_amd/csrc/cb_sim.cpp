@@ -106,6 +106,7 @@ bool valid_params(const cb_params *p, std::string &why)
     double n = std::pow((double)p->nGrids, p->D);
     if (n > 4.0e10) { why = "lattice too large"; return false; }
     if (p->expansion && p->D == 3 && !(p->Mpl > 0)) { why = "Mpl must be positive"; return false; }
+    if (p->thermal_spectrum != 0 && p->thermal_spectrum != 1) { why = "thermal_spectrum must be 0 or 1"; return false; }
     return true;
 }
 
@@ -260,7 +261,9 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
         for (size_t b = 0; b + 2 < s->xns.size(); b += 3) add(s->xns[b], s->xns[b + 1], s->xns[b + 2], 0.0, 0.0, 0.0);
     } else {                                       // readxns == -3: thermal fluctuations
         bubbles = false;
-        double Temp = p.Trad * s->mT;
+        // the reference's temperature is Trad in units of the true-vacuum
+        // mass (src/bubble.cpp:223); the Rayleigh-Jeans option takes T = Trad
+        double Temp = p.thermal_spectrum == 1 ? p.Trad : p.Trad * s->mT;
         double mF = std::sqrt(2 * c);
         cb_status r = cb::thermal_field(s, s->phi[0], Temp, mF, 0);
         if (r != CB_OK) return r;

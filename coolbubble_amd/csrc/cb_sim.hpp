@@ -41,6 +41,9 @@ struct cb_sim {
     int max_blocks = 0;         // capacity of `partials`, in blocks
     int single_variant = -1;    // one-step variant used for odd steps when `variant` fuses two
     int tiled_energy = -1;      // registry index of the energy launches of a multi-step variant
+    int tiled_pair = -1;        // registry index of the two-step tile a 1024-thread three-step variant
+                                // ends a call with (-1: none)
+    bool pair_energy = false;   // that two-step tile also measures the energies (one vector per thread)
     double t = 0.0;
     long long step = 0;
     std::vector<double> xns;    // centres (the __xns.dat rows)

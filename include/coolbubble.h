@@ -61,6 +61,12 @@ typedef struct cb_params {
     unsigned long long thermal_base; /* thermal seed base (12345: src/bubble.cpp:137) */
     int precision;               /* cb_precision (no counterpart: the reference is FP64) */
     int device;                  /* HIP device ordinal */
+    int thermal_spectrum;        /* mode -3 occupancy: 0 the reference's Bose-Einstein spectrum
+                                    (GenerateThermalFluc, src/bubble.cpp:116-130,219-228: <|phi_k|^2> =
+                                    n_k / omega, <|pi_k|^2> = n_k omega, continuum k, T = Trad mT);
+                                    1 Rayleigh-Jeans (no counterpart: <|phi_k|^2> = T / omega^2,
+                                    <|pi_k|^2> = T, omega^2 = k_lat^2 + 2c with the lattice momentum
+                                    k_lat^2 = sum (2/dx)^2 sin^2(k dx / 2), T = Trad) */
 } cb_params;
 
 /* The 16 columns of one EnergyDensities.dat row (outEP, src/bubble.cpp:977-989):

@@ -112,3 +112,11 @@ def test_run_defaults_are_the_references():
     assert cb.lib().cb_default_run_opts(ctypes.byref(o)) == 0
     assert (o.nt, o.snapshots, o.snapshot_spacing, o.energy_every, o.fft, o.full1d_every) == (1000, 20, 0, 40, 1, 100)
     assert (o.field_snapshots, o.memorysave, o.deflate, o.wall_tracking, o.wall_track_every) == (1, 0, 9, 0, 400)
+
+
+def test_thermal_spectrum_option_is_validated():
+    p = cb.default_params(thermal_spectrum=2)
+    assert cb.lib().cb_check_stability(ctypes.byref(p)) == 1        # CB_INVALID_ARG
+    for v in (0, 1):
+        p = cb.default_params(thermal_spectrum=v)
+        assert cb.lib().cb_check_stability(ctypes.byref(p)) == 0

@@ -419,3 +419,39 @@ def test_slab_probe_times_one_ranks_share():
     with pytest.raises(cb.CBError):
         e.slab_probe(32, 1)
     e.close()
+
+
+@pytest.mark.parametrize("D,N,L", [(3, 32, 8.0), (2, 256, 16.0)])
+def test_rayleigh_jeans_thermal_option_spectrum(D, N, L):
+    """Build option thermal_spectrum = 1 (no reference counterpart): the
+    seeded modes of phi have <|phi_k|^2> = T / omega^2 with T = Trad and the
+    lattice frequency omega^2 = sum_i (2/dx)^2 sin^2(pi s_i / N) + 2c.  The
+    modes are recovered from phi at t = 0 (phi = L^(-D/2) c2r(modes)); each
+    |phi_k|^2 / P(k) is exponentially distributed with mean 1, so its mean
+    over the generic modes (the planes j = 0 and N/2 are symmetrised) and
+    over the lower and upper halves of the momenta must be 1 within 5 sigma.
+    The continuum momentum instead of the lattice one, or T = Trad mT, misses
+    the upper half by far more."""
+    c, Trad = 2.0, 0.5
+    g = cb.Bubble(precision="fp64", D=D, nGrids=N, L=L, dt=0.2 * L / N, c=c, readxns=-3, Trad=Trad,
+                  thermal_spectrum=1)
+    phi, _ = g.fields()
+    X = np.fft.rfftn(phi) * L ** (D / 2) / N ** D
+    s = np.fft.fftfreq(N, 1.0 / N)
+    dx = L / N
+    kl = (2 / dx) * np.sin(np.pi * s / N)
+    grids = np.meshgrid(*([kl] * (D - 1) + [kl[:N // 2 + 1]]), indexing="ij")
+    w2 = sum(q * q for q in grids) + 2 * c
+    P = Trad / w2
+    ratio = (np.abs(X) ** 2 / P)[..., 1:N // 2]
+    w2g = w2[..., 1:N // 2]
+    assert abs(ratio.mean() - 1.0) <= 5.0 / np.sqrt(ratio.size), ratio.mean()
+    cut = np.median(w2g)
+    for band in (ratio[w2g <= cut], ratio[w2g > cut]):
+        assert abs(band.mean() - 1.0) <= 5.0 / np.sqrt(band.size), band.mean()
+    # against the continuum momentum (k >= k_lat) the upper half is 15 % or more too strong
+    kc = 2 * np.pi * s / L
+    gc = np.meshgrid(*([kc] * (D - 1) + [kc[:N // 2 + 1]]), indexing="ij")
+    Pc = (Trad / (sum(q * q for q in gc) + 2 * c))[..., 1:N // 2]
+    up = w2g > cut
+    assert (np.abs(X[..., 1:N // 2]) ** 2 / Pc)[up].mean() > 1.15
