@@ -155,28 +155,38 @@ __device__ inline double lap_term(double fl_dd, const SimConst &k)
 // keeps the reference's separately rounded operations).  a = the pair along
 // axis 0, b = along the fastest axis, c = along axis 1 of a 3D lattice; every
 // kernel sums them in this order, so all variants agree bit for bit.
+// FP32 leapfrog constants of one step.  The kick is written so that each
+// site costs 2D - 1 adds and five fused multiply-adds:
+//   g = s - x (A + x (B + x C)),  s = the sum of the 2D neighbours,
+//   pi' = pi + P g,               phi' = phi + dt pi',
+// with P = dt lapc, A = 2D + fp c2 / lapc, B = -fp c3 / lapc, C = fp c4 / lapc
+// (lapc = a^(D-2+alpha) / dx^2, fp = a^(D+alpha)): P g is dt times the
+// force lapc (s - 2D x) - fp V'(x), V'(x) = x (c2 + x (-c3 + x c4)).
 struct StepF32 {
-    float lapc;     // a^(D-2+alpha) / dx^2
-    float fac_pot;  // a^(D+alpha)
-    float c2, c3, c4, dt;
+    float P, A, B, C;
+    float dt;
 };
 
 __device__ inline StepF32 step_f32(const SimConst &k, const DevScalars *sc)
 {
-    return StepF32{(float)(sc->fac_lap / k.dx2), (float)sc->fac_pot, (float)k.c2, (float)k.c3, (float)k.c4,
-                   (float)k.dt};
+    const double lapc = sc->fac_lap / k.dx2, fp = sc->fac_pot;
+    return StepF32{(float)(k.dt * lapc), (float)(2.0 * k.D + fp * k.c2 / lapc), (float)(-fp * k.c3 / lapc),
+                   (float)(fp * k.c4 / lapc), (float)k.dt};
 }
 
+// pi after the kick of one site (FP32): x the field, aP/aM, bP/bM, cP/cM its
+// neighbours along the lattice axes (added in this order in every kernel,
+// so that all FP32 kernels agree bit for bit), pi the incoming momentum.
 template <int D>
-__device__ inline float force_f32(float x, float aP, float aM, float bP, float bM, float cP, float cM,
-                                  const StepF32 &f)
+__device__ inline float kick_f32(float x, float aP, float aM, float bP, float bM, float cP, float cM, float pi,
+                                 const StepF32 &f)
 {
     float s = aP + aM;
     if constexpr (D >= 2) s = s + (bP + bM);
     if constexpr (D == 3) s = s + (cP + cM);
-    const float dd = __builtin_fmaf(-2.0f * D, x, s);
-    const float vp = x * __builtin_fmaf(x, __builtin_fmaf(f.c4, x, -f.c3), f.c2);   // V'(x)
-    return __builtin_fmaf(f.lapc, dd, -(f.fac_pot * vp));
+    const float t = __builtin_fmaf(__builtin_fmaf(f.C, x, f.B), x, f.A);
+    const float g = __builtin_fmaf(-x, t, s);
+    return __builtin_fmaf(f.P, g, pi);
 }
 
 } // namespace cb

@@ -66,8 +66,7 @@ k_step_simple(SimConst k, const T *__restrict__ phi_in, T *__restrict__ phi_out,
             else phi_out[s] = (T)(ph + dt * pn);
         } else {
             // axis 0, the fastest axis, then axis 1 of 3D
-            const C F = force_f32<D>(ph, P[0], M[0], P[D - 1], M[D - 1], P[D == 3 ? 1 : 0], M[D == 3 ? 1 : 0], sf);
-            pn = __builtin_fmaf(sf.dt, F, po);
+            pn = kick_f32<D>(ph, P[0], M[0], P[D - 1], M[D - 1], P[D == 3 ? 1 : 0], M[D == 3 ? 1 : 0], po, sf);
             pi_out[s] = pn;
             if constexpr (LAGGED) phi_out[s] = ph;
             else phi_out[s] = __builtin_fmaf(sf.dt, pn, ph);
@@ -332,8 +331,7 @@ k_step1d_multi(SimConst k, const T *__restrict__ phi_in, T *__restrict__ phi_out
                     npi[r] = (T)pn;
                     nphi[r] = (T)(ph + dt * pn);
                 } else {
-                    const C F = force_f32<1>(ph, P0, M0, P0, M0, P0, M0, sf);
-                    pn = __builtin_fmaf(sf.dt, F, po);
+                    pn = kick_f32<1>(ph, P0, M0, P0, M0, P0, M0, po, sf);
                     npi[r] = pn;
                     nphi[r] = __builtin_fmaf(sf.dt, pn, ph);
                 }

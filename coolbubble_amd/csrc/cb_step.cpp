@@ -287,11 +287,13 @@ cb_status select_variant(cb_sim *s, int variant)
     s->tiled_pair = -1;
     s->pair_energy = false;
     if (variant >= 0 && s->variants[(size_t)variant].fuse == 3 && s->variants[(size_t)variant].nthr == 1024) {
-        // the two-step tile of the same shape (threads, rows, chunk, prefetch, stores)
+        // the two-step tile of the same shape (threads, rows, chunk, stores;
+        // prefetch one plane ahead at most)
         const TiledCfg &f = s->variants[(size_t)variant];
         for (size_t v = 0; v < s->variants.size(); ++v) {
             const TiledCfg &c = s->variants[v];
-            if (c.fuse == 2 && c.nthr == f.nthr && c.TR == f.TR && c.NI == f.NI && c.pf == f.pf && c.nt == f.nt) {
+            if (c.fuse == 2 && c.nthr == f.nthr && c.TR == f.TR && c.NI == f.NI && c.pf == std::min(f.pf, 2) &&
+                c.nt == f.nt) {
                 s->tiled_pair = tiled_register(c);
                 s->pair_energy = c.items == 1;
                 break;

@@ -51,7 +51,7 @@ bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c)
     if (c.items != 1 && c.items != 2 && c.items != 4) return false;
     if (c.nthr != 256 && c.nthr != 512 && c.nthr != 1024) return false;
     if (c.NI < 1 || (c.nt != 0 && c.nt != 1)) return false;
-    if (c.pf != 1 && !(c.pf == 2 && c.items == 1)) return false;   // prefetch: one vector per thread
+    if (c.pf != 1 && !((c.pf == 2 || c.pf == 3) && c.items == 1)) return false;   // prefetch: one vector per thread
     if (k.D == 3 && (c.TR < 1 || k.N % c.TR != 0)) return false;
     const int NV = k.N / c.vw;
     const int rowsA = (k.D == 3) ? c.TR + 4 : 1;
@@ -73,7 +73,10 @@ static void launch3_t(const SimConst &k, int energy, const TiledCfg &c, const St
         k_step3_tiled<T, D, ITEMS, NTHR, true, 1><<<grid, block, shm, st>>>(
             k, g, (const T *)p.phi_in, (T *)p.phi_out, (const T *)p.pi_in, (T *)p.pi_out, sc, partials);
     else if constexpr (ITEMS == 1) {
-        if (c.pf == 2)
+        if (c.pf == 3)
+            k_step3_tiled<T, D, 1, NTHR, false, 3><<<grid, block, shm, st>>>(
+                k, g, (const T *)p.phi_in, (T *)p.phi_out, (const T *)p.pi_in, (T *)p.pi_out, sc, partials);
+        else if (c.pf == 2)
             k_step3_tiled<T, D, 1, NTHR, false, 2><<<grid, block, shm, st>>>(
                 k, g, (const T *)p.phi_in, (T *)p.phi_out, (const T *)p.pi_in, (T *)p.pi_out, sc, partials);
         else
@@ -140,7 +143,7 @@ int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
         if (items < 1) items = 1;
         // (more vectors per thread than this spill registers)
         if (items > (nthr == 256 ? 4 : nthr == 512 ? 2 : 1)) return;
-        if (pf == 2 && items != 1) return;
+        if (pf >= 2 && items != 1) return;
         TiledCfg c{vw, items, nthr, TR, NI, pf, nt, 3};
         if (n < cap && step3_valid(k, fp32, c)) out[n++] = c;
     };
@@ -154,6 +157,11 @@ int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
             for (int nthr : {1024, 512, 256})
                 for (int NI : {32, 16, 64})
                     for (int nt : {1, 0}) push(nthr, TR, NI, nt, 2);
+        // loads two planes ahead (1024 threads: one block per CU, so the
+        // loads of one plane ahead land after the plane's arithmetic)
+        for (int TR : {4, 2})
+            for (int NI : {64, 32})
+                for (int nt : {1, 0}) push(1024, TR, NI, nt, 3);
     } else {
         for (int nthr : {256, 512})
             for (int NI : {16, 32, 8})

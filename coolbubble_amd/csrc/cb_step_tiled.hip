@@ -194,8 +194,7 @@ __device__ __forceinline__ void tile_step(const SimConst &k, const TiledGeom &g,
                 } else {
                     C rP = 0, rM = 0;
                     if constexpr (D == 3) { rP = rp.v[e]; rM = rm.v[e]; }
-                    const C F = force_f32<D>(c, mP, mM, fP, fM, rP, rM, sf);
-                    pnew = __builtin_fmaf(sf.dt, F, po.v[e]);
+                    pnew = kick_f32<D>(c, mP, mM, fP, fM, rP, rM, po.v[e], sf);
                     if constexpr (LAGGED) phn.v[e] = c;
                     else phn.v[e] = __builtin_fmaf(sf.dt, pnew, c);
                 }

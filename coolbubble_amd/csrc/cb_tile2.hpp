@@ -203,8 +203,7 @@ __device__ __forceinline__ void tile_step2(const SimConst &k, const TiledGeom &g
                 } else {
                     C rP = 0, rM = 0;
                     if constexpr (D == 3) { rP = rp.v[e]; rM = rm.v[e]; }
-                    const C F = force_f32<D>(c, n0[u].v[e], p0[u].v[e], fP, fM, rP, rM, sf);
-                    const C pnew = __builtin_fmaf(sf.dt, F, pi0[u].v[e]);
+                    const C pnew = kick_f32<D>(c, n0[u].v[e], p0[u].v[e], fP, fM, rP, rM, pi0[u].v[e], sf);
                     q1n[u].v[e] = pnew;
                     n1[u].v[e] = __builtin_fmaf(sf.dt, pnew, c);
                 }
@@ -243,8 +242,7 @@ __device__ __forceinline__ void tile_step2(const SimConst &k, const TiledGeom &g
                     } else {
                         C rP = 0, rM = 0;
                         if constexpr (D == 3) { rP = rp.v[e]; rM = rm.v[e]; }
-                        const C F = force_f32<D>(c, mP, mM, fP, fM, rP, rM, sf);
-                        pnew = __builtin_fmaf(sf.dt, F, q1c[u].v[e]);
+                        pnew = kick_f32<D>(c, mP, mM, fP, fM, rP, rM, q1c[u].v[e], sf);
                         phn.v[e] = __builtin_fmaf(sf.dt, pnew, c);
                     }
                     pn.v[e] = pnew;

@@ -70,8 +70,7 @@ __device__ __forceinline__ void leap_vec(const SimConst &k, const Vec<T, VW> &c,
         } else {
             C rP = 0, rM = 0;
             if constexpr (D == 3) { rP = rp.v[e]; rM = rm.v[e]; }
-            const C F = force_f32<D>(x, mP.v[e], mM.v[e], fP, fM, rP, rM, sf);
-            pnew = __builtin_fmaf(sf.dt, F, piv.v[e]);
+            pnew = kick_f32<D>(x, mP.v[e], mM.v[e], fP, fM, rP, rM, piv.v[e], sf);
             phout.v[e] = __builtin_fmaf(sf.dt, pnew, x);
         }
         pout.v[e] = pnew;
@@ -166,6 +165,7 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
     V q1c[ITEMS], q1n[ITEMS];               // pi_n+1/2 at planes p-1, p
     V q2c[ITEMS], q2n[ITEMS];               // pi_n+3/2 at planes p-2, p-1
     nvec nx[ITEMS], pr[ITEMS], hr[ITEMS];
+    nvec nx2[ITEMS], pr2[ITEMS], hr2[ITEMS];   // PF == 3: the loads of the plane after
 
     double acc[NQ];
 #pragma unroll
@@ -181,7 +181,7 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
                 f0[0][u] = *reinterpret_cast<const V *>(phi_in + bm2 + off[u]);
             }
     }
-    if constexpr (PF == 2) {
+    if constexpr (PF >= 2) {
         const long long b0 = (long long)wrapN(i0 - 2) * NN, b1 = (long long)wrapN(i0 - 1) * NN;
 #pragma unroll
         for (int u = 0; u < ITEMS; ++u) {
@@ -193,6 +193,18 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
             if (D == 3 && hok[u]) hr[u] = *reinterpret_cast<const nvec *>(phi_in + b0 + hoff[u]);
         }
     }
+    if constexpr (PF == 3) {
+        const long long b1 = (long long)wrapN(i0 - 1) * NN, b2 = (long long)wrapN(i0) * NN;
+#pragma unroll
+        for (int u = 0; u < ITEMS; ++u) {
+            nx2[u] = pr2[u] = hr2[u] = (nvec)(T)0;
+            if (ok[u]) {
+                nx2[u] = *reinterpret_cast<const nvec *>(phi_in + b2 + off[u]);
+                pr2[u] = *reinterpret_cast<const nvec *>(pi_in + b1 + off[u]);
+            }
+            if (D == 3 && hok[u]) hr2[u] = *reinterpret_cast<const nvec *>(phi_in + b1 + hoff[u]);
+        }
+    }
 
     // one plane p, with S3 = t % 3
     auto plane = [&](auto s3, int p) __attribute__((always_inline)) {
@@ -201,7 +213,29 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
         const long long base = (long long)wrapN(p) * NN;
         const long long baseN = (long long)wrapN(p + 1) * NN;
         V pi0[ITEMS], hv[ITEMS];
-        if constexpr (PF == 2) {
+        if constexpr (PF == 3) {
+            // plane p+1's values were loaded two planes ago, plane p+2's one
+            // plane ago; plane p+3's loads go out now (p + 3 <= i1 + 4 <= N + 4:
+            // wrapN keeps every plane in the box)
+            const long long bq2 = (long long)wrapN(p + 3) * NN, bp2 = (long long)wrapN(p + 2) * NN;
+#pragma unroll
+            for (int u = 0; u < ITEMS; ++u) {
+#pragma unroll
+                for (int e = 0; e < VW; ++e) {
+                    f0[NX][u].v[e] = nx[u][e];
+                    pi0[u].v[e] = pr[u][e];
+                    hv[u].v[e] = hr[u][e];
+                }
+                nx[u] = nx2[u];
+                pr[u] = pr2[u];
+                hr[u] = hr2[u];
+                if (ok[u]) {
+                    nx2[u] = *reinterpret_cast<const nvec *>(phi_in + bq2 + off[u]);
+                    pr2[u] = *reinterpret_cast<const nvec *>(pi_in + bp2 + off[u]);
+                }
+                if (D == 3 && hok[u]) hr2[u] = *reinterpret_cast<const nvec *>(phi_in + bp2 + hoff[u]);
+            }
+        } else if constexpr (PF == 2) {
             // plane p+1's loads (p + 2 <= i1 + 3 <= N + 3: wrapN keeps every plane in the box)
             const long long bq1 = (long long)wrapN(p + 2) * NN;
 #pragma unroll
@@ -236,7 +270,7 @@ __device__ __forceinline__ void tile_step3(const SimConst &k, const TiledGeom &g
 #pragma unroll
             for (int u = 0; u < ITEMS; ++u)
                 if (hok[u]) {
-                    if constexpr (PF == 2) *reinterpret_cast<V *>(L0 + hlds[u]) = hv[u];
+                    if constexpr (PF >= 2) *reinterpret_cast<V *>(L0 + hlds[u]) = hv[u];
                     else *reinterpret_cast<V *>(L0 + hlds[u]) = *reinterpret_cast<const V *>(phi_in + base + hoff[u]);
                 }
         }
