@@ -314,17 +314,14 @@ struct BinArgs {
 };
 
 // Columns per block: RND rounds of kSeq<N> sequences (about 32 KiB of data
-// per block keeps 4-5 blocks resident per CU with 64-byte row segments;
+// per block keeps 3-4 blocks resident per CU with 64-byte row segments;
 // 128-byte segments at 64-70 KiB per block measured slower: 1.19 against
-// 0.89 ms for the 512^3 FP32 spectrum).
+// 0.89 ms for the 512^3 FP32 spectrum).  The binning pass takes two rounds
+// too for float data (N >= 16, so that a thread's rows stay within a column):
+// with one, its row segments are 32 bytes.
 template <typename R, int N, bool BIN>
-constexpr int kColRounds = (!BIN && sizeof(R) == 4) ? 2 : 1;
+constexpr int kColRounds = (sizeof(R) == 4 && (!BIN || N >= 16)) ? 2 : 1;
 constexpr size_t kLdsBlockMax = 160 * 1024;   // gfx950: LDS a block may use
-
-// the binning pass's run table fits the data area of its columns
-template <typename R, int N>
-constexpr bool kRunInData = (size_t)2 * kSeq<N> * (N / 2 + 1) * sizeof(double) <=
-                            (size_t)kSeq<N> * kColStride<R, N> * sizeof(cpx<R>);
 
 template <typename R, int N, bool BIN>
 __global__ void __launch_bounds__(256)
@@ -336,19 +333,19 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
     const cpx<R> *twl = stage_twiddles<R, N>(smem_raw, tw);
     cpx<R> *buf = data_area<R, N>(smem_raw);
     // binning: the block's histogram, the per-thread run records and the
-    // bin range of each (column, half) line
+    // bin range of each column
     double *hist = reinterpret_cast<double *>(buf + CB * BS);
     double *rec_fs = hist + ba.nBins;
     int *rec_fk = reinterpret_cast<int *>(rec_fs + 256);
     int *rec_lk = rec_fk + 256;
     int *rec_one = rec_lk + 256;
-    int *line_lo = rec_one + 256;          // [2 CB] lowest bin of the line
-    int *line_hi = line_lo + 2 * CB;       // [2 CB] highest bin, -1: none
-    // run totals [2 CB][SPAN] by (line, bin - lowest bin): in the data area
-    // (free once an item's values are read) where it fits, else after it all
-    constexpr int SPAN = N / 2 + 1;        // bins a line can meet
-    double *runtab = kRunInData<R, N> ? reinterpret_cast<double *>(buf)
-                                      : reinterpret_cast<double *>(line_hi + 2 * CB + 2 * (CB & 1));
+    int *line_lo = rec_one + 256;          // [CB] bin at a = 0
+    int *line_hi = line_lo + CB;           // [CB] highest bin binned, -1: none
+    // run totals [CB][SPAN] by (column, bin - lowest bin), in the data area
+    // (free once an item's values are read)
+    constexpr int SPAN = N / 2 + 1;        // bins a column can meet
+    static_assert((size_t)CB * SPAN * sizeof(double) <= (size_t)CB * BS * sizeof(cpx<R>), "run table fits");
+    double *runtab = reinterpret_cast<double *>(buf);
     if constexpr (BIN)
         for (int b = threadIdx.x; b < ba.nBins; b += 256) hist[b] = 0.0;
     const long long tiles = (inner + CB - 1) / CB;
@@ -385,96 +382,120 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
                 if (cc < ncol) base[(long long)e * inner + cc] = buf[cc * BS + pidx(e)];
             }
         } else {
-            // Binning in a fixed order, no atomics.  Thread t takes E = 8
-            // consecutive rows e of one column (CB N = 2048 modes per item).
-            // Along a column the bin never decreases with |s_e| in each half
-            // of axis 0 (s_e = e up to N/2, e - N above): such a line meets
-            // every bin from its lowest to its highest once, as one
-            // contiguous run of rows.  Each thread sums the runs in its
-            // rows; a run that crosses into the next threads' rows is
-            // completed by the thread where it starts, from the next
-            // threads' first partial sums (in row order).  The run totals go
-            // to a table indexed by (line, bin - lowest bin of the line), in
+            // Binning in a fixed order, no atomics.  The modes of a column at
+            // s_e = +a and -a (rows a and N - a) have the same |k|, so the
+            // same bin and weight: thread t takes A consecutive values of a
+            // (0 <= a < N/2) in one column, both rows of each, and the last
+            // thread of a column also a = N/2.  Along a column the bin never
+            // decreases with a and grows by at most one per step, so the
+            // column meets every bin from its lowest to its highest once, as
+            // one contiguous run of a.  Each thread sums the runs in its
+            // a-range; a run that goes on into the next threads' ranges is
+            // completed by the thread where it starts, from the next threads'
+            // first partial sums (in order of a).  The run totals go to a
+            // table indexed by (column, bin - lowest bin of the column), in
             // the data area (its values are read by then); the thread that
-            // owns a bin in the block's histogram then adds that bin's
-            // totals line by line, in a fixed order.  kc^D is a product of D
-            // factors (within 2 ulp of the reference's pow(kc, D)).
-            constexpr int E = 8;
-            static_assert(CB * N == 256 * E, "eight modes per thread");
+            // owns a bin in the block's histogram then adds that bin's totals
+            // column by column, in a fixed order.  Bins are integer square
+            // roots, found once per thread and then stepped with integer
+            // compares.  The weight kc^D is a product of D factors in double
+            // for double data (within 2 ulp of the reference's pow(kc, D));
+            // for float data |k|^D is formed in float (a few float roundings,
+            // far inside the error of the float transform).
+            constexpr int TPC = 256 / CB;          // threads per column
+            constexpr int A = N / (2 * TPC);       // values of a per thread
+            static_assert(A >= 1 && TPC * A * 2 == N, "a-ranges tile the half column");
             const int t = threadIdx.x;
-            const int cc = (t * E) / N, e0 = (t * E) % N;
+            const int cc = t / TPC, kt = t - cc * TPC, a0 = kt * A;
+            const bool lastk = kt == TPC - 1;
             const int c = (int)c0 + cc;
             const int j = ba.D == 3 ? c % kNcp<N> : c;
             const int i1 = ba.D == 3 ? c / kNcp<N> : 0;
             const int s1 = (i1 <= N / 2) ? i1 : i1 - N;
             const bool colok = cc < ncol && j < nc;
-            const long long C = (long long)s1 * s1 + (long long)j * j;
+            const int C = s1 * s1 + j * j;
             const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
-            int key[E];
-            double val[E];
-#pragma unroll
-            for (int u = 0; u < E; ++u) {
-                const int e = e0 + u;
-                const int half = e > N / 2 ? 1 : 0;
-                const long long a = half ? N - e : e;
-                const double n = sqrt((double)(C + a * a));
-                const int bin = (int)n;
-                const double kc = n * ba.dk;
-                const double kd = (ba.D == 3) ? kc * kc * kc : kc * kc;
-                const cpx<R> f = buf[cc * BS + pidx(e)];
-                const double fx = (double)f.x, fy = (double)f.y;
-                val[u] = w * ba.pref * kd * ba.Pnorm * (fx * fx + fy * fy);
-                key[u] = (colok && bin < ba.nBins) ? 2 * bin + half : -1;
+            const double wc = sizeof(R) == 4 ? w * ba.pref * ba.Pnorm * (ba.D == 3 ? ba.dk * ba.dk * ba.dk : ba.dk * ba.dk)
+                                             : w * ba.pref * ba.Pnorm;
+            int key[A + 1];
+            double val[A + 1];
+            int b;
+            {
+                const int m = C + a0 * a0;
+                b = (int)__builtin_amdgcn_sqrtf((float)m);
+                b -= b * b > m ? 1 : 0;
+                b += (b + 1) * (b + 1) <= m ? 1 : 0;
             }
-            if (e0 == 0) {   // the column's two lines: half 0 (|s_e| = 0..N/2), half 1 (1..N/2-1)
-                const int lo0 = (int)sqrt((double)C), hi0 = (int)sqrt((double)(C + (long long)(N / 2) * (N / 2)));
-                const int lo1 = (int)sqrt((double)(C + 1)), hi1 = (int)sqrt((double)(C + (long long)(N / 2 - 1) * (N / 2 - 1)));
-                line_lo[2 * cc] = lo0;
-                line_hi[2 * cc] = colok ? min(hi0, ba.nBins - 1) : -1;
-                line_lo[2 * cc + 1] = lo1;
-                line_hi[2 * cc + 1] = colok ? min(hi1, ba.nBins - 1) : -1;
-            }
-            // runs of equal keys in the thread's rows (keys are monotone in a
-            // half): val[u] becomes the sum of the run from row u to its end
-            // in these rows (static indices only: no register arrays indexed
-            // at run time)
+            const int bfirst = b;
 #pragma unroll
-            for (int u = E - 2; u >= 0; --u)
+            for (int u = 0; u <= A; ++u) {
+                const int a = a0 + u;
+                const int m = C + a * a;
+                if (u > 0) b += (b + 1) * (b + 1) <= m ? 1 : 0;
+                const bool in = u < A || lastk;
+                const cpx<R> f0 = buf[cc * BS + pidx(a)];
+                double p = (double)f0.x * (double)f0.x + (double)f0.y * (double)f0.y;
+                if (a > 0 && a < N / 2) {
+                    const cpx<R> f1 = buf[cc * BS + pidx(N - a)];
+                    p += (double)f1.x * (double)f1.x + (double)f1.y * (double)f1.y;
+                }
+                double kd;
+                if constexpr (sizeof(R) == 4) {
+                    const float mf = (float)m;
+                    kd = (double)(ba.D == 3 ? mf * __builtin_amdgcn_sqrtf(mf) : mf);
+                } else {
+                    const double kc = sqrt((double)m) * ba.dk;
+                    kd = ba.D == 3 ? kc * kc * kc : kc * kc;
+                }
+                val[u] = in ? wc * kd * p : 0.0;
+                key[u] = (in && colok && b < ba.nBins) ? b : -1;
+            }
+            // the column's bin range: lowest at a = 0, highest at a = N/2
+            // (none binned when lowest > highest)
+            if (kt == 0) line_lo[cc] = bfirst;
+            if (lastk) line_hi[cc] = colok ? min(b, ba.nBins - 1) : -1;
+            // runs of equal keys: val[u] becomes the sum of the run from u to
+            // its end in this thread's range (static indices only)
+#pragma unroll
+            for (int u = A - 1; u >= 0; --u)
                 if (key[u] >= 0 && key[u + 1] == key[u]) val[u] += val[u + 1];
-            const bool starts = key[0] >= 0, ends = key[E - 1] >= 0;
+            const bool starts = key[0] >= 0, ends = key[A - 1] >= 0;
             bool one = starts;
 #pragma unroll
-            for (int u = 1; u < E; ++u) one = one && key[u] == key[0];
+            for (int u = 1; u < A; ++u) one = one && key[u] == key[0];
             rec_fk[t] = starts ? key[0] : -1;
             rec_fs[t] = starts ? val[0] : 0.0;
-            rec_lk[t] = ends ? key[E - 1] : -1;
+            rec_lk[t] = ends ? key[A - 1] : -1;
             rec_one[t] = one ? 1 : 0;
-            __syncthreads();   // records and line ranges written; the data area is free
-            const int tcol0 = (t / (N / E)) * (N / E);     // first thread of this column
+            __syncthreads();   // records and bin ranges written; the data area is free
+            const int tcol0 = cc * TPC;
             // the first run is owned by a previous thread when it goes on from there
-            const bool cont = starts && t > tcol0 && rec_lk[t - 1] == key[0];
-            // the last run's part in the following threads' rows
+            const bool cont = starts && kt > 0 && rec_lk[t - 1] == key[0];
+            // the last run's part in the following threads' ranges
             double ext = 0.0;
-            if (ends && !(one && cont))
-                for (int u2 = t + 1; u2 < tcol0 + N / E && rec_fk[u2] == key[E - 1]; ++u2) {
+            if (ends && !lastk && !(one && cont))
+                for (int u2 = t + 1; u2 < tcol0 + TPC && rec_fk[u2] == key[A - 1]; ++u2) {
                     ext += rec_fs[u2];
                     if (!rec_one[u2]) break;
                 }
+            const int lo = line_lo[cc];
 #pragma unroll
-            for (int u = 0; u < E; ++u) {
+            for (int u = 0; u <= A; ++u) {
                 const bool head = key[u] >= 0 && (u == 0 || key[u - 1] != key[u]);
                 if (!head || (u == 0 && cont)) continue;
-                const int line = 2 * cc + (key[u] & 1);
-                const bool last = ends && key[u] == key[E - 1];
-                runtab[line * SPAN + (key[u] >> 1) - line_lo[line]] = last ? val[u] + ext : val[u];
+                const bool last = u < A && ends && key[u] == key[A - 1];
+                runtab[cc * SPAN + key[u] - lo] = last ? val[u] + ext : val[u];
             }
             __syncthreads();   // run table complete
             const int bpt = bins_per_thread(ba.nBins, 256);
-            for (int b = t * bpt; b < t * bpt + bpt && b < ba.nBins; ++b) {
-                double acc = hist[b];
-                for (int line = 0; line < 2 * CB; ++line)
-                    if (b >= line_lo[line] && b <= line_hi[line]) acc += runtab[line * SPAN + b - line_lo[line]];
-                hist[b] = acc;
+            for (int bb = t * bpt; bb < t * bpt + bpt && bb < ba.nBins; ++bb) {
+                double acc = hist[bb];
+#pragma unroll 4
+                for (int col = 0; col < CB; ++col) {
+                    const int l0 = line_lo[col];
+                    if (bb >= l0 && bb <= line_hi[col]) acc += runtab[col * SPAN + bb - l0];
+                }
+                hist[bb] = acc;
             }
         }
     }
@@ -572,8 +593,7 @@ hipError_t cols_bin_n(const void *H, long long inner, const void *tw, BinArgs ba
     constexpr int CB = kCols<R, N, true>;
     constexpr size_t CBn = kCols<R, N, true>;
     const size_t shm = cols_shm<R, N, true>() + sizeof(double) * ((size_t)ba.nBins + 256) +
-                       sizeof(int) * (3 * 256 + 4 * CBn + 2 * (CBn & 1)) +
-                       (kRunInData<R, N> ? 0 : sizeof(double) * 2 * CBn * (N / 2 + 1));
+                       sizeof(int) * (3 * 256 + 2 * CBn);
     hipError_t e = allow_lds(k_p2_cols<R, N, true>, shm);
     if (e != hipSuccess) return e;
     const long long items = (inner + CB - 1) / CB;
