@@ -621,9 +621,11 @@ struct FftPlan {
     long long rows = 0;
     int nBins = 0, binblocks = 0;
     double *binpart = nullptr, *bins = nullptr;
-    // power-of-two D = 2 / 3: binning fused into the last column pass, the
-    // field-independent weights and weighted k of each bin computed once
-    // into bins[nBins, 3 nBins) (the pass itself writes bins[0, nBins))
+    // power-of-two D = 2 / 3: H transposed ([N/2+1][N][ld] or [N/2+1][ld],
+    // ld = p2_pitch(N, fp32)),
+    // binning fused into the last pass, the field-independent weights and
+    // weighted k of each bin computed once into bins[nBins, 3 nBins) (the
+    // pass itself writes bins[0, nBins))
     bool fused = false;
     int fused_blocks = 0;
     // long 1D rows (four-step): N = N1 N2
@@ -718,8 +720,8 @@ static hipError_t forward_r2c(const Stages &st, const cpx<R> *tw, const T *field
     if (p2_supported(N)) {
         const int f32 = sizeof(R) == 4;
         hipError_t e = p2_rows_r2c(f32, field, H, rows, nc, N, tw, s);
-        if (e == hipSuccess && D == 3) e = p2_cols(f32, H, N, nc, N, tw, -1, s);        // axis 1
-        if (e == hipSuccess && D >= 2) e = p2_cols(f32, H, 1, rows / N * nc, N, tw, -1, s); // axis 0
+        if (e == hipSuccess && D == 3) e = p2_cols(f32, H, N, nc, nc, N, tw, -1, s);        // axis 1
+        if (e == hipSuccess && D >= 2) e = p2_cols(f32, H, 1, rows / N * nc, rows / N * nc, N, tw, -1, s); // axis 0
         return e;
     }
     const int RB = batch_for(N, sizeof(cpx<R>));
@@ -790,23 +792,6 @@ void fft_destroy(FftPlan *p)
         }                                                                                \
     } while (0)
 
-// Clears n 8-byte words (a kernel rather than hipMemsetAsync, so a caller
-// may capture the launch into a graph).
-__global__ void __launch_bounds__(256) k_zero8(unsigned long long *p, long long n)
-{
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ULL;
-}
-
-static hipError_t zero_words(void *p, size_t bytes, hipStream_t st)
-{
-    const long long n = (long long)(bytes / 8);
-    if (n <= 0) return hipSuccess;
-    const long long blocks = std::min<long long>((n + 255) / 256, 8192);
-    k_zero8<<<(unsigned)blocks, 256, 0, st>>>((unsigned long long *)p, n);
-    return hipGetLastError();
-}
-
 // Shape of the transforms for s (no device work): factorisation, four-step
 // split of long 1D rows, fused binning, bin and block counts.
 static cb_status plan_shape(const cb_sim *s, FftPlan *p)
@@ -855,14 +840,14 @@ static cb_status plan_shape(const cb_sim *s, FftPlan *p)
 }
 
 // Device buffers of a shaped plan (hipMalloc, or carved from arena), its
-// twiddles (host staging in stage, or a temporary), the padding of the fused
-// path's array cleared, and the fused path's per-bin weights and weighted k.
+// twiddles (host staging in stage, or a temporary), and the fused path's
+// per-bin weights and weighted k.
 static hipError_t plan_buffers(FftPlan *p, Arena *arena, void *stage, double L, hipStream_t st)
 {
     const int N = p->N, D = p->D, nc = N / 2 + 1;
     const size_t csz = p->fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
-    const size_t ldh = p->fused ? (size_t)p2_row_stride(N) : (size_t)nc;
-    const size_t hbytes = csz * (p->N1 ? (size_t)N : (size_t)p->rows * ldh);
+    const size_t hbytes = csz * (p->N1 ? (size_t)N
+                                       : p->fused ? (size_t)p->rows / N * nc * p2_pitch(N, p->fp32) : (size_t)p->rows * nc);
     size_t part = 3 * (size_t)p->nBins * p->binblocks;
     if (p->fused) part = std::max(part, (size_t)p->nBins * p->fused_blocks);   // the fused pass's partial rows
     // every buffer first, so that a scratch too small fails before any launch
@@ -876,8 +861,6 @@ static hipError_t plan_buffers(FftPlan *p, Arena *arena, void *stage, double L, 
     e = tw(N, &p->tw);
     if (e == hipSuccess && p->N1) e = tw(p->N1, &p->tw1);
     if (e == hipSuccess && p->N1) e = tw(p->N2, &p->tw2);
-    // the fused path pads rows to p2_row_stride(N); the padding stays zero
-    if (e == hipSuccess && p->fused) e = zero_words(p->H, hbytes, st);
     if (e == hipSuccess && p->fused) {
         const int nB = p->nBins;
         const BinShape bs = bin_shape(N, nB);
@@ -925,12 +908,15 @@ static cb_status spectrum_run(cb_sim *s, FftPlan *p, double *h, double *kmean, d
     const double pref = (D == 1) ? 1.0 / PI : (D == 2) ? 1.0 / (2.0 * PI) : 1.0 / (2.0 * PI * PI);
     const double Pnorm = s->dxD * s->dxD / std::pow(L, D);
     if (p->fused) {
-        // rows r2c, (3D) axis 1, then axis 0 fused with the binning
-        const int f32 = s->fp32 ? 1 : 0, ncp = p2_row_stride(N);
-        HIP_TRY(p2_rows_r2c(f32, field, p->H, p->rows, ncp, N, p->tw, s->st));
-        if (D == 3) HIP_TRY(p2_cols(f32, p->H, N, ncp, N, p->tw, -1, s->st));
+        // rows r2c -> H [j][i1][i0] (3D) or [j][i0] (2D); (3D) along i1 in
+        // place; along i0 fused with the binning
+        const int f32 = s->fp32 ? 1 : 0;
+        const long long nc = N / 2 + 1;
+        const long long ld = p2_pitch(N, f32);
+        HIP_TRY(p2_rows_r2c_t(f32, field, p->H, D == 3 ? N : 1, ld, N, p->tw, s->st));
+        if (D == 3) HIP_TRY(p2_cols(f32, p->H, nc, ld, N, N, p->tw, -1, s->st));
         int nblk = 0;
-        HIP_TRY(p2_cols_bin(f32, p->H, p->rows / N * ncp, N, p->tw, D, nB, dk, pref, Pnorm, p->binpart,
+        HIP_TRY(p2_rows_bin(f32, p->H, p->rows / N * nc, ld, N, p->tw, D, nB, dk, pref, Pnorm, p->binpart,
                             p->fused_blocks, &nblk, s->st));
         k_bin_reduce<<<(nB + 63) / 64, 256, 0, s->st>>>(p->binpart, nblk, nB, p->bins);
         HIP_TRY(hipGetLastError());
@@ -1098,8 +1084,8 @@ cb_status thermal_field(cb_sim *s, void *field, double T, double m, int velocity
     const size_t shm = 2 * (size_t)CB * N * sizeof(cpx<double>);
     const double norm = 1.0 / std::sqrt(std::pow(s->p.L, D));
     if (p2_supported(N)) {
-        e = p2_cols(0, H, 1, rows / N * nc, N, tw, +1, s->st);                 // axis 0
-        if (e == hipSuccess && D == 3) e = p2_cols(0, H, N, nc, N, tw, +1, s->st);   // axis 1
+        e = p2_cols(0, H, 1, rows / N * nc, rows / N * nc, N, tw, +1, s->st);                 // axis 0
+        if (e == hipSuccess && D == 3) e = p2_cols(0, H, N, nc, nc, N, tw, +1, s->st);   // axis 1
         if (e == hipSuccess) e = p2_rows_c2r(s->fp32, H, field, rows, N, tw, norm, 0.0, s->st);
         if (e == hipSuccess) e = hipStreamSynchronize(s->st);
         cleanup();

@@ -60,22 +60,30 @@ __host__ __device__ constexpr int bins_per_thread(int nBins, int threads)
 // (cb_fft_p2.hip).  tw = exp(-2 pi i e / N), e < N, in the transform's
 // precision.  Each returns hipErrorInvalidValue for an unsupported N.
 bool p2_supported(int N);
-// row length of the padded half-complex array the fused spectrum uses
-// (N/2 + 1 rounded up to a multiple of 8 elements)
-int p2_row_stride(int N);
 // real rows (T = R) -> half-complex rows of stride ldh (>= N/2 + 1), forward
 hipError_t p2_rows_r2c(int fp32, const void *field, void *H, long long rows, long long ldh, int N, const void *tw,
                        hipStream_t st);
-// in-place transform along the middle axis of [outer][N][inner]
-hipError_t p2_cols(int fp32, void *H, long long outer, long long inner, int N, const void *tw, int sign,
-                   hipStream_t st);
-// last forward pass of a D = 2 / 3 spectrum: transform axis 0 of [N][inner]
-// (inner = ncp, or N ncp, with ncp = p2_row_stride(N)) and bin the modes
-// instead of storing them; writes *nblk rows of nBins power partials to
-// binpart, one per block of the launch (at most cap_blocks rows).
-hipError_t p2_cols_bin(int fp32, const void *H, long long inner, int N, const void *tw, int D, int nBins, double dk,
-                       double pref, double Pnorm, double *binpart, int cap_blocks, int *nblk, hipStream_t st);
-// largest block count p2_cols_bin uses
+// in-place transform along the middle axis of [outer][N][inner], columns
+// c < ncols
+hipError_t p2_cols(int fp32, void *H, long long outer, long long inner, long long ncols, int N, const void *tw,
+                   int sign, hipStream_t st);
+// row pitch of the spectrum's transposed array: N padded by 128 bytes off a
+// power of two (rows a power-of-two apart in memory load slowly; rows stay
+// 128-byte aligned)
+int p2_pitch(int N, int fp32);
+// The spectrum's first forward pass (D = 2 / 3): real rows of length N ->
+// half-complex, transposed: the field's row i0 M1 + i1 (M1 = N for D = 3,
+// 1 for D = 2) goes to out[(j M1 + i1) ld + i0], j < N/2 + 1.
+hipError_t p2_rows_r2c_t(int fp32, const void *field, void *out, long long M1, long long ld, int N, const void *tw,
+                         hipStream_t st);
+// last forward pass of a D = 2 / 3 spectrum: transform the complex rows of H
+// (pitch ld; row r = j N + i1 for D = 3, r = j for D = 2, along axis 0) and
+// bin the modes instead of storing them; writes *nblk rows of nBins power
+// partials to binpart, one per block of the launch (at most cap_blocks rows).
+hipError_t p2_rows_bin(int fp32, const void *H, long long rows, long long ld, int N, const void *tw, int D, int nBins,
+                       double dk, double pref, double Pnorm, double *binpart, int cap_blocks, int *nblk,
+                       hipStream_t st);
+// largest block count p2_rows_bin uses
 int p2_bin_blocks_max();
 // half-complex rows (double) -> real rows (float or double), backward,
 // field = x * norm + shift

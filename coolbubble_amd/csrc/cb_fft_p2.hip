@@ -42,10 +42,6 @@ template <int N>
 constexpr int kRowStride = pidx(N) + 1;
 template <typename R, int N>
 constexpr int kColStride = pidx(N) + (sizeof(R) == 4 ? (N >= 256 ? 4 : 1) : (N >= 512 ? 2 : 1));
-// padded row length of a half-complex array: a multiple of 8 elements, so
-// the row segments the column passes load start on 64-byte boundaries
-template <int N>
-constexpr int kNcp = (N / 2 + 1 + 7) / 8 * 8;
 // twiddle table staged in LDS when it is small, else read through the caches
 template <typename R, int N>
 constexpr bool kTwLds = (size_t)N * 2 * sizeof(R) <= 8192;
@@ -122,7 +118,7 @@ __device__ inline void p2_stage(cpx<R> *seq, int t, const cpx<R> *twl, int sign,
                 if (NS > 1 && r > 0) {
                     cpx<R> w = twl[r * k * TSTEP];
                     if (sign > 0) w.y = -w.y;
-                    a = cmul(a, w);
+                    a = {fma(a.x, w.x, -a.y * w.y), fma(a.x, w.y, a.y * w.x)};
                 }
                 v[b * RADIX + r] = a;
             }
@@ -303,57 +299,31 @@ k_p2_rows_c2r(const cpx<R> *__restrict__ H, T *__restrict__ field, long long row
     }
 }
 
-// Radial binning of the fused last pass.  D = 2: the columns are j of
-// [N][ncp]; D = 3: the columns are (i1, j) of [N][N][ncp] (axis 0
-// transformed), rows padded to ncp = kNcp<N> >= nc; columns j >= nc are
-// padding and skipped.
-struct BinArgs {
-    int D, nBins;
-    double dk, pref, Pnorm;
-    double *binpart;   // [grid][nBins] power partials, one row per block
-};
-
 // Columns per block: RND rounds of kSeq<N> sequences (about 32 KiB of data
 // per block keeps 3-4 blocks resident per CU with 64-byte row segments;
 // 128-byte segments at 64-70 KiB per block measured slower: 1.19 against
-// 0.89 ms for the 512^3 FP32 spectrum).  The binning pass takes two rounds
-// too for float data (N >= 16, so that a thread's rows stay within a column):
-// with one, its row segments are 32 bytes.
-template <typename R, int N, bool BIN>
-constexpr int kColRounds = (sizeof(R) == 4 && (!BIN || N >= 16)) ? 2 : 1;
+// 0.89 ms for the 512^3 FP32 spectrum).
+template <typename R, int N>
+constexpr int kColRounds = sizeof(R) == 4 ? 2 : 1;
 constexpr size_t kLdsBlockMax = 160 * 1024;   // gfx950: LDS a block may use
 
-template <typename R, int N, bool BIN>
+// In-place transform along the middle axis of [outer][N][inner], columns
+// c < ncols (the rest of a row is padding, left as it is).
+template <typename R, int N>
 __global__ void __launch_bounds__(256)
-k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict__ tw, int sign, BinArgs ba)
+k_p2_cols(cpx<R> *H, long long outer, long long inner, long long ncols, const cpx<R> *__restrict__ tw, int sign)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int RND = kColRounds<R, N, BIN>;
-    constexpr int CB = kSeq<N> * RND, BS = kColStride<R, N>, nc = N / 2 + 1;
+    constexpr int RND = kColRounds<R, N>;
+    constexpr int CB = kSeq<N> * RND, BS = kColStride<R, N>;
     const cpx<R> *twl = stage_twiddles<R, N>(smem_raw, tw);
     cpx<R> *buf = data_area<R, N>(smem_raw);
-    // binning: the block's histogram, the per-thread run records and the
-    // bin range of each column
-    double *hist = reinterpret_cast<double *>(buf + CB * BS);
-    double *rec_fs = hist + ba.nBins;
-    int *rec_fk = reinterpret_cast<int *>(rec_fs + 256);
-    int *rec_lk = rec_fk + 256;
-    int *rec_one = rec_lk + 256;
-    int *line_lo = rec_one + 256;          // [CB] bin at a = 0
-    int *line_hi = line_lo + CB;           // [CB] highest bin binned, -1: none
-    // run totals [CB][SPAN] by (column, bin - lowest bin), in the data area
-    // (free once an item's values are read)
-    constexpr int SPAN = N / 2 + 1;        // bins a column can meet
-    static_assert((size_t)CB * SPAN * sizeof(double) <= (size_t)CB * BS * sizeof(cpx<R>), "run table fits");
-    double *runtab = reinterpret_cast<double *>(buf);
-    if constexpr (BIN)
-        for (int b = threadIdx.x; b < ba.nBins; b += 256) hist[b] = 0.0;
-    const long long tiles = (inner + CB - 1) / CB;
+    const long long tiles = (ncols + CB - 1) / CB;
     const Sched sc = xcd_sched(outer * tiles);
     for (long long it = sc.begin; it < sc.end; it += sc.stride) {
         const long long o = it / tiles;
         const long long c0 = (it - o * tiles) * CB;
-        const int ncol = (int)((inner - c0) < CB ? (inner - c0) : CB);
+        const int ncol = (int)((ncols - c0) < CB ? (ncols - c0) : CB);
         cpx<R> *base = H + o * (long long)N * inner + c0;
         __syncthreads();
         // the loads of a batch are all issued before any is waited for (a
@@ -376,27 +346,182 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
         }
         __syncthreads();
         p2_fft<R, N, RND>(buf, BS, twl, sign);
-        if constexpr (!BIN) {
-            for (int idx = threadIdx.x; idx < N * CB; idx += 256) {
-                const int e = idx / CB, cc = idx - e * CB;
-                if (cc < ncol) base[(long long)e * inner + cc] = buf[cc * BS + pidx(e)];
-            }
+        for (int idx = threadIdx.x; idx < N * CB; idx += 256) {
+            const int e = idx / CB, cc = idx - e * CB;
+            if (cc < ncol) base[(long long)e * inner + cc] = buf[cc * BS + pidx(e)];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The spectrum's forward transforms.  The first pass transposes as it goes,
+// so that the last one reads whole rows:
+//   D = 3: field [i0][i1][i2] --r2c along i2--> H [j][i1][i0]
+//          --along i1 (in place, strided)--> --along i0, binned--> bins
+//   D = 2: field [i0][i1] --r2c along i1--> H [j][i0] --along i0, binned-->
+// (j < N/2 + 1 the half-complex index).  The transposing pass reads whole
+// rows and writes element j of the row (i0, i1) to H[(j M1 + i1) N + i0]
+// (M1 = N for D = 3, 1 for D = 2): its stores are the strided side, row
+// segments of 64 or 128 bytes (consecutive i0 of one i1).
+
+// Rows per item of a transposing pass: RND rounds of kSeq<N> sequences.
+template <typename R, int N>
+constexpr int kRowRounds = (sizeof(R) == 4 && N >= 16) ? 2 : 1;
+
+// Load CB = RND kSeq<N> complex rows of length N and pitch ld (nr of them
+// valid) into buf with 16-byte loads, all issued before any is waited for.
+template <typename R, int N, int CB>
+__device__ inline void load_rows(cpx<R> *buf, const cpx<R> *__restrict__ src, long long ld, int nr)
+{
+    constexpr int BS = kRowStride<N>;
+    constexpr int VW = 16 / sizeof(cpx<R>);         // elements per load
+    constexpr int LIT = CB * N / VW / 256;          // loads per thread
+    static_assert(LIT * 256 * VW == CB * N, "whole loads per thread");
+    Vec16<cpx<R>> v[LIT];
+#pragma unroll
+    for (int u = 0; u < LIT; ++u) {
+        const int q = threadIdx.x + u * 256;
+        const int row = (q * VW) / N, e = q * VW - row * N;
+        if (row < nr) {
+            v[u] = *reinterpret_cast<const Vec16<cpx<R>> *>(src + row * ld + e);
         } else {
-            // Binning in a fixed order, no atomics.  The modes of a column at
-            // s_e = +a and -a (rows a and N - a) have the same |k|, so the
-            // same bin and weight: thread t takes A consecutive values of a
-            // (0 <= a < N/2) in one column, both rows of each, and the last
-            // thread of a column also a = N/2.  Along a column the bin never
-            // decreases with a and grows by at most one per step, so the
-            // column meets every bin from its lowest to its highest once, as
-            // one contiguous run of a.  Each thread sums the runs in its
+#pragma unroll
+            for (int w = 0; w < VW; ++w) v[u].v[w] = cpx<R>{(R)0, (R)0};
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < LIT; ++u) {
+        const int idx = (threadIdx.x + u * 256) * VW;
+        const int row = idx / N, e = idx - row * N;
+#pragma unroll
+        for (int w = 0; w < VW; ++w) buf[row * BS + pidx(e + w)] = v[u].v[w];
+    }
+}
+
+// real rows -> half-complex, transposed: row p = i1 N + i0 of the output
+// order (i0 fastest) is the field's row i0 M1 + i1; element k < N/2 + 1 of
+// it goes to out[(k M1 + i1) ld + i0]; rows = M1 N
+template <typename T, typename R, int N>
+__global__ void __launch_bounds__(256)
+k_p2_rows_r2c_t(const T *__restrict__ field, cpx<R> *__restrict__ out, long long rows, long long M1, long long ld,
+                const cpx<R> *__restrict__ tw)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int RND = kRowRounds<R, N>;
+    constexpr int nc = N / 2 + 1, NSQ = kSeq<N> * RND, RR = 2 * NSQ, BS = kRowStride<N>;
+    constexpr int VW = 16 / sizeof(T);
+    const cpx<R> *twl = stage_twiddles<R, N>(smem_raw, tw);
+    cpx<R> *buf = data_area<R, N>(smem_raw);
+    const long long items = (rows + RR - 1) / RR;
+    const Sched sc = xcd_sched(items);
+    for (long long it = sc.begin; it < sc.end; it += sc.stride) {
+        const long long r0 = it * RR;
+        const long long left = rows - r0;
+        const int nr = (int)(left < RR ? left : RR);
+        __syncthreads();   // the previous item's readers are done with buf
+        constexpr int LIT = (RR * N / VW + 255) / 256;
+        Vec16<T> x[LIT];
+#pragma unroll
+        for (int u = 0; u < LIT; ++u) {
+            const int q = threadIdx.x + u * 256;
+            const int row = q * VW / N, e = q * VW - row * N;
+            if (q < RR * N / VW && row < nr) {
+                const long long pr = r0 + row, i1 = pr / N, i0 = pr - i1 * N;
+                x[u] = *reinterpret_cast<const Vec16<T> *>(field + (i0 * M1 + i1) * N + e);
+            } else {
+#pragma unroll
+                for (int w = 0; w < VW; ++w) x[u].v[w] = (T)0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < LIT; ++u) {
+            const int q = threadIdx.x + u * 256;
+            if (q >= RR * N / VW) continue;
+            const int idx = q * VW;
+            const int row = idx / N, e = idx - row * N;
+#pragma unroll
+            for (int w = 0; w < VW; ++w) {
+                R *slot = reinterpret_cast<R *>(&buf[(row >> 1) * BS + pidx(e + w)]);
+                slot[row & 1] = (R)x[u].v[w];
+            }
+        }
+        __syncthreads();
+        p2_fft<R, N, RND>(buf, BS, twl, -1);
+        // z = a + i b transformed: A_k = (Z_k + conj Z_{N-k}) / 2 (row 2s),
+        // B_k = (Z_k - conj Z_{N-k}) / 2i (row 2s + 1); rows fastest in the
+        // store order, so each k's rows form one segment
+        for (int idx = threadIdx.x; idx < nc * RR; idx += 256) {
+            const int k = idx / RR, row = idx - k * RR, sq = row >> 1;
+            const cpx<R> z = buf[sq * BS + pidx(k)], zn = buf[sq * BS + pidx((N - k) & (N - 1))];
+            const cpx<R> v = (row & 1) ? cpx<R>{(R)0.5 * (z.y + zn.y), (R)0.5 * (zn.x - z.x)}
+                                       : cpx<R>{(R)0.5 * (z.x + zn.x), (R)0.5 * (z.y - zn.y)};
+            const long long pr = r0 + row, i1 = pr / N;
+            if (row < nr) out[(k * M1 + i1) * ld + (pr - i1 * N)] = v;
+        }
+    }
+}
+
+// Radial binning of the last pass.  Row r of its input is the line of
+// modes (s_0 = -N/2+1 .. N/2, fixed s_1, j) along axis 0: D = 3, r = j N + i1
+// (s_1 = i1, or i1 - N above N/2); D = 2, r = j (s_1 = 0).
+struct BinArgs {
+    int D, nBins;
+    double dk, pref, Pnorm;
+    double *binpart;   // [grid][nBins] power partials, one row per block
+};
+
+// complex rows of pitch ld, forward, binned (ComputePowerSpectrum,
+// src/bubble.cpp:668-709); writes one row of nBins power partials per block
+template <typename R, int N>
+__global__ void __launch_bounds__(256)
+k_p2_rows_bin(const cpx<R> *__restrict__ H, long long rows, long long ld, const cpx<R> *__restrict__ tw, BinArgs ba)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int RND = kRowRounds<R, N>;
+    constexpr int CB = kSeq<N> * RND, BS = kRowStride<N>, nc = N / 2 + 1;
+    const cpx<R> *twl = stage_twiddles<R, N>(smem_raw, tw);
+    cpx<R> *buf = data_area<R, N>(smem_raw);
+    // the block's histogram, the per-thread run records and the bin range
+    // of each row
+    double *hist = reinterpret_cast<double *>(buf + CB * BS);
+    double *rec_fs = hist + ba.nBins;
+    int *rec_fk = reinterpret_cast<int *>(rec_fs + 256);
+    int *rec_lk = rec_fk + 256;
+    int *rec_one = rec_lk + 256;
+    int *line_lo = rec_one + 256;          // [CB] bin at a = 0
+    int *line_hi = line_lo + CB;           // [CB] highest bin binned, -1: none
+    // run totals [CB][SPAN] by (row, bin - lowest bin), in the data area
+    // (free once an item's values are read)
+    constexpr int SPAN = N / 2 + 1;        // bins a row can meet
+    static_assert((size_t)CB * SPAN * sizeof(double) <= (size_t)CB * BS * sizeof(cpx<R>), "run table fits");
+    double *runtab = reinterpret_cast<double *>(buf);
+    for (int b = threadIdx.x; b < ba.nBins; b += 256) hist[b] = 0.0;
+    const long long items = (rows + CB - 1) / CB;
+    const Sched sc = xcd_sched(items);
+    for (long long it = sc.begin; it < sc.end; it += sc.stride) {
+        const long long r0 = it * CB;
+        const long long left = rows - r0;
+        const int ncol = (int)(left < CB ? left : CB);
+        __syncthreads();
+        load_rows<R, N, CB>(buf, H + r0 * ld, ld, ncol);
+        __syncthreads();
+        p2_fft<R, N, RND>(buf, BS, twl, -1);
+        {
+            // Binning in a fixed order, no atomics.  The modes of a row at
+            // s_0 = +a and -a (elements a and N - a) have the same |k|, so
+            // the same bin and weight: thread t takes A consecutive values of
+            // a (0 <= a < N/2) in one row, both elements of each, and the
+            // last thread of a row also a = N/2.  Along a row the bin never
+            // decreases with a and grows by at most one per step, so the row
+            // meets every bin from its lowest to its highest once, as one
+            // contiguous run of a.  Each thread sums the runs in its
             // a-range; a run that goes on into the next threads' ranges is
             // completed by the thread where it starts, from the next threads'
             // first partial sums (in order of a).  The run totals go to a
-            // table indexed by (column, bin - lowest bin of the column), in
-            // the data area (its values are read by then); the thread that
-            // owns a bin in the block's histogram then adds that bin's totals
-            // column by column, in a fixed order.  Bins are integer square
+            // table indexed by (row, bin - lowest bin of the row), in the
+            // data area (its values are read by then); the thread that owns a
+            // bin in the block's histogram then adds that bin's totals row by
+            // row, in a fixed order.  Bins are integer square
             // roots, found once per thread and then stepped with integer
             // compares.  The weight kc^D is a product of D factors in double
             // for double data (within 2 ulp of the reference's pow(kc, D));
@@ -408,11 +533,11 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
             const int t = threadIdx.x;
             const int cc = t / TPC, kt = t - cc * TPC, a0 = kt * A;
             const bool lastk = kt == TPC - 1;
-            const int c = (int)c0 + cc;
-            const int j = ba.D == 3 ? c % kNcp<N> : c;
-            const int i1 = ba.D == 3 ? c / kNcp<N> : 0;
+            const long long q = r0 + cc;
+            const int j = (int)(ba.D == 3 ? q / N : q);
+            const int i1 = (int)(ba.D == 3 ? q % N : 0);
             const int s1 = (i1 <= N / 2) ? i1 : i1 - N;
-            const bool colok = cc < ncol && j < nc;
+            const bool colok = cc < ncol;
             const int C = s1 * s1 + j * j;
             const double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
             const double wc = sizeof(R) == 4 ? w * ba.pref * ba.Pnorm * (ba.D == 3 ? ba.dk * ba.dk * ba.dk : ba.dk * ba.dk)
@@ -434,23 +559,24 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
                 if (u > 0) b += (b + 1) * (b + 1) <= m ? 1 : 0;
                 const bool in = u < A || lastk;
                 const cpx<R> f0 = buf[cc * BS + pidx(a)];
-                double p = (double)f0.x * (double)f0.x + (double)f0.y * (double)f0.y;
-                if (a > 0 && a < N / 2) {
-                    const cpx<R> f1 = buf[cc * BS + pidx(N - a)];
-                    p += (double)f1.x * (double)f1.x + (double)f1.y * (double)f1.y;
-                }
-                double kd;
+                const cpx<R> f1 = buf[cc * BS + pidx((N - a) & (N - 1))];
+                const bool pair = a > 0 && a < N / 2;
+                double v;
                 if constexpr (sizeof(R) == 4) {
+                    // |f|^2 |k|^D in float, the weight in double
+                    const float p = fmaf(f0.x, f0.x, f0.y * f0.y) + (pair ? fmaf(f1.x, f1.x, f1.y * f1.y) : 0.0f);
                     const float mf = (float)m;
-                    kd = (double)(ba.D == 3 ? mf * __builtin_amdgcn_sqrtf(mf) : mf);
+                    v = wc * (double)((ba.D == 3 ? mf * __builtin_amdgcn_sqrtf(mf) : mf) * p);
                 } else {
+                    double p = f0.x * f0.x + f0.y * f0.y;
+                    if (pair) p += f1.x * f1.x + f1.y * f1.y;
                     const double kc = sqrt((double)m) * ba.dk;
-                    kd = ba.D == 3 ? kc * kc * kc : kc * kc;
+                    v = wc * (ba.D == 3 ? kc * kc * kc : kc * kc) * p;
                 }
-                val[u] = in ? wc * kd * p : 0.0;
+                val[u] = in ? v : 0.0;
                 key[u] = (in && colok && b < ba.nBins) ? b : -1;
             }
-            // the column's bin range: lowest at a = 0, highest at a = N/2
+            // the row's bin range: lowest at a = 0, highest at a = N/2
             // (none binned when lowest > highest)
             if (kt == 0) line_lo[cc] = bfirst;
             if (lastk) line_hi[cc] = colok ? min(b, ba.nBins - 1) : -1;
@@ -471,13 +597,34 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
             const int tcol0 = cc * TPC;
             // the first run is owned by a previous thread when it goes on from there
             const bool cont = starts && kt > 0 && rec_lk[t - 1] == key[0];
-            // the last run's part in the following threads' ranges
+            // the last run's part in the following threads' ranges: the
+            // next four threads' records read at once, further ones (runs
+            // longer than 4 A + 1 values of a) one by one
             double ext = 0.0;
-            if (ends && !lastk && !(one && cont))
-                for (int u2 = t + 1; u2 < tcol0 + TPC && rec_fk[u2] == key[A - 1]; ++u2) {
-                    ext += rec_fs[u2];
-                    if (!rec_one[u2]) break;
+            if (ends && !lastk && !(one && cont)) {
+                constexpr int PF = 4;
+                int fk[PF], on[PF];
+                double fs[PF];
+#pragma unroll
+                for (int d = 0; d < PF; ++d) {
+                    const int u2 = min(t + 1 + d, tcol0 + TPC - 1);
+                    fk[d] = rec_fk[u2];
+                    fs[d] = rec_fs[u2];
+                    on[d] = rec_one[u2];
                 }
+                const int L = key[A - 1];
+                bool alive = true;
+#pragma unroll
+                for (int d = 0; d < PF; ++d) {
+                    alive = alive && kt + 1 + d < TPC && fk[d] == L;
+                    ext += alive ? fs[d] : 0.0;
+                    alive = alive && on[d];
+                }
+                for (int u2 = t + 1 + PF; alive && u2 < tcol0 + TPC && rec_fk[u2] == L; ++u2) {
+                    ext += rec_fs[u2];
+                    alive = rec_one[u2] != 0;
+                }
+            }
             const int lo = line_lo[cc];
 #pragma unroll
             for (int u = 0; u <= A; ++u) {
@@ -487,24 +634,37 @@ k_p2_cols(cpx<R> *H, long long outer, long long inner, const cpx<R> *__restrict_
                 runtab[cc * SPAN + key[u] - lo] = last ? val[u] + ext : val[u];
             }
             __syncthreads();   // run table complete
+            // each bin's totals added row by row (rows that miss the bin add
+            // nothing; the reads are issued together, no branches)
             const int bpt = bins_per_thread(ba.nBins, 256);
+            constexpr int GC = CB < 8 ? CB : 8;    // rows read per batch
             for (int bb = t * bpt; bb < t * bpt + bpt && bb < ba.nBins; ++bb) {
                 double acc = hist[bb];
-#pragma unroll 4
-                for (int col = 0; col < CB; ++col) {
-                    const int l0 = line_lo[col];
-                    if (bb >= l0 && bb <= line_hi[col]) acc += runtab[col * SPAN + bb - l0];
+                for (int g0 = 0; g0 < CB; g0 += GC) {
+                    int rlo[GC];
+                    double tv[GC];
+#pragma unroll
+                    for (int g = 0; g < GC; ++g) {
+                        rlo[g] = line_lo[g0 + g];
+                        tv[g] = bb <= line_hi[g0 + g] ? 1.0 : 0.0;
+                    }
+#pragma unroll
+                    for (int g = 0; g < GC; ++g) {
+                        const bool hit = bb >= rlo[g] && tv[g] != 0.0;
+                        const double x = runtab[(g0 + g) * SPAN + (hit ? bb - rlo[g] : 0)];
+                        tv[g] = hit ? x : 0.0;
+                    }
+#pragma unroll
+                    for (int g = 0; g < GC; ++g) acc += tv[g];
                 }
                 hist[bb] = acc;
             }
         }
     }
-    if constexpr (BIN) {
-        // one row of partials per block, summed in block order afterwards
-        __syncthreads();   // a block may have had no item
-        double *dst = ba.binpart + (long long)blockIdx.x * ba.nBins;
-        for (int b = threadIdx.x; b < ba.nBins; b += 256) dst[b] = hist[b];
-    }
+    // one row of partials per block, summed in block order afterwards
+    __syncthreads();   // a block may have had no item
+    double *dst = ba.binpart + (long long)blockIdx.x * ba.nBins;
+    for (int b = threadIdx.x; b < ba.nBins; b += 256) dst[b] = hist[b];
 }
 
 int current_cus()
@@ -532,15 +692,25 @@ unsigned persistent_grid(K kernel, size_t shm, long long items, long long cap)
 
 template <typename R, int N>
 constexpr size_t rows_shm() { return twiddle_bytes<R, N>() + (size_t)kSeq<N> * kRowStride<N> * sizeof(cpx<R>); }
-template <typename R, int N, bool BIN>
-constexpr int kCols = kSeq<N> * kColRounds<R, N, BIN>;
-template <typename R, int N, bool BIN>
-constexpr size_t cols_shm()
-{
-    return twiddle_bytes<R, N>() + (size_t)kCols<R, N, BIN> * kColStride<R, N> * sizeof(cpx<R>);
-}
+template <typename R, int N>
+constexpr int kCols = kSeq<N> * kColRounds<R, N>;
+template <typename R, int N>
+constexpr size_t cols_shm() { return twiddle_bytes<R, N>() + (size_t)kCols<R, N> * kColStride<R, N> * sizeof(cpx<R>); }
+template <typename R, int N>
+constexpr int kRowsT = kSeq<N> * kRowRounds<R, N>;   // sequences per item of a row pass
+template <typename R, int N>
+constexpr size_t rows_t_shm() { return twiddle_bytes<R, N>() + (size_t)kRowsT<R, N> * kRowStride<N> * sizeof(cpx<R>); }
 
 constexpr long long kNoCap = 1LL << 40;
+
+// Kernels launched with more than 64 KiB of dynamic LDS ask for it first.
+template <typename K>
+hipError_t allow_lds(K kernel, size_t shm)
+{
+    if (shm <= 65536) return hipSuccess;
+    if (shm > kLdsBlockMax) return hipErrorInvalidValue;
+    return hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+}
 
 template <typename R, int N>
 hipError_t rows_r2c_n(const void *field, void *H, long long rows, long long ldh, const void *tw, hipStream_t st)
@@ -564,42 +734,44 @@ hipError_t rows_c2r_n(const void *H, void *field, long long rows, const void *tw
     return hipGetLastError();
 }
 
-// Kernels launched with more than 64 KiB of dynamic LDS ask for it first.
-template <typename K>
-hipError_t allow_lds(K kernel, size_t shm)
-{
-    if (shm <= 65536) return hipSuccess;
-    if (shm > kLdsBlockMax) return hipErrorInvalidValue;
-    return hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-}
-
 template <typename R, int N>
-hipError_t cols_n(void *H, long long outer, long long inner, const void *tw, int sign, hipStream_t st)
+hipError_t cols_n(void *H, long long outer, long long inner, long long ncols, const void *tw, int sign, hipStream_t st)
 {
-    constexpr int CB = kCols<R, N, false>;
-    const size_t shm = cols_shm<R, N, false>();
-    hipError_t e = allow_lds(k_p2_cols<R, N, false>, shm);
+    constexpr int CB = kCols<R, N>;
+    const size_t shm = cols_shm<R, N>();
+    hipError_t e = allow_lds(k_p2_cols<R, N>, shm);
     if (e != hipSuccess) return e;
-    const long long items = outer * ((inner + CB - 1) / CB);
-    const unsigned grid = persistent_grid(k_p2_cols<R, N, false>, shm, items, kNoCap);
-    k_p2_cols<R, N, false><<<grid, 256, shm, st>>>((cpx<R> *)H, outer, inner, (const cpx<R> *)tw, sign, BinArgs{});
+    const long long items = outer * ((ncols + CB - 1) / CB);
+    const unsigned grid = persistent_grid(k_p2_cols<R, N>, shm, items, kNoCap);
+    k_p2_cols<R, N><<<grid, 256, shm, st>>>((cpx<R> *)H, outer, inner, ncols, (const cpx<R> *)tw, sign);
     return hipGetLastError();
 }
 
 template <typename R, int N>
-hipError_t cols_bin_n(const void *H, long long inner, const void *tw, BinArgs ba, int cap_blocks, int *nblk,
-                      hipStream_t st)
+hipError_t rows_r2c_t_n(const void *field, void *out, long long M1, long long ld, const void *tw, hipStream_t st)
 {
-    constexpr int CB = kCols<R, N, true>;
-    constexpr size_t CBn = kCols<R, N, true>;
-    const size_t shm = cols_shm<R, N, true>() + sizeof(double) * ((size_t)ba.nBins + 256) +
-                       sizeof(int) * (3 * 256 + 2 * CBn);
-    hipError_t e = allow_lds(k_p2_cols<R, N, true>, shm);
+    const size_t shm = rows_t_shm<R, N>();
+    hipError_t e = allow_lds(k_p2_rows_r2c_t<R, R, N>, shm);
     if (e != hipSuccess) return e;
-    const long long items = (inner + CB - 1) / CB;
-    const unsigned grid = persistent_grid(k_p2_cols<R, N, true>, shm, items, cap_blocks);
+    const long long rows = M1 * N;
+    const long long items = (rows + 2 * kRowsT<R, N> - 1) / (2 * kRowsT<R, N>);
+    const unsigned grid = persistent_grid(k_p2_rows_r2c_t<R, R, N>, shm, items, kNoCap);
+    k_p2_rows_r2c_t<R, R, N><<<grid, 256, shm, st>>>((const R *)field, (cpx<R> *)out, rows, M1, ld, (const cpx<R> *)tw);
+    return hipGetLastError();
+}
+
+template <typename R, int N>
+hipError_t rows_bin_n(const void *H, long long rows, long long ld, const void *tw, BinArgs ba, int cap_blocks,
+                      int *nblk, hipStream_t st)
+{
+    constexpr size_t CB = kRowsT<R, N>;
+    const size_t shm = rows_t_shm<R, N>() + sizeof(double) * ((size_t)ba.nBins + 256) + sizeof(int) * (3 * 256 + 2 * CB);
+    hipError_t e = allow_lds(k_p2_rows_bin<R, N>, shm);
+    if (e != hipSuccess) return e;
+    const long long items = (rows + (long long)CB - 1) / (long long)CB;
+    const unsigned grid = persistent_grid(k_p2_rows_bin<R, N>, shm, items, cap_blocks);
     *nblk = (int)grid;
-    k_p2_cols<R, N, true><<<grid, 256, shm, st>>>((cpx<R> *)H, 1, inner, (const cpx<R> *)tw, -1, ba);
+    k_p2_rows_bin<R, N><<<grid, 256, shm, st>>>((const cpx<R> *)H, rows, ld, (const cpx<R> *)tw, ba);
     return hipGetLastError();
 }
 
@@ -624,8 +796,6 @@ bool p2_supported(int N)
     return N >= 8 && N <= 2048 && (N & (N - 1)) == 0;
 }
 
-int p2_row_stride(int N) { return (N / 2 + 1 + 7) / 8 * 8; }
-
 int p2_bin_blocks_max() { return 8 * current_cus(); }
 
 hipError_t p2_rows_r2c(int fp32, const void *field, void *H, long long rows, long long ldh, int N, const void *tw,
@@ -635,19 +805,29 @@ hipError_t p2_rows_r2c(int fp32, const void *field, void *H, long long rows, lon
     CB_P2_SWITCH(N, return (rows_r2c_n<double, NN>(field, H, rows, ldh, tw, st)))
 }
 
-hipError_t p2_cols(int fp32, void *H, long long outer, long long inner, int N, const void *tw, int sign,
-                   hipStream_t st)
+int p2_pitch(int N, int fp32) { return N >= 64 ? N + (fp32 ? 16 : 8) : N; }
+
+hipError_t p2_cols(int fp32, void *H, long long outer, long long inner, long long ncols, int N, const void *tw,
+                   int sign, hipStream_t st)
 {
-    if (fp32) { CB_P2_SWITCH(N, return (cols_n<float, NN>(H, outer, inner, tw, sign, st))) }
-    CB_P2_SWITCH(N, return (cols_n<double, NN>(H, outer, inner, tw, sign, st)))
+    if (fp32) { CB_P2_SWITCH(N, return (cols_n<float, NN>(H, outer, inner, ncols, tw, sign, st))) }
+    CB_P2_SWITCH(N, return (cols_n<double, NN>(H, outer, inner, ncols, tw, sign, st)))
 }
 
-hipError_t p2_cols_bin(int fp32, const void *H, long long inner, int N, const void *tw, int D, int nBins, double dk,
-                       double pref, double Pnorm, double *binpart, int cap_blocks, int *nblk, hipStream_t st)
+hipError_t p2_rows_r2c_t(int fp32, const void *field, void *out, long long M1, long long ld, int N, const void *tw,
+                         hipStream_t st)
+{
+    if (fp32) { CB_P2_SWITCH(N, return (rows_r2c_t_n<float, NN>(field, out, M1, ld, tw, st))) }
+    CB_P2_SWITCH(N, return (rows_r2c_t_n<double, NN>(field, out, M1, ld, tw, st)))
+}
+
+hipError_t p2_rows_bin(int fp32, const void *H, long long rows, long long ld, int N, const void *tw, int D, int nBins,
+                       double dk, double pref, double Pnorm, double *binpart, int cap_blocks, int *nblk,
+                       hipStream_t st)
 {
     const BinArgs ba{D, nBins, dk, pref, Pnorm, binpart};
-    if (fp32) { CB_P2_SWITCH(N, return (cols_bin_n<float, NN>(H, inner, tw, ba, cap_blocks, nblk, st))) }
-    CB_P2_SWITCH(N, return (cols_bin_n<double, NN>(H, inner, tw, ba, cap_blocks, nblk, st)))
+    if (fp32) { CB_P2_SWITCH(N, return (rows_bin_n<float, NN>(H, rows, ld, tw, ba, cap_blocks, nblk, st))) }
+    CB_P2_SWITCH(N, return (rows_bin_n<double, NN>(H, rows, ld, tw, ba, cap_blocks, nblk, st)))
 }
 
 hipError_t p2_rows_c2r(int field_fp32, const void *H, void *field, long long rows, int N, const void *tw,
