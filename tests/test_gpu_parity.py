@@ -372,6 +372,35 @@ def test_multi_step_variants_bit_identical_with_odd_counts(name, fuse):
         g.close()
 
 
+@pytest.mark.parametrize("D,N,precision", [(3, 256, "fp32"), (3, 256, "fp64"), (2, 512, "fp32"), (2, 1024, "fp64")])
+def test_wide_row_variants_bit_identical(D, N, precision):
+    """The prefetching multi-step tiles at lattice sizes where a 64-lane wave
+    holds whole rows or parts of one row only (N = 256 and up), the shapes
+    the measured workloads use: every such variant gives the general
+    kernel's fields bit for bit over a call of 7 steps (multi-step launches
+    plus the leftover steps on the one-step companion)."""
+    kw = params(D=D, readxns=-2, nGrids=N, L=N / 8.0)
+    ref = cb.Bubble(precision=precision, **kw)
+    ref.select_kernel(-1)
+    ref.step(7)
+    phi0, pi0 = ref.fields()
+    n, _ = ref.kernel_variants()
+    ref.close()
+    g = cb.Bubble(precision=precision, **kw)
+    infos = [g.variant_info(v) for v in range(n)]
+    g.close()
+    multi = [v for v in range(n) if infos[v]["fuse"] >= 2 and infos[v]["pf"] >= 2]
+    assert multi
+    for v in multi:
+        g = cb.Bubble(precision=precision, **kw)
+        g.select_kernel(v)
+        g.step(7)
+        phi, pi = g.fields()
+        np.testing.assert_array_equal(phi, phi0, err_msg=str(infos[v]))
+        np.testing.assert_array_equal(pi, pi0, err_msg=str(infos[v]))
+        g.close()
+
+
 def test_default_variant_follows_the_lattice():
     """cb_create starts small lattices on the general kernel, large static
     boxes on a three-step tile, expanding ones on a one-step tile."""
