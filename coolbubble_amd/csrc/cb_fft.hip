@@ -398,23 +398,38 @@ static BinShape bin_shape(int N, int nBins)
     return b;
 }
 
-// Fixed-order sum of the per-block bin partials: 64 bins per block, the
-// partials split over the 4 waves, then the 4 wave sums merged in order.
-__global__ void __launch_bounds__(256)
+// Fixed-order sum of the per-block bin partials: 64 bins per block of
+// 64 * BR_WAVES threads.  Wave w sums rows w, w + W, w + 2W, ... in that
+// order (Neumaier), eight rows' loads issued before any is added (a lone
+// small grid has nothing else to hide their latency behind); the W wave sums
+// are then merged in wave order.  The order of every addition depends only
+// on (nblk, W), so the sums are the same on every call.
+constexpr int BR_WAVES = 16;
+__global__ void __launch_bounds__(64 * BR_WAVES)
 k_bin_reduce(const double *__restrict__ binpart, int nblk, int nq, double *__restrict__ out)
 {
-    __shared__ double ss[4][64], sc[4][64];
+    constexpr int W = BR_WAVES;
+    __shared__ double ss[W][64], sc[W][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int q = blockIdx.x * 64 + lane;
     Neu s = {0.0, 0.0};
-    if (q < nq)
-        for (int b = w; b < nblk; b += 4) s.add(binpart[(long long)b * nq + q]);
+    if (q < nq) {
+        int b = w;
+        for (; b + 7 * W < nblk; b += 8 * W) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = binpart[(long long)(b + u * W) * nq + q];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s.add(v[u]);
+        }
+        for (; b < nblk; b += W) s.add(binpart[(long long)b * nq + q]);
+    }
     ss[w][lane] = s.s;
     sc[w][lane] = s.c;
     __syncthreads();
     if (w == 0 && q < nq) {
         Neu t = {ss[0][lane], sc[0][lane]};
-        for (int k = 1; k < 4; ++k) t.merge(Neu{ss[k][lane], sc[k][lane]});
+        for (int k = 1; k < W; ++k) t.merge(Neu{ss[k][lane], sc[k][lane]});
         out[q] = t.value();
     }
 }
@@ -869,7 +884,7 @@ static hipError_t plan_buffers(FftPlan *p, Arena *arena, void *stage, double L, 
         const double dk = 2.0 * 3.14159265358979323846 / L;
         k_bin<double, true><<<p->binblocks, 256, bs.shm, st>>>(nullptr, p->rows, N, D, nB, dk, 0.0, 0.0, bs.E, bs.K,
                                                                bs.RI, p->binpart);
-        k_bin_reduce<<<(3 * nB + 63) / 64, 256, 0, st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
+        k_bin_reduce<<<(3 * nB + 63) / 64, 64 * BR_WAVES, 0, st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
         e = hipGetLastError();
     }
     return e;
@@ -918,7 +933,7 @@ static cb_status spectrum_run(cb_sim *s, FftPlan *p, double *h, double *kmean, d
         int nblk = 0;
         HIP_TRY(p2_rows_bin(f32, p->H, p->rows / N * nc, ld, N, p->tw, D, nB, dk, pref, Pnorm, p->binpart,
                             p->fused_blocks, &nblk, s->st));
-        k_bin_reduce<<<(nB + 63) / 64, 256, 0, s->st>>>(p->binpart, nblk, nB, p->bins);
+        k_bin_reduce<<<(nB + 63) / 64, 64 * BR_WAVES, 0, s->st>>>(p->binpart, nblk, nB, p->bins);
         HIP_TRY(hipGetLastError());
     } else if (p->N1) {
         HIP_TRY(s->fp32 ? long_row<float>(p, (const float *)field, pref, dk, Pnorm, s->st)
@@ -941,7 +956,7 @@ static cb_status spectrum_run(cb_sim *s, FftPlan *p, double *h, double *kmean, d
             k_bin<double><<<p->binblocks, 256, bs.shm, s->st>>>((const cpx<double> *)p->H, p->rows, N, D, nB, dk, pref,
                                                                  Pnorm, bs.E, bs.K, bs.RI, p->binpart);
         HIP_TRY(hipGetLastError());
-        k_bin_reduce<<<(3 * nB + 63) / 64, 256, 0, s->st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
+        k_bin_reduce<<<(3 * nB + 63) / 64, 64 * BR_WAVES, 0, s->st>>>(p->binpart, p->binblocks, 3 * nB, p->bins);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipMemcpyAsync(h, p->bins, sizeof(double) * 3 * nB, hipMemcpyDeviceToHost, s->st));
