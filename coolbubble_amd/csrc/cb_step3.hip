@@ -3,6 +3,7 @@
 // launch's configurations, validity and candidates.
 #include "cb_device.hpp"
 #include "cb_tile3.hpp"
+#include "cb_tile3d.hpp"
 #include "cb_tile_common.hpp"
 #include "cb_tiled.hpp"
 
@@ -21,9 +22,23 @@ k_step3_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restri
                                           reinterpret_cast<T *>(smem_raw));
 }
 
+template <bool ENERGY>
+__global__ void __launch_bounds__(kDmaThreads)
+k_step3_dma(SimConst k, TiledGeom g, const float *__restrict__ phi_in, float *__restrict__ phi_out,
+            const float *__restrict__ pi_in, float *__restrict__ pi_out, const DevScalars *__restrict__ sc,
+            double *__restrict__ partials)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int G = gridDim.x, b = blockIdx.x;
+    const int lt = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;   // XCD-contiguous tiles
+    tile_step3_dma<ENERGY>(k, g, lt, phi_in, phi_out, pi_in, pi_out, sc, partials, reinterpret_cast<float *>(smem_raw));
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+constexpr size_t kLdsWide = 160 * 1024;   // gfx950: the LDS one block may use
+
 static TiledGeom geom3(const SimConst &k, const TiledCfg &c)
 {
     TiledGeom g;
@@ -39,25 +54,41 @@ static TiledGeom geom3(const SimConst &k, const TiledCfg &c)
 size_t step3_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c)
 {
     const size_t esz = fp32 ? 4 : 8;
-    const int rows = (k.D == 3) ? (c.TR + 6) + (c.TR + 4) + (c.TR + 2) : 3;
+    // (the direct-load form, prefetch kind 4, keeps four planes of phi_n)
+    const int l0 = (c.vw == 8 && c.pf == 4) ? 4 : 1;
+    const int rows = (k.D == 3) ? l0 * (c.TR + 6) + (c.TR + 4) + (c.TR + 2) : 3;
     return (size_t)rows * k.N * esz;
+}
+
+// The wide tile (cb_tile3d.hpp): FP32 3D, eight sites per thread vector,
+// 768 threads, one vector per thread, phi_n loaded straight into four LDS
+// plane buffers (prefetch kind 4) -- a tile of TR = 8 rows fits one block
+// (12 rows of step A x 64 vectors at N = 512), so the rows recomputed for
+// steps A and B are 1.25x the owned ones instead of 1.5x at TR = 4, at up
+// to 160 KiB of LDS.  Rows of a multiple of 256 sites (whole waves of
+// 16-byte pieces).
+static bool step3_wide(const SimConst &k, int fp32, const TiledCfg &c)
+{
+    return fp32 && k.D == 3 && c.vw == 8 && c.nthr == kDmaThreads && c.items == 1;
 }
 
 bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c)
 {
     if (k.D < 2 || k.expanding || c.fuse != 3) return false;
+    const bool wide = step3_wide(k, fp32, c);
     const int want_vw = fp32 ? 4 : 2;
-    if (c.vw != want_vw || k.N % c.vw != 0 || k.N < 8) return false;
+    if ((c.vw != want_vw && !wide) || k.N % c.vw != 0 || k.N < 8) return false;
     if (c.items != 1 && c.items != 2 && c.items != 4) return false;
-    if (c.nthr != 256 && c.nthr != 512 && c.nthr != 1024) return false;
+    if (c.nthr != 256 && c.nthr != 512 && c.nthr != 1024 && !wide) return false;
+    if (wide && !(c.pf == 4 && k.N % 256 == 0)) return false;
     if (c.NI < 1 || (c.nt != 0 && c.nt != 1)) return false;
-    if (c.pf != 1 && !((c.pf == 2 || c.pf == 3) && c.items == 1)) return false;   // prefetch: one vector per thread
+    if (!wide && c.pf != 1 && !((c.pf == 2 || c.pf == 3) && c.items == 1)) return false;   // prefetch: one vector per thread
     if (k.D == 3 && (c.TR < 1 || k.N % c.TR != 0)) return false;
     const int NV = k.N / c.vw;
     const int rowsA = (k.D == 3) ? c.TR + 4 : 1;
     if ((long long)rowsA * NV > (long long)c.items * c.nthr) return false;
     if (k.D == 3 && 2 * NV > c.items * c.nthr) return false;
-    if (step3_lds_bytes(k, fp32, c) > 65536) return false;
+    if (step3_lds_bytes(k, fp32, c) > (wide ? kLdsWide : 65536)) return false;
     return true;
 }
 
@@ -116,6 +147,19 @@ hipError_t launch_step3(const SimConst &k, int fp32, int energy, const TiledCfg 
     TiledGeom g = geom3(k, c);
     if (nchunks > 0 && nchunks < g.nic) g.nic = nchunks;   // the first nchunks chunks of planes only
     if (nblocks) *nblocks = g.ntr * g.nic;
+    if (step3_wide(k, fp32, c)) {
+        // phi_n staged by direct loads into LDS (cb_tile3d.hpp); the
+        // energies on the same tile
+        dim3 grid(g.ntr * g.nic), block(kDmaThreads);
+        const float *pin = (const float *)p.phi_in, *qin = (const float *)p.pi_in;
+        float *pout = (float *)p.phi_out, *qout = (float *)p.pi_out;
+        const void *kern = energy ? (const void *)k_step3_dma<true> : (const void *)k_step3_dma<false>;
+        hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+        if (energy) k_step3_dma<true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+        else k_step3_dma<false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+        return hipGetLastError();
+    }
     if (fp32) {
         if (k.D == 3) launch3_d<float, 3>(k, energy, c, p, sc, partials, st, shm, g.nic);
         else launch3_d<float, 2>(k, energy, c, p, sc, partials, st, shm, g.nic);
@@ -162,6 +206,16 @@ int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
         for (int TR : {4, 2})
             for (int NI : {64, 32})
                 for (int nt : {1, 0}) push(1024, TR, NI, nt, 3);
+        // one chunk of 128 planes per tile (half the chunk-end planes of 64)
+        for (int nt : {1, 0}) push(1024, 4, 128, nt, 2);
+        // the wide tile (FP32): eight sites per thread, 768 threads, phi_n
+        // loaded straight into LDS
+        if (fp32)
+            for (int NI : {128, 64})
+                for (int nt : {1, 0}) {
+                    TiledCfg c{8, 1, kDmaThreads, 8, NI, 4, nt, 3};
+                    if (n < cap && step3_valid(k, fp32, c) && 12 * (k.N / 8) > 512) out[n++] = c;
+                }
     } else {
         for (int nthr : {256, 512})
             for (int NI : {16, 32, 8})

@@ -18,6 +18,15 @@ __device__ inline void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// Wait until every vector memory load (direct-to-LDS loads included) and
+// store of this wave has completed: s_waitcnt vmcnt(0), expcnt and lgkmcnt
+// left at their maxima.  Written as the builtin so that the compiler's own
+// wait insertion sees it.
+__device__ inline void vm_wait_all()
+{
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+}
+
 template <typename T, int VW>
 __device__ inline void store_vec(T *p, const Vec<T, VW> &v, bool nt)
 {

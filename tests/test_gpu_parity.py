@@ -372,13 +372,17 @@ def test_multi_step_variants_bit_identical_with_odd_counts(name, fuse):
         g.close()
 
 
-@pytest.mark.parametrize("D,N,precision", [(3, 256, "fp32"), (3, 256, "fp64"), (2, 512, "fp32"), (2, 1024, "fp64")])
+@pytest.mark.parametrize("D,N,precision", [(3, 256, "fp32"), (3, 384, "fp32"), (3, 512, "fp32"), (3, 256, "fp64"),
+                                         (2, 512, "fp32"), (2, 1024, "fp64")])
 def test_wide_row_variants_bit_identical(D, N, precision):
     """The prefetching multi-step tiles at lattice sizes where a 64-lane wave
     holds whole rows or parts of one row only (N = 256 and up), the shapes
-    the measured workloads use: every such variant gives the general
-    kernel's fields bit for bit over a call of 7 steps (multi-step launches
-    plus the leftover steps on the one-step companion)."""
+    the measured workloads use, and the wide three-step tile (eight sites
+    per thread, 768 threads, phi_n loaded straight into LDS: prefetch kind
+    4; 3D FP32 at N = 512): every such variant gives the general kernel's
+    fields bit for bit over a call of 7 steps (multi-step launches plus the
+    leftover steps on the one-step companion).  At N = 512 only the wide
+    tiles are run (the others are the C2 tests')."""
     kw = params(D=D, readxns=-2, nGrids=N, L=N / 8.0)
     ref = cb.Bubble(precision=precision, **kw)
     ref.select_kernel(-1)
@@ -389,8 +393,12 @@ def test_wide_row_variants_bit_identical(D, N, precision):
     g = cb.Bubble(precision=precision, **kw)
     infos = [g.variant_info(v) for v in range(n)]
     g.close()
-    multi = [v for v in range(n) if infos[v]["fuse"] >= 2 and infos[v]["pf"] >= 2]
+    multi = [v for v in range(n) if infos[v]["fuse"] >= 2 and (infos[v]["pf"] >= 2 or infos[v]["vw"] == 8)]
+    if N == 512 and D == 3:
+        multi = [v for v in multi if infos[v]["vw"] == 8]
+        assert any(infos[v]["pf"] == 4 for v in multi)
     assert multi
+
     for v in multi:
         g = cb.Bubble(precision=precision, **kw)
         g.select_kernel(v)
