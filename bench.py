@@ -73,7 +73,9 @@ def run_precision(cb, hip, precision, steps, warmup, log, variant=None):
     sim = cb.Bubble(precision=precision, **CONFIG)
     log("%s: seeded %d^3 in %.1f s" % (precision, CONFIG["nGrids"], time.time() - t0))
     if variant is None:
-        best, ms = sim.autotune(steps=ENERGY_EVERY)   # trials at the output cadence
+        # trials in calls of the length then timed (the output cadence, or
+        # fewer steps when fewer are timed): a variant's tail launches count
+        best, ms = sim.autotune(steps=min(ENERGY_EVERY, steps))
         nvar, sel = sim.kernel_variants()
         log("%s: autotune picked variant %d of %d (%.3f ms/step; general kernel %.3f ms/step)"
             % (precision, sel, nvar, ms[sel] if sel >= 0 else ms[-1], ms[-1]))

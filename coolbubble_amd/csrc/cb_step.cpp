@@ -343,8 +343,9 @@ extern "C" cb_status cb_autotune(cb_sim *s, int steps, double *ms_per_step, int 
         return r;
     };
     const int nvar = (int)s->variants.size();
-    // time `reps` runs of `nsteps` steps of one variant (-1 = general kernel); min per step
-    auto trial = [&](int var, int nsteps, int reps) -> double {
+    // time `reps` runs of `calls` calls of `nsteps` steps each of one variant
+    // (-1 = general kernel); min per step
+    auto trial = [&](int var, int nsteps, int calls, int reps) -> double {
         select_variant(s, var);
         double per = 1e300;
         for (int rep = 0; rep < reps && e == hipSuccess; ++rep) {
@@ -352,19 +353,23 @@ extern "C" cb_status cb_autotune(cb_sim *s, int steps, double *ms_per_step, int 
             if (e != hipSuccess) break;
             if (step_n(s, 2) != CB_OK) { e = hipErrorUnknown; break; }
             e = hipEventRecord(ev[0], s->st);
-            // one call: energies on its last step only, as in a run printing every `nsteps`
-            if (e == hipSuccess && step_n(s, nsteps) != CB_OK) e = hipErrorUnknown;
+            // each call measures energies on its last step only, as a run
+            // printing every `nsteps` steps does
+            for (int c = 0; c < calls && e == hipSuccess; ++c)
+                if (step_n(s, nsteps) != CB_OK) e = hipErrorUnknown;
             if (e == hipSuccess) e = hipEventRecord(ev[1], s->st);
             if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
             float ms = 0.f;
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
-            if ((double)ms / nsteps < per) per = (double)ms / nsteps;
+            if ((double)ms / (nsteps * calls) < per) per = (double)ms / (nsteps * calls);
         }
         return per;
     };
-    // phase 1: every variant, short runs; phase 2: the 8 fastest, runs 4x longer
+    // phase 1: every variant, one call; phase 2: the 8 fastest, four calls
+    // in a row (the same call length, so a variant is judged with the launch
+    // sequence a call of `steps` steps gives it, its tail launches included)
     std::vector<double> quick((size_t)nvar + 1, 1e300);
-    for (int v = 0; v <= nvar && e == hipSuccess; ++v) quick[(size_t)v] = trial(v == nvar ? -1 : v, steps, 2);
+    for (int v = 0; v <= nvar && e == hipSuccess; ++v) quick[(size_t)v] = trial(v == nvar ? -1 : v, steps, 1, 2);
     std::vector<int> order((size_t)nvar + 1);
     for (int v = 0; v <= nvar; ++v) order[(size_t)v] = v;
     std::sort(order.begin(), order.end(), [&](int a, int b) { return quick[(size_t)a] < quick[(size_t)b]; });
@@ -373,7 +378,7 @@ extern "C" cb_status cb_autotune(cb_sim *s, int steps, double *ms_per_step, int 
     int best_v = -1;
     for (int i = 0; i < (int)order.size() && i < 8 && e == hipSuccess; ++i) {
         const int v = order[(size_t)i];
-        final_ms[(size_t)v] = trial(v == nvar ? -1 : v, 4 * steps, 3);
+        final_ms[(size_t)v] = trial(v == nvar ? -1 : v, steps, 4, 3);
         if (final_ms[(size_t)v] < best_ms) { best_ms = final_ms[(size_t)v]; best_v = v == nvar ? -1 : v; }
     }
     if (ms_per_step)

@@ -68,8 +68,13 @@ static void launch2_t(const SimConst &k, int energy, const TiledCfg &c, const St
     TiledGeom g = geom2(k, c);
     g.nic = nic;
     dim3 grid(g.ntr * g.nic), block(NTHR);
-    // (the energy launches do not prefetch: their FP64 sums take the registers)
-    if (energy)
+    // (energy launches prefetch only with one vector per thread: with two
+    // their FP64 sums leave no registers for it; with one, FP32 at 1024
+    // threads, the prefetching energy tile takes 122 VGPRs, no spills)
+    if (energy && ITEMS == 1 && c.pf == 2)
+        k_step2_tiled<T, D, 1, NTHR, true, 2><<<grid, block, shm, st>>>(
+            k, g, (const T *)p.phi_in, (T *)p.phi_out, (const T *)p.pi_in, (T *)p.pi_out, sc, partials);
+    else if (energy)
         k_step2_tiled<T, D, ITEMS, NTHR, true, 1><<<grid, block, shm, st>>>(
             k, g, (const T *)p.phi_in, (T *)p.phi_out, (const T *)p.pi_in, (T *)p.pi_out, sc, partials);
     else if constexpr (ITEMS <= 2) {

@@ -155,16 +155,20 @@ cb_status cb_get_xns(cb_sim *s, double *xyz, int cap, int *n);
 cb_status cb_kernel_variants(cb_sim *s, int *count, int *selected);
 cb_status cb_select_kernel(cb_sim *s, int variant);
 /* Shape of tiled variant `variant`: info = {vector width, vectors per thread,
- * threads per block, rows per tile, planes per block, prefetch depth,
- * non-temporal stores, steps per launch (1, or 2 = two leapfrog steps per
- * pass over the lattice, static box only)}. */
+ * threads per block, rows per tile, planes per block, prefetch (1 none, 2
+ * the next plane's loads one plane ahead, 3 two planes ahead, 4 phi loaded
+ * straight into LDS buffers two planes ahead), non-temporal stores, steps
+ * per launch (1, or 2 / 3 leapfrog steps per pass over the lattice, static
+ * box only)}. */
 cb_status cb_kernel_variant_info(cb_sim *s, int variant, int info[8]);
 
-/* Autotune: time every variant over `steps` steps, then the 8 fastest again
- * over 4 x `steps` steps (the lattice state is saved and restored around
- * the trials), and select the fastest.  ms_per_step (optional, capacity
- * cap) receives each variant's time per step (the longer trial where there
- * was one), general kernel last. */
+/* Autotune: time every variant over one call of `steps` steps, then the 8
+ * fastest again over four calls of `steps` steps in a row (each call
+ * measures the energies on its last step, as a run printing every `steps`
+ * steps does; the lattice state is saved and restored around the trials),
+ * and select the fastest.  ms_per_step (optional, capacity cap) receives
+ * each variant's time per step (the longer trial where there was one),
+ * general kernel last. */
 cb_status cb_autotune(cb_sim *s, int steps, double *ms_per_step, int cap, int *best);
 
 /* The HIP stream (hipStream_t) every kernel of this handle runs on. */
