@@ -85,9 +85,12 @@ def run_precision(cb, hip, precision, steps, warmup, log, variant=None):
             "%d %.4f ms %s" % (v, ms[v], sim.variant_info(v) if v < nvar else "general") for v in order)))
     else:
         sim.select_kernel(variant)
+    # warm-up calls no longer than the timed ones, so that every kernel the
+    # timed calls launch (their tails included) has been launched once
+    # before the timed region (a kernel's first launch loads its code)
     done = 0
     while done < warmup:
-        n = min(ENERGY_EVERY, warmup - done)
+        n = min(ENERGY_EVERY, steps, warmup - done)
         sim.step(n)
         done += n
     stream = sim.stream()

@@ -27,7 +27,7 @@ extern "C" cb_status cb_slab_probe(cb_sim *s, int planes, int launches, double *
     }
     const cb::TiledCfg c = s->variants[(size_t)s->variant];
     const int N = s->p.nGrids, F = c.fuse;
-    if (F < 2) { set_error("slab probe: needs a two- or three-step variant"); return CB_UNSUPPORTED; }
+    if (F < 2) { set_error("slab probe: needs a multi-step variant"); return CB_UNSUPPORTED; }
     if (planes < c.NI || planes < F || planes % c.NI != 0 || planes + 2 * F > N) {
         set_error("slab probe: planes must be a multiple of the variant's chunk (" + std::to_string(c.NI) +
                   ") and leave room for the halo planes");
@@ -48,7 +48,7 @@ extern "C" cb_status cb_slab_probe(cb_sim *s, int planes, int launches, double *
         p.pi_out = s->pi[s->ipi ^ 1];
         int nb = 0;
         const hipError_t r =
-            F == 3 ? cb::launch_step3(s->k, s->fp32, 0, c, p, s->dsc, s->partials, s->st, &nb, nchunks)
+            F >= 3 ? cb::launch_step3(s->k, s->fp32, 0, c, p, s->dsc, s->partials, s->st, &nb, nchunks)
                    : cb::launch_step2(s->k, s->fp32, 0, c, p, s->dsc, s->partials, s->st, &nb, nchunks);
         s->iphi ^= 1;
         s->ipi ^= 1;

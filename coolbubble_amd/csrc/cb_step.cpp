@@ -52,7 +52,7 @@ static hipError_t launch_one(cb_sim *s, bool energy, bool lite = false)
     return e;
 }
 
-// Two or three steps in one launch (static box): phi and pi both ping-pong.
+// Two, three or four steps in one launch (static box): phi and pi both ping-pong.
 static hipError_t launch_multi(cb_sim *s, int fuse, bool energy)
 {
     StepPtrs p;
@@ -62,7 +62,7 @@ static hipError_t launch_multi(cb_sim *s, int fuse, bool energy)
     p.pi_out = s->pi[s->ipi ^ 1];
     int nblocks = 0;
     const TiledCfg c = tiled_get(energy && s->tiled_energy >= 0 ? s->tiled_energy : s->tiled);
-    hipError_t e = fuse == 3 ? launch_step3(s->k, s->fp32, energy, c, p, s->dsc, s->partials, s->st, &nblocks)
+    hipError_t e = fuse >= 3 ? launch_step3(s->k, s->fp32, energy, c, p, s->dsc, s->partials, s->st, &nblocks)
                              : launch_step2(s->k, s->fp32, energy, c, p, s->dsc, s->partials, s->st, &nblocks);
     if (e == hipSuccess && energy) e = launch_finalize(s->k, s->partials, nblocks, s->dsc, 1, s->st);
     return e;

@@ -22,16 +22,17 @@ k_step3_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restri
                                           reinterpret_cast<T *>(smem_raw));
 }
 
-template <bool ENERGY>
-__global__ void __launch_bounds__(kDmaThreads)
-k_step3_dma(SimConst k, TiledGeom g, const float *__restrict__ phi_in, float *__restrict__ phi_out,
+template <int S, bool ENERGY>
+__global__ void __launch_bounds__(kDmaThreads<S>)
+k_stepS_dma(SimConst k, TiledGeom g, const float *__restrict__ phi_in, float *__restrict__ phi_out,
             const float *__restrict__ pi_in, float *__restrict__ pi_out, const DevScalars *__restrict__ sc,
             double *__restrict__ partials)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int G = gridDim.x, b = blockIdx.x;
     const int lt = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;   // XCD-contiguous tiles
-    tile_step3_dma<ENERGY>(k, g, lt, phi_in, phi_out, pi_in, pi_out, sc, partials, reinterpret_cast<float *>(smem_raw));
+    tile_stepS_dma<S, ENERGY>(k, g, lt, phi_in, phi_out, pi_in, pi_out, sc, partials,
+                              reinterpret_cast<float *>(smem_raw));
 }
 
 // ---------------------------------------------------------------------------
@@ -54,9 +55,15 @@ static TiledGeom geom3(const SimConst &k, const TiledCfg &c)
 size_t step3_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c)
 {
     const size_t esz = fp32 ? 4 : 8;
-    // (the direct-load form, prefetch kind 4, keeps four planes of phi_n)
-    const int l0 = (c.vw == 8 && c.pf == 4) ? 4 : 1;
-    const int rows = (k.D == 3) ? l0 * (c.TR + 6) + (c.TR + 4) + (c.TR + 2) : 3;
+    if (c.vw == 8 && c.pf == 4) {
+        // the direct-load form: four planes of phi_n (TR + 2S rows), then
+        // level l's plane (TR + 2S - 2l rows), l = 1 .. S-1
+        const int S = c.fuse;
+        int rows = 4 * (c.TR + 2 * S);
+        for (int l = 1; l < S; ++l) rows += c.TR + 2 * S - 2 * l;
+        return (size_t)rows * k.N * esz;
+    }
+    const int rows = (k.D == 3) ? (c.TR + 6) + (c.TR + 4) + (c.TR + 2) : 3;
     return (size_t)rows * k.N * esz;
 }
 
@@ -69,13 +76,13 @@ size_t step3_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c)
 // 16-byte pieces).
 static bool step3_wide(const SimConst &k, int fp32, const TiledCfg &c)
 {
-    return fp32 && k.D == 3 && c.vw == 8 && c.nthr == kDmaThreads && c.items == 1;
+    return fp32 && k.D == 3 && c.vw == 8 && c.items == 1 && c.fuse == 3 && c.nthr == kDmaThreads<3>;
 }
 
 bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c)
 {
-    if (k.D < 2 || k.expanding || c.fuse != 3) return false;
     const bool wide = step3_wide(k, fp32, c);
+    if (k.D < 2 || k.expanding || (c.fuse != 3 && !wide)) return false;
     const int want_vw = fp32 ? 4 : 2;
     if ((c.vw != want_vw && !wide) || k.N % c.vw != 0 || k.N < 8) return false;
     if (c.items != 1 && c.items != 2 && c.items != 4) return false;
@@ -85,7 +92,7 @@ bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c)
     if (!wide && c.pf != 1 && !((c.pf == 2 || c.pf == 3) && c.items == 1)) return false;   // prefetch: one vector per thread
     if (k.D == 3 && (c.TR < 1 || k.N % c.TR != 0)) return false;
     const int NV = k.N / c.vw;
-    const int rowsA = (k.D == 3) ? c.TR + 4 : 1;
+    const int rowsA = (k.D == 3) ? c.TR + 2 * c.fuse - 2 : 1;
     if ((long long)rowsA * NV > (long long)c.items * c.nthr) return false;
     if (k.D == 3 && 2 * NV > c.items * c.nthr) return false;
     if (step3_lds_bytes(k, fp32, c) > (wide ? kLdsWide : 65536)) return false;
@@ -150,14 +157,14 @@ hipError_t launch_step3(const SimConst &k, int fp32, int energy, const TiledCfg 
     if (step3_wide(k, fp32, c)) {
         // phi_n staged by direct loads into LDS (cb_tile3d.hpp); the
         // energies on the same tile
-        dim3 grid(g.ntr * g.nic), block(kDmaThreads);
+        dim3 grid(g.ntr * g.nic), block(c.nthr);
         const float *pin = (const float *)p.phi_in, *qin = (const float *)p.pi_in;
         float *pout = (float *)p.phi_out, *qout = (float *)p.pi_out;
-        const void *kern = energy ? (const void *)k_step3_dma<true> : (const void *)k_step3_dma<false>;
+        const void *kern = energy ? (const void *)k_stepS_dma<3, true> : (const void *)k_stepS_dma<3, false>;
         hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
-        if (energy) k_step3_dma<true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
-        else k_step3_dma<false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+        if (energy) k_stepS_dma<3, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+        else k_stepS_dma<3, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
         return hipGetLastError();
     }
     if (fp32) {
@@ -213,9 +220,10 @@ int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
         if (fp32)
             for (int NI : {128, 64})
                 for (int nt : {1, 0}) {
-                    TiledCfg c{8, 1, kDmaThreads, 8, NI, 4, nt, 3};
+                    TiledCfg c{8, 1, kDmaThreads<3>, 8, NI, 4, nt, 3};
                     if (n < cap && step3_valid(k, fp32, c) && 12 * (k.N / 8) > 512) out[n++] = c;
                 }
+
     } else {
         for (int nthr : {256, 512})
             for (int NI : {16, 32, 8})

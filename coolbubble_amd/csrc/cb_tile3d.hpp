@@ -1,24 +1,33 @@
-// cb_tile3d.hpp -- the three-step tile of cb_tile3.hpp with phi_n staged in
-// LDS by direct global-to-LDS loads (internal; included by cb_step3.hip).
+// cb_tile3d.hpp -- the multi-step tile of cb_tile3.hpp with phi_n staged in
+// LDS by direct global-to-LDS loads, S = 3 or 4 steps per pass (internal;
+// included by cb_step3.hip).
 //
-// 3D FP32 only: eight sites per thread vector, 768 threads, tiles of TR rows
-// (TR = 8 at N = 512: 12 rows of step A x 64 vectors), rows of a multiple of
-// 256 sites.  The levels and steps are those of tile_step3 (step A on plane
-// p for TR+4 rows, B on p-1 for TR+2, C on p-2 for the TR owned rows, each
-// site's arithmetic that of three single steps, so fields are bit-identical
-// to the one-step kernels).  What differs is where phi_n lives: instead of a
-// ring of three planes in registers plus a register prefetch of the next
-// plane and the outer rows, four LDS buffers hold the TR+6 rows of phi_n at
-// planes p-1, p, p+1 (step A's i taps and its plane's j and k taps) and
-// p+2, which arrives while plane p is computed.  The loads of plane p+2 go
-// from HBM straight into LDS (global_load_lds_dwordx4: each lane's 16 bytes
-// land at the wave's LDS base + 16 x lane, so a wave moves 1 KiB of a row),
-// issued after the wait at the top of plane p into the buffer plane p-2 held
-// (last read before plane p-1's closing barrier) and waited for at the top
-// of plane p+1, before the barrier that publishes it.  Without the
-// registers of the phi_n ring and of its prefetch the tile fits 768 threads
-// at 3 waves per SIMD without spilling; pi_{n-1/2} of the next plane is
-// still prefetched into registers.
+// 3D FP32 only: eight sites per thread vector, one vector per thread, rows of
+// a multiple of 256 sites.  A block marching the slow axis i chains S steps
+// plane by plane, as tile_step3 does for three: step 0 (n -> n+1) on plane p
+// for TR + 2(S-1) rows, step l on plane p-l for TR + 2(S-1-l) rows, the
+// last step (S-1) on plane p-S+1 for the TR owned rows, which go to HBM.
+// Each site's arithmetic is that of S single steps, so fields are
+// bit-identical to the one-step kernels.  Level l (phi_{n+l}, l >= 1) keeps
+// its planes p-l-1, p-l, p-l+1 in a register ring and its plane p-l in LDS
+// for the next step's j and k taps.  Level 0 (phi_n) lives in four LDS
+// buffers instead: planes p-1, p, p+1 (step 0's i taps and its plane's j
+// and k taps) and p+2, which arrives while plane p is computed.  The loads
+// of plane p+2 go from HBM straight into LDS (global_load_lds_dwordx4: each
+// lane's 16 bytes land at the wave's LDS base + 16 x lane, so a wave moves
+// 1 KiB of a row), issued after the wait at the top of plane p into the
+// buffer plane p-2 held (last read before plane p-1's closing barrier) and
+// waited for at the top of plane p+1, before the barrier that publishes it.
+// Without a register ring and prefetch for phi_n the tile fits eight sites
+// per thread at three waves per SIMD without spilling; pi_{n-1/2} of the
+// next plane is still prefetched into registers.
+//   S = 3: TR = 8 at N = 512 (12 rows of step 0 x 64 vectors, 768 threads),
+//          1.25x the owned rows recomputed, 158 VGPRs;
+//   S = 4: TR = 4 at N = 512 (10 rows x 64 vectors, 640 threads) would move
+//          three quarters of the HBM bytes per step, but its three rings and
+//          six pi vectors need more than the 168 VGPRs of three waves per
+//          SIMD (200 bytes of scratch per lane: 0.575 ms/step at 3D N = 512
+//          against 0.135), so only S = 3 is launched.
 #pragma once
 
 #include <type_traits>
@@ -37,18 +46,22 @@ __device__ __forceinline__ void lds_dma16(const float *src, float *dst)
     __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
 }
 
-constexpr int kDmaThreads = 768;
+// threads per block of the S-step direct-load tile
+template <int S>
+constexpr int kDmaThreads = S == 3 ? 768 : 640;
 
-template <bool ENERGY>
-__device__ __forceinline__ void tile_step3_dma(const SimConst &k, const TiledGeom &g, int lt,
+template <int S, bool ENERGY>
+__device__ __forceinline__ void tile_stepS_dma(const SimConst &k, const TiledGeom &g, int lt,
                                                const float *__restrict__ phi_in, float *__restrict__ phi_out,
                                                const float *__restrict__ pi_in, float *__restrict__ pi_out,
                                                const DevScalars *__restrict__ sc, double *__restrict__ partials,
                                                float *lds)
 {
+    static_assert(S == 3 || S == 4, "three or four steps per pass");
     constexpr int VW = 8;
     constexpr int D = 3;
-    constexpr int W = kDmaThreads / 64;      // waves per block
+    constexpr int W = kDmaThreads<S> / 64;   // waves per block
+    constexpr int NL = S - 1;                // register levels 1 .. S-1
     using T = float;
     using V = Vec<T, VW>;
     typedef T nvec __attribute__((ext_vector_type(VW)));
@@ -62,11 +75,19 @@ __device__ __forceinline__ void tile_step3_dma(const SimConst &k, const TiledGeo
     const int r0 = rt * TR;
     const int i0 = ic * g.NI;
     const int i1 = min(i0 + g.NI, N);
-    const int rows0 = TR + 6;                         // phi_n rows r0-3 .. r0+TR+2
+    const int rows0 = TR + 2 * S;                     // phi_n rows r0-S .. r0+TR+S-1
     const size_t pl0 = (size_t)rows0 * N;
     float *L0b = lds;                                 // four phi_n plane buffers
-    float *L1 = L0b + 4 * pl0;                        // phi_n+1 plane p-1, rows r0-2 .. r0+TR+1
-    float *L2 = L1 + (size_t)(TR + 4) * N;            // phi_n+2 plane p-2, rows r0-1 .. r0+TR
+    // Ll[l-1]: phi_n+l at plane p-l, rows r0-S+l .. r0+TR+S-1-l (TR + 2S - 2l rows)
+    float *Ll[NL];
+    {
+        float *q = L0b + 4 * pl0;
+#pragma unroll
+        for (int l = 1; l <= NL; ++l) {
+            Ll[l - 1] = q;
+            q += (size_t)(TR + 2 * S - 2 * l) * N;
+        }
+    }
     const bool nt = g.nt != 0;
 
     auto wrapN = [&](int i) { return i < 0 ? i + N : (i >= N ? i - N : i); };
@@ -75,15 +96,18 @@ __device__ __forceinline__ void tile_step3_dma(const SimConst &k, const TiledGeo
     const T fac_lap = (T)sc->fac_lap, fac_pot = (T)sc->fac_pot;
     const StepF32 sf = step_f32(k, sc);
 
-    // this thread's vector: row rowA of the step-A rows (tile row r0-2+rowA)
+    // this thread's vector: row rowA of step 0's rows (tile row r0-S+1+rowA)
     const int q = threadIdx.x;
-    const bool ok = q < (TR + 4) * NV;
+    const int rowsA = TR + 2 * S - 2;
+    const bool ok = q < rowsA * NV;
     const int qq = ok ? q : 0;
     const int rowA = qq / NV;
     const int col = (qq - rowA * NV) * VW;
-    const long long off = (long long)wrapN(r0 - 2 + rowA) * N + col;
-    const bool inB = ok && rowA >= 1 && rowA < 3 + TR;
-    const bool inC = ok && rowA >= 2 && rowA < 2 + TR;
+    const long long off = (long long)wrapN(r0 - S + 1 + rowA) * N + col;
+    // in[l]: the thread takes part in step l (rows l .. rowsA-1-l)
+    bool in[S];
+#pragma unroll
+    for (int l = 0; l < S; ++l) in[l] = ok && rowA >= l && rowA < rowsA - l;
     const int l0row = (rowA + 1) * N + col;           // the vector's place in an L0 buffer
 
     // direct loads of one plane of phi_n into buffer b: the plane's rows0
@@ -96,12 +120,12 @@ __device__ __forceinline__ void tile_step3_dma(const SimConst &k, const TiledGeo
         float *dst = L0b + (size_t)b * pl0;
         for (int d = wv; d < npieces; d += W) {
             const int r = d / segs, s = d - r * segs;
-            lds_dma16(phi_in + base + (long long)wrapN(r0 - 3 + r) * N + s * 256 + lane * 4, dst + r * N + s * 256);
+            lds_dma16(phi_in + base + (long long)wrapN(r0 - S + r) * N + s * 256 + lane * 4, dst + r * N + s * 256);
         }
     };
 
-    V f1[3], f2[3];                  // phi_n+1, phi_n+2 rings (as in tile_step3)
-    V q1c, q1n, q2c, q2n;            // pi_n+1/2 at planes p-1, p; pi_n+3/2 at p-2, p-1
+    V f[NL][3];                      // level l+1 ring: planes p-l-2, p-l-1, p-l (by S3)
+    V qc[NL], qn[NL];                // pi_{n+l+1/2} at planes p-l-1 (this trip), p-l (next)
     nvec pr;                         // pi_n-1/2 of the next plane, loaded a plane ahead
 
     double acc[NQ];
@@ -109,12 +133,13 @@ __device__ __forceinline__ void tile_step3_dma(const SimConst &k, const TiledGeo
     for (int u = 0; u < NQ; ++u) acc[u] = 0.0;
     double nacc[NQ];
 
-    // buffers by the plane counter t = p - (i0 - 2): plane p in buffer (t + 1) & 3
-    dma_plane(i0 - 3, 0);
-    dma_plane(i0 - 2, 1);
-    dma_plane(i0 - 1, 2);
+    // plane counter t = p - p_first: plane p in buffer (t + 1) & 3
+    const int p_first = i0 - S + 1, p_last = i1 + S - 2;
+    dma_plane(p_first - 1, 0);
+    dma_plane(p_first, 1);
+    dma_plane(p_first + 1, 2);
     pr = (nvec)0.0f;
-    if (ok) pr = *reinterpret_cast<const nvec *>(pi_in + (long long)wrapN(i0 - 2) * NN + off);
+    if (ok) pr = *reinterpret_cast<const nvec *>(pi_in + (long long)wrapN(p_first) * NN + off);
 
     auto plane = [&](auto s3, int p, int t) __attribute__((always_inline)) {
         constexpr int S3 = decltype(s3)::value;
@@ -127,52 +152,59 @@ __device__ __forceinline__ void tile_step3_dma(const SimConst &k, const TiledGeo
         V pi0;
 #pragma unroll
         for (int e = 0; e < VW; ++e) pi0.v[e] = pr[e];
-        const bool doB = p >= i0;            // step B on plane p-1 in [i0-1, i1]
-        const bool doC = p >= i0 + 2;        // step C on plane p-2 in [i0, i1-1]
-        if (doB && ok) *reinterpret_cast<V *>(L1 + rowA * N + col) = f1[S3];
-        if (doC && inB) *reinterpret_cast<V *>(L2 + (rowA - 1) * N + col) = f2[S3];
+        // step l on plane p-l (1 <= l <= S-1) runs from p >= i0 - S + 1 + 2l
+        bool doS[S];
+#pragma unroll
+        for (int l = 0; l < S; ++l) doS[l] = p >= p_first + 2 * l;
+        // stage each level's plane p-l for its step's j and k taps
+#pragma unroll
+        for (int l = 1; l < S; ++l)
+            if (doS[l] && in[l - 1]) *reinterpret_cast<V *>(Ll[l - 1] + (rowA - l + 1) * N + col) = f[l - 1][S3];
         // plane p+2 into the buffer plane p-2 held; pi of plane p+1
         dma_plane(p + 2, (t + 3) & 3);
         if (ok) pr = *reinterpret_cast<const nvec *>(pi_in + (long long)wrapN(p + 1) * NN + off);
         lds_barrier();
-        // step A on plane p, all TR + 4 rows: phi_n at p-1, p, p+1 from LDS
+        // step 0 on plane p: phi_n at p-1, p, p+1 from LDS
         if (ok) {
             const V c = *reinterpret_cast<const V *>(Lc + l0row);
             const V mM = *reinterpret_cast<const V *>(Lm + l0row);
             const V mP = *reinterpret_cast<const V *>(Lp + l0row);
-            leap_vec<T, D, VW, false>(k, c, mM, mP, Lc + (rowA + 1) * N, col, pi0, fac_lap, fac_pot, sf, dt, q1n,
-                                      f1[NX], sc, nacc);
+            leap_vec<T, D, VW, false>(k, c, mM, mP, Lc + (rowA + 1) * N, col, pi0, fac_lap, fac_pot, sf, dt, qn[0],
+                                      f[0][NX], sc, nacc);
         }
-        // step B on plane p-1, TR + 2 rows
-        if (doB && inB)
-            leap_vec<T, D, VW, false>(k, f1[S3], f1[PV], f1[NX], L1 + rowA * N, col, q1c, fac_lap, fac_pot, sf, dt,
-                                      q2n, f2[NX], sc, nacc);
-        // step C on plane p-2, owned rows: to HBM
-        if (doC && inC) {
-            const long long baseq = (long long)wrapN(p - 2) * NN;
+        // steps 1 .. S-2 on planes p-l
+#pragma unroll
+        for (int l = 1; l < S - 1; ++l)
+            if (doS[l] && in[l])
+                leap_vec<T, D, VW, false>(k, f[l - 1][S3], f[l - 1][PV], f[l - 1][NX], Ll[l - 1] + (rowA - l + 1) * N,
+                                          col, qc[l - 1], fac_lap, fac_pot, sf, dt, qn[l], f[l][NX], sc, nacc);
+        // the last step on plane p-S+1, owned rows: to HBM
+        if (doS[S - 1] && in[S - 1]) {
+            const long long baseq = (long long)wrapN(p - S + 1) * NN;
             V pn, phn;
-            leap_vec<T, D, VW, ENERGY>(k, f2[S3], f2[PV], f2[NX], L2 + (rowA - 1) * N, col, q2c, fac_lap, fac_pot,
-                                       sf, dt, pn, phn, sc, acc);
+            leap_vec<T, D, VW, ENERGY>(k, f[NL - 1][S3], f[NL - 1][PV], f[NL - 1][NX],
+                                       Ll[NL - 1] + (rowA - S + 2) * N, col, qc[NL - 1], fac_lap, fac_pot, sf, dt,
+                                       pn, phn, sc, acc);
             store_vec<T, VW>(pi_out + baseq + off, pn, nt);
             store_vec<T, VW>(phi_out + baseq + off, phn, nt);
         }
         lds_barrier();
-        q1c = q1n;
-        q2c = q2n;
+#pragma unroll
+        for (int l = 0; l < NL; ++l) qc[l] = qn[l];
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
     using I2 = std::integral_constant<int, 2>;
     // (p is uniform over the block: every wave leaves at the same plane)
-    for (int p = i0 - 2, t = 0; p <= i1 + 1; p += 3, t += 3) {
+    for (int p = p_first, t = 0; p <= p_last; p += 3, t += 3) {
         plane(I0{}, p, t);
-        if (p + 1 > i1 + 1) break;
+        if (p + 1 > p_last) break;
         plane(I1{}, p + 1, t + 1);
-        if (p + 2 > i1 + 1) break;
+        if (p + 2 > p_last) break;
         plane(I2{}, p + 2, t + 2);
     }
-    // the loads of the last two planes' look-ahead (beyond the chunk) are
-    // waited for before the block ends
+    // the direct loads of the look-ahead planes (beyond the chunk) land
+    // before the block ends and its LDS goes to another block
     vm_wait_all();
     if constexpr (ENERGY) energy_partials(acc, k, sc, partials + (long long)lt * NQ);
 }
