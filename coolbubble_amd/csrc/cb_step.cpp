@@ -351,7 +351,10 @@ extern "C" cb_status cb_autotune(cb_sim *s, int steps, double *ms_per_step, int 
         for (int rep = 0; rep < reps && e == hipSuccess; ++rep) {
             e = restore();
             if (e != hipSuccess) break;
-            if (step_n(s, 2) != CB_OK) { e = hipErrorUnknown; break; }
+            // an untimed call of the same length first: every kernel the
+            // timed calls launch has then been launched (and its code
+            // loaded) once
+            if (step_n(s, rep == 0 ? nsteps : 2) != CB_OK) { e = hipErrorUnknown; break; }
             e = hipEventRecord(ev[0], s->st);
             // each call measures energies on its last step only, as a run
             // printing every `nsteps` steps does
