@@ -162,8 +162,9 @@ cb_status cb_select_kernel(cb_sim *s, int variant);
  * box only)}. */
 cb_status cb_kernel_variant_info(cb_sim *s, int variant, int info[8]);
 
-/* Autotune: time every variant over one call of `steps` steps, then the 8
- * fastest again over four calls of `steps` steps in a row (each call
+/* Autotune: time every variant over one call of `steps` steps (after an
+ * untimed call of the same length, so that its kernels are loaded), then the
+ * 8 fastest again over four calls of `steps` steps in a row (each call
  * measures the energies on its last step, as a run printing every `steps`
  * steps does; the lattice state is saved and restored around the trials),
  * and select the fastest.  ms_per_step (optional, capacity cap) receives
