@@ -62,18 +62,21 @@ __device__ inline void sum_partials_block(const double *partials, int nblocks, d
 #pragma unroll
     for (int q = 0; q < NV; ++q) n[q] = {0.0, 0.0};
     // Each thread adds rows t, t + 256, t + 512, ... in that order; the rows
-    // are loaded four at a time so that their latencies overlap (one block
-    // alone on the GPU has nothing else to hide them behind).
+    // are loaded U at a time so that their latencies overlap (one block
+    // alone on the GPU has nothing else to hide them behind): 16 when a row
+    // holds few quantities (the Friedmann sums of an expanding step), 4
+    // for all of them.
+    constexpr int U = NV <= 4 ? 16 : 4;
     const int stride = blockDim.x;
     int b = t;
-    for (; b + 3 * stride < nblocks; b += 4 * stride) {
-        double v[4][NV];
+    for (; b + (U - 1) * stride < nblocks; b += U * stride) {
+        double v[U][NV];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int q = 0; q < NV; ++q) v[u][q] = partials[(long long)(b + u * stride) * NQ + q];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int q = 0; q < NV; ++q) n[q].add(v[u][q]);
     }

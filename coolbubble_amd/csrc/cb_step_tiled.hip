@@ -496,7 +496,10 @@ bool tiled_valid(const SimConst &k, int fp32, const TiledCfg &c)
     if (c.vw != want_vw) return false;
     if (k.N % c.vw != 0 || k.N < 4) return false;
     if (c.items != 1 && c.items != 2 && c.items != 4) return false;
-    if (c.nthr != 128 && c.nthr != 256) return false;
+    // 192 and 384 threads (3D, one or two vectors per thread): tiles whose
+    // vector count is a multiple of 3 x 64 (N = 3 x 2^m) without idle lanes
+    const bool three = k.D == 3 && (c.nthr == 192 || c.nthr == 384) && c.items <= 2;
+    if (c.nthr != 128 && c.nthr != 256 && !three) return false;
     if (c.NI < 1) return false;
     if (k.D == 3 && (c.TR < 1 || k.N % c.TR != 0)) return false;
     const int rows = (k.D == 3) ? c.TR : 1;
@@ -538,13 +541,18 @@ static int single_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap
     if (k.D == 3) {
         const int trs[] = {8, 4, 2, 1};
         const int nis[] = {32, 16, 64, 8};
-        for (int TR : trs)
+        for (int TR : trs) {
+            const int nv = TR * (k.N / vw);
             for (int nthr : {256, 128}) {
-                const int nv = TR * (k.N / vw);
                 int items = (nv + nthr - 1) / nthr;
                 if (items == 3) items = 4;
                 for (int NI : nis) push(items, nthr, TR, NI);
             }
+            // whole multiples of 192 or 384 vectors: every lane has a vector
+            for (int nthr : {192, 384})
+                if (nv % nthr == 0 && nv / nthr <= 2)
+                    for (int NI : nis) push(nv / nthr, nthr, TR, NI);
+        }
     } else {
         const int nis[] = {16, 8, 32, 4};
         for (int nthr : {128, 256}) {
@@ -584,6 +592,18 @@ template <typename T, int D, int PF>
 static void launch_p(const SimConst &k, int lagged, int energy, const TiledCfg &c, const StepPtrs &p,
                      const DevScalars *sc, double *partials, hipStream_t st, size_t shm)
 {
+    if constexpr (D == 3) {
+        if (c.nthr == 192 || c.nthr == 384) {
+            if (c.nthr == 192) {
+                if (c.items == 1) launch_t<T, D, 1, 192, PF>(k, lagged, energy, c, p, sc, partials, st, shm);
+                else launch_t<T, D, 2, 192, PF>(k, lagged, energy, c, p, sc, partials, st, shm);
+            } else {
+                if (c.items == 1) launch_t<T, D, 1, 384, PF>(k, lagged, energy, c, p, sc, partials, st, shm);
+                else launch_t<T, D, 2, 384, PF>(k, lagged, energy, c, p, sc, partials, st, shm);
+            }
+            return;
+        }
+    }
     if (c.nthr == 256) {
         if (c.items == 1) launch_t<T, D, 1, 256, PF>(k, lagged, energy, c, p, sc, partials, st, shm);
         else if (c.items == 2) launch_t<T, D, 2, 256, PF>(k, lagged, energy, c, p, sc, partials, st, shm);

@@ -409,6 +409,42 @@ def test_wide_row_variants_bit_identical(D, N, precision):
         g.close()
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("N,expanding", [(96, False), (192, True), (384, True)])
+def test_three_wave_one_step_tiles_bit_identical(N, expanding, precision):
+    """One-step tiles of 192 and 384 threads (3D lattices whose rows hold a
+    multiple of three 64-lane waves of vectors, N = 3 x 2^m): fields bit for
+    bit the general kernel's over a call of 7 steps, energy rows within
+    1e-13; static and expanding (lagged drift, Friedmann sums only on the
+    steps before the last).  N = 384 runs the two-vector 384-thread tiles
+    only (the others are covered at N = 96 and 192)."""
+    if expanding:
+        kw = params(D=3, readxns=-3, nGrids=N, L=N / 8.0, c=2.0, Trad=0.2, expansion=1, Mpl=10.0)
+    else:
+        kw = params(D=3, readxns=-2, nGrids=N, L=N / 8.0)
+    ref = cb.Bubble(precision=precision, **kw)
+    ref.select_kernel(-1)
+    ref.step(7)
+    phi0, pi0 = ref.fields()
+    e0 = ref.energy_row()
+    n, _ = ref.kernel_variants()
+    infos = [ref.variant_info(v) for v in range(n)]
+    ref.close()
+    three = [v for v in range(n) if infos[v]["fuse"] == 1 and infos[v]["nthr"] in (192, 384)]
+    if N == 384:
+        three = [v for v in three if infos[v]["nthr"] == 384 and infos[v]["items"] == 2]
+    assert three
+    for v in three:
+        g = cb.Bubble(precision=precision, **kw)
+        g.select_kernel(v)
+        g.step(7)
+        phi, pi = g.fields()
+        np.testing.assert_array_equal(phi, phi0, err_msg=str(infos[v]))
+        np.testing.assert_array_equal(pi, pi0, err_msg=str(infos[v]))
+        assert np.max(row_errors(e0[None], g.energy_row()[None])) <= 1e-13, infos[v]
+        g.close()
+
+
 def test_default_variant_follows_the_lattice():
     """cb_create starts small lattices on the general kernel, large static
     boxes on a three-step tile, expanding ones on a one-step tile."""
