@@ -144,7 +144,9 @@ __device__ inline void neighbours(long long s, int N, long long (&nb)[6])
 // double.  gP = {forward neighbour along axis 1, 0, 2} per the reference's
 // gx, gy, gz naming (1D: gx only).  EXACT: the FP64 path.
 // LITE: only the sums the Friedmann update reads (src/bubble.cpp:747-757):
-// kinetic and potential, gradient in conformal time.
+// kinetic, potential and gradient (read in conformal time only, but summed
+// whatever alpha is: a branch on alpha here raised the one-step FP32 tile
+// from 78 to 151 VGPRs, three waves per SIMD instead of six).
 template <int D, bool EXACT, bool LITE = false>
 __device__ inline void energy_terms(double ph, const double (&fwd)[3], double pn, double po,
                                     const SimConst &k, const DevScalars *sc, double (&acc)[NQ])
@@ -163,7 +165,7 @@ __device__ inline void energy_terms(double ph, const double (&fwd)[3], double pn
             acc[Q_POT] += ((k.c * ph2 - k.c5 * (ph2 * ph)) + k.c6 * (ph2 * ph2)) + k.DeltaV;
         }
         acc[Q_KIN] += 0.5 * pk2 * pn * pn;
-        if (k.alpha != 0) {
+        {
             double g0 = 0., g1 = 0., g2 = 0.;
             if constexpr (D == 1) {
                 g0 = (fwd[0] - ph) * iadx;
