@@ -113,6 +113,8 @@ struct cb_server {
     std::chrono::steady_clock::time_point t0;
     cb::BatchMember *h_tab = nullptr, *d_tab = nullptr;
     size_t tab_cap = 0;
+    cb::DevScalars *h_sc = nullptr;          // pinned: each active request's scalars after a tick
+    size_t sc_cap = 0;
     int cus = 256;
 
     double now() const
@@ -258,6 +260,16 @@ cb_status tick(cb_server *sv)
             return CB_OUT_OF_MEMORY;
         }
     }
+    if (act.size() > sv->sc_cap) {
+        if (sv->h_sc) hipHostFree(sv->h_sc);
+        sv->h_sc = nullptr;
+        sv->sc_cap = act.size() * 2;
+        if (hipHostMalloc((void **)&sv->h_sc, sizeof(cb::DevScalars) * sv->sc_cap, hipHostMallocDefault) != hipSuccess) {
+            sv->sc_cap = 0;
+            set_error("server scalar staging allocation failed");
+            return CB_OUT_OF_MEMORY;
+        }
+    }
     const int fp32 = sv->o.precision == CB_FP32;
     // one-step table: L launches, energies where a member ends (every step if one expands)
     int tiles1 = 0, L1 = 0;
@@ -329,14 +341,26 @@ cb_status tick(cb_server *sv)
     }
     for (size_t i : single) cb::advance_host(act[i]->sim, m[i]);
     for (size_t i : pair) cb::advance_host_multi(act[i]->sim, m[i]);
+    // the scalars of the requests with a row at this step, copied after the
+    // tick's launches (one stream) and waited for with them
+    for (size_t i = 0; i < act.size(); ++i) {
+        const long long step = act[i]->sim->step;
+        if (step % 40 != 0 && step != act[i]->r.nt) continue;
+        if (hipMemcpyAsync(&sv->h_sc[i], act[i]->sim->dsc, sizeof(cb::DevScalars), hipMemcpyDeviceToHost, sv->st) !=
+            hipSuccess) {
+            set_error("server tick: scalar copy failed");
+            return CB_HIP_ERROR;
+        }
+    }
     if (hipStreamSynchronize(sv->st) != hipSuccess) { set_error("server tick: stream failed"); return CB_HIP_ERROR; }
     // events
     std::vector<Req *> keep;
-    for (Req *q : act) {
+    for (size_t i = 0; i < act.size(); ++i) {
+        Req *q = act[i];
         const long long step = q->sim->step;
         double row[16];
         bool have = false;
-        if (step % 40 == 0 || step == q->r.nt) { read_row(q->sim, row); have = true; }
+        if (step % 40 == 0 || step == q->r.nt) { cb::row_from_scalars(q->sim, sv->h_sc[i], row); have = true; }
         if (step % 40 == 0) q->rows.insert(q->rows.end(), row, row + 16);
         if (q->next_snap < q->snaps.size() && q->snaps[q->next_snap] == step) {
             record_spectrum(q);
@@ -573,6 +597,7 @@ cb_status cb_server_destroy(cb_server *sv)
     for (auto &kv : sv->kept) delete kv.second;
     if (sv->d_tab) hipFree(sv->d_tab);
     if (sv->h_tab) hipHostFree(sv->h_tab);
+    if (sv->h_sc) hipHostFree(sv->h_sc);
     if (sv->st) hipStreamDestroy(sv->st);
     delete sv;
     return CB_OK;

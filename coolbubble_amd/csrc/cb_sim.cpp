@@ -374,6 +374,26 @@ cb_status create_sim(const cb_params *p, const double *xns, int n_xns, bool keep
 }
 cb_status materialise_drift(cb_sim *s) { return materialise(s); }
 cb_status sim_scalars(cb_sim *s, DevScalars &h) { return read_scalars(s, h); }
+bool row_from_scalars(const cb_sim *s, const DevScalars &h, double c[16])
+{
+    c[0] = s->t;
+    c[1] = h.a;
+    c[2] = h.Etot;
+    c[3] = h.Ekintot;
+    c[4] = h.Egradtot;
+    c[5] = h.Epottot;
+    c[6] = std::sqrt(h.EtotVar);
+    c[7] = std::sqrt(h.EkintotVar);
+    c[8] = std::sqrt(h.EgradtotVar);
+    c[9] = std::sqrt(h.EpottotVar);
+    c[10] = h.avphi;
+    c[11] = std::sqrt(h.avphiVar);
+    c[12] = h.avphiP;
+    c[13] = std::sqrt(h.avphiPVar);
+    c[14] = h.H;
+    c[15] = h.C;
+    return !h.nan_flag;
+}
 cb_status construct_sim(cb_sim *s, const double *xns, int n_xns) { return construct(s, xns, n_xns); }
 void release_sim(cb_sim *s) { release(s); }
 bool params_ok(const cb_params *p, std::string &why) { return valid_params(p, why); }
@@ -434,24 +454,7 @@ cb_status cb_energies_row(cb_sim *s, cb_energies *out)
     cb::DevScalars h;
     cb_status r = read_scalars(s, h);
     if (r != CB_OK) return r;
-    double *c = out->col;
-    c[0] = s->t;
-    c[1] = h.a;
-    c[2] = h.Etot;
-    c[3] = h.Ekintot;
-    c[4] = h.Egradtot;
-    c[5] = h.Epottot;
-    c[6] = std::sqrt(h.EtotVar);
-    c[7] = std::sqrt(h.EkintotVar);
-    c[8] = std::sqrt(h.EgradtotVar);
-    c[9] = std::sqrt(h.EpottotVar);
-    c[10] = h.avphi;
-    c[11] = std::sqrt(h.avphiVar);
-    c[12] = h.avphiP;
-    c[13] = std::sqrt(h.avphiPVar);
-    c[14] = h.H;
-    c[15] = h.C;
-    if (h.nan_flag) { set_error("non-finite energy detected"); return CB_NAN_DETECTED; }
+    if (!cb::row_from_scalars(s, h, out->col)) { set_error("non-finite energy detected"); return CB_NAN_DETECTED; }
     return CB_OK;
 }
 

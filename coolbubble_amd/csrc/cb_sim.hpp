@@ -69,6 +69,9 @@ cb_status create_sim(const cb_params *p, const double *xns, int n_xns, bool keep
 cb_status materialise_drift(cb_sim *s);
 // the device scalars of s (synchronises its stream)
 cb_status sim_scalars(cb_sim *s, DevScalars &h);
+// the 16 outEP columns of s at its current step from its device scalars h
+// (read back by the caller); false when h flags a non-finite energy
+bool row_from_scalars(const cb_sim *s, const DevScalars &h, double col[16]);
 cb_status construct_sim(cb_sim *s, const double *xns, int n_xns);
 void release_sim(cb_sim *s);
 bool params_ok(const cb_params *p, std::string &why);
