@@ -229,14 +229,14 @@ int default_variant(const cb_sim *s)
     if (s->variants.empty() || (D == 3 ? s->nsites < (1LL << 19) : N < 512)) return -1;
     struct Want { int fuse, nthr, TR, NI, pf, nt; } w;
     const bool f32 = s->fp32 != 0;
-    // expanding: one step per launch; the smallest tiles of 2 rows from
-    // N = 384 (192 threads where N is 3 x 2^m), taller ones below; plain
+    // expanding: one step per launch; from N = 320 tiles of 2 rows, or of
+    // 4 rows x 384 threads in chunks of 48 planes where N is a multiple of
+    // 48 (N = 384: 768 blocks, three per CU), taller ones below; plain
     // stores while phi and pi fit the 256 MiB last-level cache
     const int nt_big = 2.0 * (double)s->nsites * (double)s->esz > 256.0 * 1048576.0 ? 1 : 0;
-    const int n3 = N % 3 == 0 ? 192 : 256;
     if (s->lagged && N <= 160) w = f32 ? Want{1, 256, 8, 8, 1, nt_big} : Want{1, 256, 4, 8, 1, nt_big};
     else if (s->lagged && N <= 288) w = f32 ? Want{1, 256, 4, 16, 1, nt_big} : Want{1, 256, 2, 32, 2, nt_big};
-    else if (s->lagged) w = Want{1, n3, 2, n3 == 192 ? 64 : 32, 1, nt_big};
+    else if (s->lagged) w = N % 48 == 0 ? Want{1, 384, 4, 48, 1, nt_big} : Want{1, 256, 2, 32, 1, nt_big};
     else if (D == 3 && N >= 512) w = f32 ? Want{3, 1024, 4, 64, 2, 1} : Want{2, 1024, 4, 64, 2, 1};
     else if (D == 3 && N >= 320) w = f32 ? Want{3, 1024, 6, 32, 2, 1} : Want{3, 1024, 4, 32, 2, 1};
     else if (D == 3 && N >= 224) w = f32 ? Want{3, 1024, 8, 32, 2, 0} : Want{3, 1024, 4, 64, 2, 1};

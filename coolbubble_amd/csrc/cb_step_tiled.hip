@@ -548,10 +548,14 @@ static int single_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap
                 if (items == 3) items = 4;
                 for (int NI : nis) push(items, nthr, TR, NI);
             }
-            // whole multiples of 192 or 384 vectors: every lane has a vector
+            // whole multiples of 192 or 384 vectors: every lane has a vector;
+            // chunks of 48 planes too where they tile N (N = 384: 8 chunks,
+            // 6 two-row tiles of 192 threads per CU, all resident at once)
             for (int nthr : {192, 384})
-                if (nv % nthr == 0 && nv / nthr <= 2)
+                if (nv % nthr == 0 && nv / nthr <= 2) {
                     for (int NI : nis) push(nv / nthr, nthr, TR, NI);
+                    if (k.N % 48 == 0) push(nv / nthr, nthr, TR, 48);
+                }
         }
     } else {
         const int nis[] = {16, 8, 32, 4};
