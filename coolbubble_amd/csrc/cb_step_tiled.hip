@@ -298,11 +298,31 @@ __global__ void __launch_bounds__(256) k_batch_finalize(const BatchMember *__res
     finalize_member(m.k, m.partials, m.ntiles, m.sc, m.lagged ? 0 : 1);
 }
 
+// Blocks of a persistent batched launch of kernel `kern`: at most as many as
+// the GPU holds at once for it (by its registers and LDS), a multiple of 8
+// (one share per XCD).  A block beyond those would start only when a
+// resident one ends and then walk its whole share of tiles while most CUs
+// idle.  The tiles' arithmetic and partials do not depend on the grid.
+template <typename K>
+static int resident_grid(K kern, size_t shm, int grid)
+{
+    int nb = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(kern), 256, shm) != hipSuccess ||
+        nb < 1 || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        return grid;
+    int cap = nb * cus;
+    cap -= cap % 8;
+    return (cap >= 8 && cap < grid) ? cap : grid;
+}
+
 template <typename T, bool ENERGY, int DSEL>
 static void batch_launch_ip(int items, int pf, const BatchMember *table, int nmem, int total, int grid, size_t shm,
                             int step_it, int sel, hipStream_t st)
 {
-#define CB_BATCH_L(I, P) k_batch_step<T, ENERGY, I, P, DSEL><<<grid, 256, shm, st>>>(table, nmem, total, step_it, sel)
+#define CB_BATCH_L(I, P)                                                                                          \
+    k_batch_step<T, ENERGY, I, P, DSEL><<<resident_grid(k_batch_step<T, ENERGY, I, P, DSEL>, shm, grid), 256, shm, st>>>( \
+        table, nmem, total, step_it, sel)
     if (pf == 2) {
         if (items == 1) CB_BATCH_L(1, 2);
         else if (items == 2) CB_BATCH_L(2, 2);
@@ -420,8 +440,12 @@ template <typename T, bool ENERGY, int DSEL, int FSEL>
 static void batch_multi_launch_i(int items, const BatchMember *table, int nmem, int total, int grid, size_t shm, int L,
                                  int sel, hipStream_t st)
 {
-    if (items == 1) k_batch_multi<T, ENERGY, 1, DSEL, FSEL><<<grid, 256, shm, st>>>(table, nmem, total, L, sel);
-    else k_batch_multi<T, ENERGY, 2, DSEL, FSEL><<<grid, 256, shm, st>>>(table, nmem, total, L, sel);
+    if (items == 1)
+        k_batch_multi<T, ENERGY, 1, DSEL, FSEL><<<resident_grid(k_batch_multi<T, ENERGY, 1, DSEL, FSEL>, shm, grid), 256,
+                                                 shm, st>>>(table, nmem, total, L, sel);
+    else
+        k_batch_multi<T, ENERGY, 2, DSEL, FSEL><<<resident_grid(k_batch_multi<T, ENERGY, 2, DSEL, FSEL>, shm, grid), 256,
+                                                 shm, st>>>(table, nmem, total, L, sel);
 }
 
 template <typename T, int FSEL>
