@@ -1,10 +1,10 @@
 """FP64 GPU path vs the oracle on identical seeded inputs.
 
 Bar: fields, energy rows and spectrum bins agree to 1e-12 relative (norm-wise
-for fields: max|diff| <= 1e-12 max|oracle|), the tolerance the north star sets
-for the FP64 path.  Each case runs the reference sequence: Bubble ctor
-(src/bubble.cpp:156-316), then oneStep n times (:849-876), reading the
-outEP row (:977-989) and the fields (OutField, :887-917).
+for fields: max|diff| <= 1e-12 max|oracle|), the tolerance of the FP64 path.
+Each case runs the reference sequence: Bubble ctor (src/bubble.cpp:156-316),
+then oneStep n times (:849-876), reading the outEP row (:977-989) and the
+fields (OutField, :887-917).
 """
 import numpy as np
 import pytest
