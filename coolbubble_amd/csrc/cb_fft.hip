@@ -439,11 +439,18 @@ k_bin_reduce(const double *__restrict__ binpart, int nblk, int nq, double *__res
 // mt19937_64 seeded with seed + r (state words interleaved across rows, so
 // the 312-word twists are coalesced), libstdc++'s generate_canonical and
 // Marsaglia polar method, sigma = sqrt(P(T, k^2, m) / 2), re drawn before im.
+// The state lives in global memory, where the compiler cannot tell the
+// words apart, so every access would wait for the one before it: the twist
+// loads 4 words of each operand before it stores 4 new ones (larger blocks
+// cost more registers than the occupancy they leave is worth), and the
+// outputs are read 8 words at a time into registers.
 // ---------------------------------------------------------------------------
 struct MtRow {
     unsigned long long *st;
     long long rows, r;
-    int mti;
+    int mti;                      // next word to read into the ring
+    int rpos;                     // next word of the ring to temper
+    unsigned long long ring[8];
     __device__ unsigned long long &w(int i) { return st[(long long)i * rows + r]; }
     __device__ void seed(unsigned long long s)
     {
@@ -454,25 +461,52 @@ struct MtRow {
             w(i) = x;
         }
         mti = 312;
+        rpos = 8;
+    }
+    static __device__ unsigned long long mix(unsigned long long hi, unsigned long long lo, unsigned long long far)
+    {
+        const unsigned long long UM = 0xFFFFFFFF80000000ULL, LM = 0x7FFFFFFFULL, A = 0xB5026F5AA96619E9ULL;
+        const unsigned long long x = (hi & UM) | (lo & LM);
+        return far ^ (x >> 1) ^ ((x & 1ULL) ? A : 0ULL);
+    }
+    // new words i0 .. i0+K-1 from old words i0 .. i0+K and words i0+off ..
+    // (old ones above the block, new ones below it)
+    template <int K>
+    __device__ void twist_block(int i0, int off)
+    {
+        unsigned long long a[K + 1], b[K];
+#pragma unroll
+        for (int k = 0; k <= K; ++k) a[k] = w(i0 + k);
+#pragma unroll
+        for (int k = 0; k < K; ++k) b[k] = w(i0 + off + k);
+#pragma unroll
+        for (int k = 0; k < K; ++k) w(i0 + k) = mix(a[k], a[k + 1], b[k]);
+    }
+    __device__ void twist()
+    {
+#pragma unroll 1
+        for (int i0 = 0; i0 < 156; i0 += 4) twist_block<4>(i0, 156);      // 39 blocks
+#pragma unroll 1
+        for (int i0 = 156; i0 < 300; i0 += 4) twist_block<4>(i0, -156);    // 36 blocks
+#pragma unroll 1
+        for (int i = 300; i < 311; ++i) w(i) = mix(w(i), w(i + 1), w(i - 156));
+        w(311) = mix(w(311), w(0), w(155));
+        mti = 0;
     }
     __device__ unsigned long long next()
     {
-        const unsigned long long UM = 0xFFFFFFFF80000000ULL, LM = 0x7FFFFFFFULL, A = 0xB5026F5AA96619E9ULL;
-        if (mti >= 312) {
-            int i;
-            for (i = 0; i < 156; ++i) {
-                unsigned long long x = (w(i) & UM) | (w(i + 1) & LM);
-                w(i) = w(i + 156) ^ (x >> 1) ^ ((x & 1ULL) ? A : 0ULL);
-            }
-            for (; i < 311; ++i) {
-                unsigned long long x = (w(i) & UM) | (w(i + 1) & LM);
-                w(i) = w(i - 156) ^ (x >> 1) ^ ((x & 1ULL) ? A : 0ULL);
-            }
-            unsigned long long x = (w(311) & UM) | (w(0) & LM);
-            w(311) = w(155) ^ (x >> 1) ^ ((x & 1ULL) ? A : 0ULL);
-            mti = 0;
+        if (rpos == 8) {
+            if (mti >= 312) twist();
+#pragma unroll
+            for (int k = 0; k < 8; ++k) ring[k] = w(mti + k);   // 312 = 39 x 8
+            mti += 8;
+            rpos = 0;
         }
-        unsigned long long y = w(mti++);
+        unsigned long long y = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k == rpos) y = ring[k];
+        rpos += 1;
         y ^= (y >> 29) & 0x5555555555555555ULL;
         y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
         y ^= (y << 37) & 0xFFF7EEE000000000ULL;
@@ -495,7 +529,7 @@ k_thermal_modes(cpx<double> *H, unsigned long long *state, long long rows, int N
     if (r >= rows) return;
     const double PI = 3.14159265358979323846;
     const int nc = N / 2 + 1;
-    MtRow g{state, rows, r, 0};
+    MtRow g{state, rows, r, 0, 8, {}};
     g.seed(seed + (unsigned long long)r);
     auto kphys = [&](long long i) -> double {
         const long long s = (i <= N / 2) ? i : i - N;
