@@ -684,18 +684,22 @@ struct FftPlan {
     // device buffers from hipMalloc (a simulation's own plan, freed by
     // fft_destroy), or carved from the no-allocation scratch
     bool owned = true;
+    hipStream_t pool_st = nullptr;   // owned buffers from the memory pool, in order on this stream
 };
 
-// Device memory for a plan: hipMalloc when arena is null, else the next
-// 256-byte-aligned piece of the arena.
+// Device memory for a plan: hipMalloc when arena is null, the memory pool
+// in order on `pool` when the arena has no base, else the next 256-byte-
+// aligned piece of the arena.
 struct Arena {
     char *base = nullptr;
     size_t cap = 0, used = 0;
+    hipStream_t pool = nullptr;
 };
 
 static hipError_t dev_get(Arena *a, void **out, size_t bytes)
 {
     if (!a) return hipMalloc(out, bytes);
+    if (!a->base) return hipMallocAsync(out, bytes, a->pool);
     const size_t off = (a->used + 255) & ~(size_t)255;
     if (off > a->cap || bytes > a->cap - off) return hipErrorOutOfMemory;
     *out = a->base + off;
@@ -823,12 +827,12 @@ void fft_destroy(FftPlan *p)
         delete p;
         return;
     }
-    if (p->tw) hipFree(p->tw);
-    if (p->tw1) hipFree(p->tw1);
-    if (p->tw2) hipFree(p->tw2);
-    if (p->H) hipFree(p->H);
-    if (p->binpart) hipFree(p->binpart);
-    if (p->bins) hipFree(p->bins);
+    void *bufs[6] = {p->tw, p->tw1, p->tw2, p->H, p->binpart, p->bins};
+    for (void *b : bufs) {
+        if (!b) continue;
+        if (p->pool_st) hipFreeAsync(b, p->pool_st);
+        else hipFree(b);
+    }
     delete p;
 }
 
@@ -933,7 +937,10 @@ static cb_status plan_for(cb_sim *s)
         delete p;
         return r;
     }
-    hipError_t e = plan_buffers(p, nullptr, nullptr, s->p.L, s->st);
+    Arena pool{};
+    pool.pool = s->pool_st;
+    p->pool_st = s->pool_st;
+    hipError_t e = plan_buffers(p, s->pool_st ? &pool : nullptr, nullptr, s->p.L, s->st);
     if (e == hipSuccess) e = hipStreamSynchronize(s->st);
     if (e != hipSuccess) {
         fft_destroy(p);
@@ -1108,13 +1115,15 @@ cb_status thermal_field(cb_sim *s, void *field, double T, double m, int velocity
     const int nc = N / 2 + 1;
     void *tw = nullptr, *H = nullptr, *state = nullptr;
     auto cleanup = [&]() {
-        if (tw) hipFree(tw);
-        if (H) hipFree(H);
-        if (state) hipFree(state);
+        sim_free(s, tw);
+        sim_free(s, H);
+        sim_free(s, state);
     };
-    hipError_t e = make_twiddles<double>(N, &tw, s->st);
-    if (e == hipSuccess) e = hipMalloc(&H, sizeof(cpx<double>) * (size_t)rows * nc);
-    if (e == hipSuccess) e = hipMalloc(&state, sizeof(unsigned long long) * 312 * (size_t)rows);
+    Arena pool{};
+    pool.pool = s->pool_st;
+    hipError_t e = make_twiddles<double>(N, &tw, s->st, s->pool_st ? &pool : nullptr);
+    if (e == hipSuccess) e = sim_alloc(s, &H, sizeof(cpx<double>) * (size_t)rows * nc);
+    if (e == hipSuccess) e = sim_alloc(s, &state, sizeof(unsigned long long) * 312 * (size_t)rows);
     if (e != hipSuccess) {
         cleanup();
         set_error(std::string("thermal IC: ") + hipGetErrorString(e));

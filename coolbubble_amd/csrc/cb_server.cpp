@@ -200,6 +200,7 @@ void admit(cb_server *sv)
             s->p.device = sv->o.device;
             s->st = sv->st;
             s->own_stream = false;
+            s->pool_st = sv->st;    // freed requests' memory serves the next ones
             r = cb::construct_sim(s, nullptr, 0);
             if (r != CB_OK) { cb::release_sim(s); s = nullptr; }
         }
@@ -426,6 +427,14 @@ cb_status cb_server_create(const cb_server_opts *o, cb_server **out)
     }
     size_t freeb = 0, totalb = 0;
     hipMemGetInfo(&freeb, &totalb);
+    {   // requests allocate from the device's memory pool in stream order; the
+        // pool keeps what they free (a request's construction then costs no
+        // hipMalloc, its end no device-wide hipFree)
+        hipMemPool_t pool = nullptr;
+        unsigned long long keep = ~0ULL;
+        if (hipDeviceGetDefaultMemPool(&pool, o->device) == hipSuccess)
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     sv->budget = o->mem_budget > 0 ? o->mem_budget : (long long)(0.9 * (double)freeb);
     hipDeviceGetAttribute(&sv->cus, hipDeviceAttributeMultiprocessorCount, o->device);
     sv->t0 = std::chrono::steady_clock::now();
@@ -598,7 +607,14 @@ cb_status cb_server_destroy(cb_server *sv)
     if (sv->d_tab) hipFree(sv->d_tab);
     if (sv->h_tab) hipHostFree(sv->h_tab);
     if (sv->h_sc) hipHostFree(sv->h_sc);
-    if (sv->st) hipStreamDestroy(sv->st);
+    if (sv->st) {
+        hipStreamSynchronize(sv->st);   // the requests' frees have run
+        hipStreamDestroy(sv->st);
+    }
+    {   // what the requests freed goes back to the device
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, sv->o.device) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
+    }
     delete sv;
     return CB_OK;
 }

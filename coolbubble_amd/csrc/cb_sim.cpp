@@ -124,13 +124,13 @@ void release(cb_sim *s)
     if (s->st) hipStreamSynchronize(s->st);
     if (s->fft) cb::fft_destroy(s->fft);
     for (int i = 0; i < 2; ++i) {
-        if (s->phi[i]) hipFree(s->phi[i]);
-        if (s->pi[i]) hipFree(s->pi[i]);
+        cb::sim_free(s, s->phi[i]);
+        cb::sim_free(s, s->pi[i]);
     }
-    if (s->pi0) hipFree(s->pi0);
-    if (s->partials) hipFree(s->partials);
-    if (s->dtot) hipFree(s->dtot);
-    if (s->dsc) hipFree(s->dsc);
+    cb::sim_free(s, s->pi0);
+    cb::sim_free(s, s->partials);
+    cb::sim_free(s, s->dtot);
+    cb::sim_free(s, s->dsc);
     if (s->st && s->own_stream) hipStreamDestroy(s->st);
     delete s;
 }
@@ -223,12 +223,12 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
     }
     const size_t bytes = (size_t)s->nsites * s->esz;
     for (int i = 0; i < 2; ++i) {
-        HIP_TRY(hipMalloc(&s->phi[i], bytes));
-        if (i == 0 || s->lagged || s->p.D == 1) HIP_TRY(hipMalloc(&s->pi[i], bytes));
+        HIP_TRY(cb::sim_alloc(s, &s->phi[i], bytes));
+        if (i == 0 || s->lagged || s->p.D == 1) HIP_TRY(cb::sim_alloc(s, &s->pi[i], bytes));
     }
-    HIP_TRY(hipMalloc((void **)&s->partials, sizeof(double) * cb::NQ * (size_t)s->max_blocks));
-    HIP_TRY(hipMalloc((void **)&s->dtot, sizeof(double) * cb::NQ));
-    HIP_TRY(hipMalloc((void **)&s->dsc, sizeof(cb::DevScalars)));
+    HIP_TRY(cb::sim_alloc(s, (void **)&s->partials, sizeof(double) * cb::NQ * (size_t)s->max_blocks));
+    HIP_TRY(cb::sim_alloc(s, (void **)&s->dtot, sizeof(double) * cb::NQ));
+    HIP_TRY(cb::sim_alloc(s, (void **)&s->dsc, sizeof(cb::DevScalars)));
     HIP_TRY(hipMemsetAsync(s->phi[0], 0, bytes, s->st));
     HIP_TRY(hipMemsetAsync(s->pi[0], 0, bytes, s->st));
 
@@ -273,7 +273,7 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
     if (bubbles && !cen.empty()) {
         const int ncen = (int)(cen.size() / 3);
         double *dcen = nullptr;
-        HIP_TRY(hipMalloc((void **)&dcen, sizeof(double) * 6 * (size_t)ncen));
+        HIP_TRY(cb::sim_alloc(s, (void **)&dcen, sizeof(double) * 6 * (size_t)ncen));
         hipError_t e = hipMemcpyAsync(dcen, cen.data(), sizeof(double) * 3 * ncen, hipMemcpyHostToDevice, s->st);
         if (e == hipSuccess)
             e = hipMemcpyAsync(dcen + 3 * ncen, sub.data(), sizeof(double) * 3 * ncen, hipMemcpyHostToDevice, s->st);
@@ -281,14 +281,14 @@ cb_status construct(cb_sim *s, const double *xns, int n_xns)
             e = cb::launch_seed_bubbles(k, s->fp32, s->phi[0], dcen, dcen + 3 * ncen, ncen, L, s->phiC, s->Rn, s->lw,
                                         s->cfg, s->st);
         if (e == hipSuccess) e = hipStreamSynchronize(s->st);
-        hipFree(dcen);
+        cb::sim_free(s, dcen);
         if (e != hipSuccess) { set_error(std::string("seeding: ") + hipGetErrorString(e)); return CB_HIP_ERROR; }
     }
 
     // pi at t = 0, before the back-step below: the kinetic-energy map of the
     // initial snapshot reads it (src/bubble.cpp:253-261)
     if (s->keep_pi0) {
-        HIP_TRY(hipMalloc(&s->pi0, s->esz * (size_t)s->nsites));
+        HIP_TRY(cb::sim_alloc(s, &s->pi0, s->esz * (size_t)s->nsites));
         HIP_TRY(hipMemcpyAsync(s->pi0, s->pi[0], s->esz * (size_t)s->nsites, hipMemcpyDeviceToDevice, s->st));
     }
 
@@ -374,6 +374,25 @@ cb_status create_sim(const cb_params *p, const double *xns, int n_xns, bool keep
 }
 cb_status materialise_drift(cb_sim *s) { return materialise(s); }
 cb_status sim_scalars(cb_sim *s, DevScalars &h) { return read_scalars(s, h); }
+hipError_t sim_alloc(const cb_sim *s, void **p, size_t bytes)
+{
+    if (!s->pool_st) return hipMalloc(p, bytes);
+    hipError_t e = hipMallocAsync(p, bytes, s->pool_st);
+    if (e != hipErrorOutOfMemory) return e;
+    (void)hipGetLastError();
+    // free memory held by the pool back to the device, then once more
+    hipMemPool_t pool = nullptr;
+    if (hipStreamSynchronize(s->pool_st) != hipSuccess ||
+        hipDeviceGetDefaultMemPool(&pool, s->p.device) != hipSuccess || hipMemPoolTrimTo(pool, 0) != hipSuccess)
+        return hipErrorOutOfMemory;
+    return hipMallocAsync(p, bytes, s->pool_st);
+}
+void sim_free(const cb_sim *s, void *p)
+{
+    if (!p) return;
+    if (s->pool_st) hipFreeAsync(p, s->pool_st);
+    else hipFree(p);
+}
 bool row_from_scalars(const cb_sim *s, const DevScalars &h, double c[16])
 {
     c[0] = s->t;

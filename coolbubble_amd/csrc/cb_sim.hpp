@@ -27,6 +27,9 @@ struct cb_sim {
     double dxD = 0, phiC = 0, Rn = 0, lw = 0, mT = 0, phi0 = 1.0;
     hipStream_t st = nullptr;
     bool own_stream = false;    // false when the stream belongs to a batch / server
+    // non-null: device memory comes from the device's default memory pool,
+    // allocated and freed in order on this stream (a server's requests)
+    hipStream_t pool_st = nullptr;
     bool batch_owned = false;   // member of a cb_batch (destroyed with it)
     void *phi[2] = {nullptr, nullptr};
     void *pi[2] = {nullptr, nullptr};
@@ -67,6 +70,11 @@ cb_status step_n(cb_sim *s, int nsteps, bool multi1d = true);
 cb_status create_sim(const cb_params *p, const double *xns, int n_xns, bool keep_pi0, cb_sim **out);
 // apply a lagged drift still pending on phi (expanding runs)
 cb_status materialise_drift(cb_sim *s);
+// device memory of s: from the memory pool in order on s->pool_st when that
+// is set (after one trim of the pool's free memory if the first try fails),
+// else hipMalloc / hipFree
+hipError_t sim_alloc(const cb_sim *s, void **p, size_t bytes);
+void sim_free(const cb_sim *s, void *p);
 // the device scalars of s (synchronises its stream)
 cb_status sim_scalars(cb_sim *s, DevScalars &h);
 // the 16 outEP columns of s at its current step from its device scalars h

@@ -265,7 +265,7 @@ cb_status select_variant(cb_sim *s, int variant)
     }
     if (variant >= 0 && s->variants[(size_t)variant].fuse >= 2) {
         if (!s->pi[1]) {   // the fused kernel needs pi double-buffered
-            if (hipMalloc(&s->pi[1], (size_t)s->nsites * s->esz) != hipSuccess) {
+            if (sim_alloc(s, &s->pi[1], (size_t)s->nsites * s->esz) != hipSuccess) {
                 set_error("out of device memory for the second pi buffer");
                 return CB_OUT_OF_MEMORY;
             }
