@@ -219,7 +219,10 @@ cb_status cb_batch_destroy(cb_batch *b);
  * request (an energy row every 40 steps, a snapshot, completion).  Requests
  * arriving while others run are admitted at the next tick.  Each request's
  * results are bit-identical to running it alone with the same kernel
- * variant.  Times are seconds on the server's steady clock. */
+ * variant.  Requests' device memory comes from the device's default memory
+ * pool, allocated and freed in order on the server's stream; the server
+ * sets that pool to keep what it frees (for the next requests) and trims it
+ * when it is destroyed.  Times are seconds on the server's steady clock. */
 typedef struct cb_server cb_server;
 typedef struct cb_server_opts {
     int device;
