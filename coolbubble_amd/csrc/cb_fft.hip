@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -722,21 +723,53 @@ static bool factorise(int n, Stages &st)
 }
 
 // Twiddles exp(-2 pi i e / n), e < n, computed in long double on the host
-// and copied to device memory from dev_get.  stage: a host buffer of at
-// least n entries for the table (null: a temporary vector).
+// (x87 cosl / sinl: a mix of served requests, 1D N = 65536 among them, spent
+// 5 ms per request's plan on them before the tables were kept).
 template <typename R>
-static hipError_t make_twiddles(int n, void **out, hipStream_t s, Arena *arena = nullptr, void *stage = nullptr)
+static void fill_twiddles(cpx<R> *h, int n)
 {
-    std::vector<cpx<R>> tmp;
-    cpx<R> *h = reinterpret_cast<cpx<R> *>(stage);
-    if (!h) {
-        tmp.resize((size_t)n);
-        h = tmp.data();
-    }
     const long double twopi = 6.283185307179586476925286766559005768L;
     for (int e = 0; e < n; ++e) {
         long double a = twopi * (long double)e / (long double)n;
         h[e] = {(R)cosl(a), (R)-sinl(a)};
+    }
+}
+
+// The table for n, computed once per process and kept (entries are never
+// removed, so the pointer stays valid); null beyond 1024 distinct lengths.
+template <typename R>
+static const cpx<R> *kept_twiddles(int n)
+{
+    static std::mutex mu;
+    static std::map<int, std::vector<cpx<R>>> kept;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = kept.find(n);
+    if (it != kept.end()) return it->second.data();
+    if (kept.size() >= 1024) return nullptr;
+    std::vector<cpx<R>> &v = kept[n];
+    v.resize((size_t)n);
+    fill_twiddles(v.data(), n);
+    return v.data();
+}
+
+// The table copied to device memory from dev_get.  stage: a host buffer of
+// at least n entries to compute the table in (the no-allocation path: no
+// host allocation either); null: the kept table.
+template <typename R>
+static hipError_t make_twiddles(int n, void **out, hipStream_t s, Arena *arena = nullptr, void *stage = nullptr)
+{
+    std::vector<cpx<R>> tmp;
+    const cpx<R> *h = nullptr;
+    if (stage) {
+        fill_twiddles(reinterpret_cast<cpx<R> *>(stage), n);
+        h = reinterpret_cast<const cpx<R> *>(stage);
+    } else {
+        h = kept_twiddles<R>(n);
+        if (!h) {
+            tmp.resize((size_t)n);
+            fill_twiddles(tmp.data(), n);
+            h = tmp.data();
+        }
     }
     hipError_t err = dev_get(arena, out, sizeof(cpx<R>) * (size_t)n);
     if (err != hipSuccess) return err;
