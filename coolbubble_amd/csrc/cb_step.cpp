@@ -90,7 +90,8 @@ cb_status step_n(cb_sim *s, int nsteps, bool multi1d)
     int it = 0;
     if (fuse == 3 && s->tiled_pair >= 0) {
         // A 1024-thread three-step tile measures its energies on a
-        // 512-thread sibling at 2.7x a plain launch's time, so a call ends
+        // 512-thread sibling at 2.7x a plain launch's time (the direct-load
+        // tile's own energy form spills), so a call ends
         // on lighter launches: three-step launches, then for 3q + 2 steps one
         // two-step launch that measures the energies (one vector per thread:
         // no spilling), for 3q + 1 one one-step launch that does, and for 3q
@@ -305,6 +306,23 @@ cb_status select_variant(cb_sim *s, int variant)
                 s->pair_energy = c.items == 1;
                 break;
             }
+        }
+    } else if (variant >= 0 && s->variants[(size_t)variant].fuse == 3 && s->variants[(size_t)variant].pf == 4) {
+        // the direct-load tile has no two-step sibling: its calls end on the
+        // 1024-thread two-step tile with one vector per thread and the plane
+        // prefetch (the same per-site arithmetic), the chunk nearest its own
+        const TiledCfg &f = s->variants[(size_t)variant];
+        int best = -1;
+        double best_d = 1e300;
+        for (size_t v = 0; v < s->variants.size(); ++v) {
+            const TiledCfg &c = s->variants[v];
+            if (c.fuse != 2 || c.nthr != 1024 || c.items != 1 || c.pf != 2 || c.nt != f.nt) continue;
+            const double d = std::fabs(std::log2((double)c.NI / f.NI)) - 0.01 * c.TR;
+            if (d < best_d) { best_d = d; best = (int)v; }
+        }
+        if (best >= 0) {
+            s->tiled_pair = tiled_register(s->variants[(size_t)best]);
+            s->pair_energy = true;
         }
     }
     s->variant = variant;

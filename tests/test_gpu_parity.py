@@ -380,14 +380,17 @@ def test_wide_row_variants_bit_identical(D, N, precision):
     the measured workloads use, and the wide three-step tile (eight sites
     per thread, 768 threads, phi_n loaded straight into LDS: prefetch kind
     4; 3D FP32 at N = 512): every such variant gives the general kernel's
-    fields bit for bit over a call of 7 steps (multi-step launches plus the
-    leftover steps on the one-step companion).  At N = 512 only the wide
-    tiles are run (the others are the C2 tests')."""
+    fields bit for bit over calls of 7 and then 8 steps (multi-step
+    launches, then the leftover step on the one-step companion, or the last
+    two on the two-step tile that measures the energies).  At N = 512 only
+    the wide tiles are run (the others are the C2 tests')."""
     kw = params(D=D, readxns=-2, nGrids=N, L=N / 8.0)
     ref = cb.Bubble(precision=precision, **kw)
     ref.select_kernel(-1)
     ref.step(7)
     phi0, pi0 = ref.fields()
+    ref.step(8)
+    phi1, pi1 = ref.fields()
     n, _ = ref.kernel_variants()
     ref.close()
     g = cb.Bubble(precision=precision, **kw)
@@ -406,6 +409,10 @@ def test_wide_row_variants_bit_identical(D, N, precision):
         phi, pi = g.fields()
         np.testing.assert_array_equal(phi, phi0, err_msg=str(infos[v]))
         np.testing.assert_array_equal(pi, pi0, err_msg=str(infos[v]))
+        g.step(8)
+        phi, pi = g.fields()
+        np.testing.assert_array_equal(phi, phi1, err_msg=str(infos[v]))
+        np.testing.assert_array_equal(pi, pi1, err_msg=str(infos[v]))
         g.close()
 
 
