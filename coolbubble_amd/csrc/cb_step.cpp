@@ -316,8 +316,11 @@ cb_status select_variant(cb_sim *s, int variant)
         double best_d = 1e300;
         for (size_t v = 0; v < s->variants.size(); ++v) {
             const TiledCfg &c = s->variants[v];
-            if (c.fuse != 2 || c.nthr != 1024 || c.items != 1 || c.pf != 2 || c.nt != f.nt) continue;
-            const double d = std::fabs(std::log2((double)c.NI / f.NI)) - 0.01 * c.TR;
+            if (c.fuse != 2 || c.items != 1 || c.nt != f.nt) continue;
+            // its own two-step form first, else the 1024-thread prefetching tile
+            const bool dma = c.pf == 4, tile = c.nthr == 1024 && c.pf == 2;
+            if (!dma && !tile) continue;
+            const double d = std::fabs(std::log2((double)c.NI / f.NI)) - 0.01 * c.TR + (dma ? 0.0 : 10.0);
             if (d < best_d) { best_d = d; best = (int)v; }
         }
         if (best >= 0) {

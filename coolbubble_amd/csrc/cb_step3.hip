@@ -76,7 +76,8 @@ size_t step3_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c)
 // 16-byte pieces).
 static bool step3_wide(const SimConst &k, int fp32, const TiledCfg &c)
 {
-    return fp32 && k.D == 3 && c.vw == 8 && c.items == 1 && c.fuse == 3 && c.nthr == kDmaThreads<3>;
+    return fp32 && k.D == 3 && c.vw == 8 && c.items == 1 &&
+           ((c.fuse == 3 && c.nthr == kDmaThreads<3>) || (c.fuse == 2 && c.nthr == kDmaThreads<2>));
 }
 
 bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c)
@@ -88,6 +89,7 @@ bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c)
     if (c.items != 1 && c.items != 2 && c.items != 4) return false;
     if (c.nthr != 256 && c.nthr != 512 && c.nthr != 1024 && !wide) return false;
     if (wide && !(c.pf == 4 && k.N % 256 == 0)) return false;
+    if (wide && c.fuse == 2 && (c.TR + 2) * (k.N / 8) != c.nthr) return false;   // every thread a vector
     if (c.NI < 1 || (c.nt != 0 && c.nt != 1)) return false;
     if (!wide && c.pf != 1 && !((c.pf == 2 || c.pf == 3) && c.items == 1)) return false;   // prefetch: one vector per thread
     if (k.D == 3 && (c.TR < 1 || k.N % c.TR != 0)) return false;
@@ -160,11 +162,17 @@ hipError_t launch_step3(const SimConst &k, int fp32, int energy, const TiledCfg 
         dim3 grid(g.ntr * g.nic), block(c.nthr);
         const float *pin = (const float *)p.phi_in, *qin = (const float *)p.pi_in;
         float *pout = (float *)p.phi_out, *qout = (float *)p.pi_out;
-        const void *kern = energy ? (const void *)k_stepS_dma<3, true> : (const void *)k_stepS_dma<3, false>;
+        const void *kern = c.fuse == 2 ? (energy ? (const void *)k_stepS_dma<2, true> : (const void *)k_stepS_dma<2, false>)
+                                       : (energy ? (const void *)k_stepS_dma<3, true> : (const void *)k_stepS_dma<3, false>);
         hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
-        if (energy) k_stepS_dma<3, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
-        else k_stepS_dma<3, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+        if (c.fuse == 2) {
+            if (energy) k_stepS_dma<2, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            else k_stepS_dma<2, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+        } else {
+            if (energy) k_stepS_dma<3, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            else k_stepS_dma<3, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+        }
         return hipGetLastError();
     }
     if (fp32) {
@@ -222,6 +230,14 @@ int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
                 for (int nt : {1, 0}) {
                     TiledCfg c{8, 1, kDmaThreads<3>, 8, NI, 4, nt, 3};
                     if (n < cap && step3_valid(k, fp32, c) && 12 * (k.N / 8) > 512) out[n++] = c;
+                }
+        // its two-step form (ten rows of 64 vectors, 640 threads), listed
+        // last so that the other variants keep their indices
+        if (fp32)
+            for (int NI : {128, 64})
+                for (int nt : {1, 0}) {
+                    TiledCfg c{8, 1, kDmaThreads<2>, 8, NI, 4, nt, 2};
+                    if (n < cap && step3_valid(k, fp32, c)) out[n++] = c;
                 }
 
     } else {

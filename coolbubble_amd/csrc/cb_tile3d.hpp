@@ -1,5 +1,5 @@
 // cb_tile3d.hpp -- the multi-step tile of cb_tile3.hpp with phi_n staged in
-// LDS by direct global-to-LDS loads, S = 3 or 4 steps per pass (internal;
+// LDS by direct global-to-LDS loads, S = 2, 3 or 4 steps per pass (internal;
 // included by cb_step3.hip).
 //
 // 3D FP32 only: eight sites per thread vector, one vector per thread, rows of
@@ -21,6 +21,9 @@
 // Without a register ring and prefetch for phi_n the tile fits eight sites
 // per thread at three waves per SIMD without spilling; pi_{n-1/2} of the
 // next plane is still prefetched into registers.
+//   S = 2: TR = 8 at N = 512 (10 rows of step 0 x 64 vectors, 640 threads),
+//          1.125x the owned rows recomputed, 116 KiB of LDS; the two-step
+//          tail of the three-step tile's calls;
 //   S = 3: TR = 8 at N = 512 (12 rows of step 0 x 64 vectors, 768 threads),
 //          1.25x the owned rows recomputed, 158 VGPRs;
 //   S = 4: TR = 4 at N = 512 (10 rows x 64 vectors, 640 threads) would move
@@ -46,7 +49,8 @@ __device__ __forceinline__ void lds_dma16(const float *src, float *dst)
     __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
 }
 
-// threads per block of the S-step direct-load tile
+// threads per block of the S-step direct-load tile: TR + 2S - 2 rows of 64
+// vectors at N = 512 (S = 2 and 3: TR = 8; S = 4: TR = 4)
 template <int S>
 constexpr int kDmaThreads = S == 3 ? 768 : 640;
 
@@ -57,7 +61,7 @@ __device__ __forceinline__ void tile_stepS_dma(const SimConst &k, const TiledGeo
                                                const DevScalars *__restrict__ sc, double *__restrict__ partials,
                                                float *lds)
 {
-    static_assert(S == 3 || S == 4, "three or four steps per pass");
+    static_assert(S >= 2 && S <= 4, "two to four steps per pass");
     constexpr int VW = 8;
     constexpr int D = 3;
     constexpr int W = kDmaThreads<S> / 64;   // waves per block
