@@ -58,14 +58,15 @@ int batch_variant(const cb_sim *s) { return pick_variant(s, kBatchItems, 1, 1, 3
 // 256 threads, at most kBatchItems vectors per thread, prefetch 1, its
 // one-step companion fitting the batched launches too.  Three steps per
 // launch where such a tile of 2 rows or more exists (at most max_fuse),
-// else two; the tallest tile of at most 4 rows, chunks nearest 16 planes
-// (8 in 2D), non-temporal stores preferred.  An ensemble of 64 x 3D
-// N = 128 and 16 x 2D N = 1024 members steps 1000 times in 249 ms with
-// three steps per launch, 335 ms with two.  -1 if there is none.
-int pick_variant_multi(const cb_sim *s, int max_fuse)
+// else two; the tallest tile of at most 4 rows, chunks nearest ni3 planes
+// (ni2 in 2D; the server's requests 16 and 8, an ensemble's its autotuned
+// length), non-temporal stores preferred.  An ensemble of 64 x 3D N = 128
+// and 16 x 2D N = 1024 members steps 1000 times in 249 ms with three steps
+// per launch, 335 ms with two.  -1 if there is none.
+int pick_variant_multi(const cb_sim *s, int max_fuse, int ni3, int ni2)
 {
     if (s->lagged || s->p.D < 2) return -1;
-    const double ni_pref = s->p.D == 3 ? 16.0 : 8.0;
+    const double ni_pref = s->p.D == 3 ? (double)ni3 : (double)ni2;
     for (int fuse : {3, 2}) {
         if (fuse > max_fuse) continue;
         int best = -1;
@@ -325,7 +326,7 @@ cb_status build_table(cb_batch *b)
     b->general.clear();
     for (size_t i = 0; i < b->mem.size(); ++i) {
         cb_sim *s = b->mem[i];
-        const int pick2 = b->fused ? cb::pick_variant_multi(s, 3) : -1;
+        const int pick2 = b->fused ? cb::pick_variant_multi(s, 3, b->ni, b->ni) : -1;
         if (pick2 >= 0 && cb::select_variant(s, pick2) == CB_OK) {
             b->paired.push_back((int)i);
             continue;
@@ -565,7 +566,8 @@ cb_status cb_batch_autotune(cb_batch *b, int steps, int shape[5], double *best_m
     // (vectors per thread, prefetch, non-temporal stores, chunk planes, two steps per launch)
     const int shapes[][5] = {{4, 1, 1, 32, 0}, {4, 2, 1, 32, 0}, {4, 1, 0, 32, 0}, {2, 1, 1, 32, 0},
                              {1, 1, 1, 32, 0}, {4, 1, 1, 16, 0}, {4, 1, 1, 64, 0}, {1, 2, 1, 32, 0},
-                             {2, 2, 1, 16, 0}, {4, 2, 0, 16, 0}, {1, 1, 1, 32, 1}, {2, 1, 1, 32, 1}};
+                             {2, 2, 1, 16, 0}, {4, 2, 0, 16, 0}, {1, 1, 1, 32, 1}, {2, 1, 1, 32, 1},
+                             {1, 1, 1, 16, 1}, {1, 1, 1, 64, 1}, {2, 1, 1, 64, 1}};
     double best = 1e300;
     int best_i = 0;
     for (int c = 0; c < (int)(sizeof shapes / sizeof shapes[0]) && r == CB_OK; ++c) {

@@ -90,8 +90,9 @@ constexpr int kBatchItems = 2;
 int batch_variant(const cb_sim *s);
 int pick_variant(const cb_sim *s, int items_max, int pf, int nt, int ni_pref);
 // a static member's two- or three-step tile (at most max_fuse steps) for
-// the batched multi-step launch, or -1
-int pick_variant_multi(const cb_sim *s, int max_fuse);
+// the batched multi-step launch, chunks nearest ni3 planes in 3D, ni2 in
+// 2D, or -1
+int pick_variant_multi(const cb_sim *s, int max_fuse, int ni3 = 16, int ni2 = 8);
 // the variant the server gives a request: a lone run's for the large
 // static lattices that step on their own (server_solo), else a two-step
 // tile from pick_variant_multi when there is one, else batch_variant

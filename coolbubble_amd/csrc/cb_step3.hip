@@ -239,6 +239,10 @@ int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
                     TiledCfg c{8, 1, kDmaThreads<2>, 8, NI, 4, nt, 2};
                     if (n < cap && step3_valid(k, fp32, c)) out[n++] = c;
                 }
+        // 256-thread tiles without prefetch in 64-plane chunks (an
+        // ensemble's members), also listed last
+        for (int TR : {4, 2})
+            for (int nt : {1, 0}) push(256, TR, 64, nt, 1);
 
     } else {
         for (int nthr : {256, 512})
