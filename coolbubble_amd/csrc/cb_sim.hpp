@@ -47,6 +47,7 @@ struct cb_sim {
     int tiled_pair = -1;        // registry index of the two-step tile a 1024-thread three-step variant
                                 // ends a call with (-1: none)
     bool pair_energy = false;   // that two-step tile also measures the energies (one vector per thread)
+    bool pair_dma = false;      // that two-step tile is the direct-load one (cheaper than a one-step launch)
     double t = 0.0;
     long long step = 0;
     std::vector<double> xns;    // centres (the __xns.dat rows)

@@ -94,11 +94,14 @@ cb_status step_n(cb_sim *s, int nsteps, bool multi1d)
         // tile's own energy form spills), so a call ends
         // on lighter launches: three-step launches, then for 3q + 2 steps one
         // two-step launch that measures the energies (one vector per thread:
-        // no spilling), for 3q + 1 one one-step launch that does, and for 3q
-        // the last three as a plain two-step launch and a measuring one-step.
+        // no spilling), for 3q + 1 one one-step launch that does (the
+        // direct-load tile: the last four as two two-step launches of its
+        // own, the second measuring), and for 3q the last three as a plain
+        // two-step launch and a measuring one-step.
         const int q = nsteps / 3, r = nsteps % 3;
-        const bool pair_last = r == 2 && s->pair_energy;
-        const int q3 = (r == 0 && q > 0) ? q - 1 : q;
+        const bool four = r == 1 && q > 0 && s->pair_dma;
+        const bool pair_last = (r == 2 && s->pair_energy) || four;
+        const int q3 = ((r == 0 || four) && q > 0) ? q - 1 : q;
         auto advance = [&](int n, bool flip_pi) {
             s->iphi ^= 1;
             if (flip_pi) s->ipi ^= 1;
@@ -122,6 +125,10 @@ cb_status step_n(cb_sim *s, int nsteps, bool multi1d)
             HIP_TRY(single(true));
             advance(1, false);
         } else if (pair_last) {
+            if (four) {
+                HIP_TRY(launch_pair(s, false));
+                advance(2, true);
+            }
             HIP_TRY(launch_pair(s, true));
             advance(2, true);
         } else {
@@ -294,6 +301,7 @@ cb_status select_variant(cb_sim *s, int variant)
     }
     s->tiled_pair = -1;
     s->pair_energy = false;
+    s->pair_dma = false;
     if (variant >= 0 && s->variants[(size_t)variant].fuse == 3 && s->variants[(size_t)variant].nthr == 1024) {
         // the two-step tile of the same shape (threads, rows, chunk, stores;
         // prefetch one plane ahead at most)
@@ -326,6 +334,7 @@ cb_status select_variant(cb_sim *s, int variant)
         if (best >= 0) {
             s->tiled_pair = tiled_register(s->variants[(size_t)best]);
             s->pair_energy = true;
+            s->pair_dma = s->variants[(size_t)best].pf == 4;
         }
     }
     s->variant = variant;
