@@ -94,14 +94,15 @@ cb_status step_n(cb_sim *s, int nsteps, bool multi1d)
         // tile's own energy form spills), so a call ends
         // on lighter launches: three-step launches, then for 3q + 2 steps one
         // two-step launch that measures the energies (one vector per thread:
-        // no spilling), for 3q + 1 one one-step launch that does (the
-        // direct-load tile: the last four as two two-step launches of its
-        // own, the second measuring), and for 3q the last three as a plain
-        // two-step launch and a measuring one-step.
+        // no spilling), for 3q + 1 one one-step launch that does, and for 3q
+        // the last three as a plain two-step launch and a measuring one-step
+        // (the direct-load tile, whose two-step form costs less than a
+        // one-step launch: the last four, or six, as two, or three, two-step
+        // launches of its own, the last one measuring).
         const int q = nsteps / 3, r = nsteps % 3;
-        const bool four = r == 1 && q > 0 && s->pair_dma;
-        const bool pair_last = (r == 2 && s->pair_energy) || four;
-        const int q3 = ((r == 0 || four) && q > 0) ? q - 1 : q;
+        const int npairs = !s->pair_dma ? 0 : (r == 1 && q >= 1) ? 2 : (r == 0 && q >= 2) ? 3 : 0;
+        const bool pair_last = (r == 2 && s->pair_energy) || npairs > 0;
+        const int q3 = npairs ? q - (npairs == 2 ? 1 : 2) : ((r == 0 && q > 0) ? q - 1 : q);
         auto advance = [&](int n, bool flip_pi) {
             s->iphi ^= 1;
             if (flip_pi) s->ipi ^= 1;
@@ -119,13 +120,13 @@ cb_status step_n(cb_sim *s, int nsteps, bool multi1d)
             HIP_TRY(launch_multi(s, 3, false));
             advance(3, true);
         }
-        if (r == 0 && q > 0) {            // the last three steps: two, then one that measures
+        if (r == 0 && q > 0 && npairs == 0) {   // the last three steps: two, then one that measures
             HIP_TRY(launch_pair(s, false));
             advance(2, true);
             HIP_TRY(single(true));
             advance(1, false);
         } else if (pair_last) {
-            if (four) {
+            for (int i = 1; i < npairs; ++i) {
                 HIP_TRY(launch_pair(s, false));
                 advance(2, true);
             }
