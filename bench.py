@@ -144,10 +144,11 @@ def run_expanding(cb, hip, steps, log):
 
 def launch_time_live(sim, hip, fuse, m=13, reps=3):
     """Average duration of one plain launch of the selected variant (fuse
-    steps), live with HIP events on the simulation's stream: a call of
-    fuse * m steps is m - 1 plain launches plus one that measures the
-    energies, a call of fuse steps is that last launch alone, so
-    (T(fuse * m) - T(fuse)) / (m - 1) is one plain launch."""
+    steps), live with HIP events on the simulation's stream: calls of
+    fuse * m and fuse * 2 steps end on the same tail launches (the one that
+    measures the energies, and for a three-step tile with a two-step form
+    the launches of that form), so they differ by m - 2 plain launches:
+    (T(fuse * m) - T(fuse * 2)) / (m - 2) is one plain launch."""
     stream = sim.stream()
 
     def timed(n):
@@ -162,7 +163,7 @@ def launch_time_live(sim, hip, fuse, m=13, reps=3):
             best = ms if best is None else min(best, ms)
         return best
 
-    return (timed(fuse * m) - timed(fuse)) / (m - 1) * 1e3
+    return (timed(fuse * m) - timed(fuse * 2)) / (m - 2) * 1e3
 
 
 def slab_share(sim, info, log, planes=64):
