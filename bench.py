@@ -168,25 +168,36 @@ def launch_time_live(sim, hip, fuse, m=13, reps=3):
 
 def slab_share(sim, info, log, planes=64):
     """One GPU's slab of the same lattice sharded 8 ways along the slowest
-    axis (512 x 512 x 64 planes): the selected three-step tile and its
-    variants with other chunk lengths, restricted to the slab's planes, and
-    the device copies standing in for the halo exchange."""
+    axis (512 x 512 x 64 planes): the selected multi-step tile and the
+    1024-thread three-step tile, each with the chunk lengths that divide the
+    slab, restricted to the slab's planes (a rank re-tunes for its slab: the
+    direct-load tile's 256 row tiles of a whole box are only 64 blocks of a
+    64-plane slab), and the device copies standing in for the halo
+    exchange."""
     if not info or info.get("fuse", 1) < 2:
         return None
     n, _ = sim.kernel_variants()
+    infos = [sim.variant_info(v) for v in range(n)]
     out = {"planes": planes, "shape": "%d x %d x %d" % (CONFIG["nGrids"], CONFIG["nGrids"], planes), "tiles": []}
-    for ni in (64, 32, 16):
-        want = dict(info, NI=ni)
-        v = next((v for v in range(n) if sim.variant_info(v) == want), None)
-        if v is None or planes % ni:
-            continue
-        sim.select_kernel(v)
-        ml, mx = sim.slab_probe(planes, 20)
-        su = CONFIG["nGrids"] ** 2 * planes * info["fuse"] / (ml * 1e-3)
-        log("slab %s, NI %d: %.1f us per launch (%d steps), %.3e site-updates/s; exchange copies %.1f us"
-            % (out["shape"], ni, ml * 1e3, info["fuse"], su, mx * 1e3))
-        out["tiles"].append({"NI": ni, "us_per_launch": ml * 1e3, "site_updates_per_s": su,
-                             "exchange_copies_us": mx * 1e3})
+    bases = [info]
+    tile1024 = {"vw": 4, "items": 1, "nthr": 1024, "TR": 4, "NI": 64, "pf": 2, "nt": 1, "fuse": 3}
+    if any(i == tile1024 for i in infos) and info != tile1024:
+        bases.append(tile1024)
+    for base in bases:
+        for ni in (64, 32, 16):
+            want = dict(base, NI=ni)
+            v = next((v for v in range(n) if infos[v] == want), None)
+            if v is None or planes % ni:
+                continue
+            sim.select_kernel(v)
+            ml, mx = sim.slab_probe(planes, 20)
+            su = CONFIG["nGrids"] ** 2 * planes * base["fuse"] / (ml * 1e-3)
+            log("slab %s, %d threads, NI %d: %.1f us per launch (%d steps), %.3e site-updates/s; exchange copies %.1f us"
+                % (out["shape"], base["nthr"], ni, ml * 1e3, base["fuse"], su, mx * 1e3))
+            out["tiles"].append({"variant": want, "NI": ni, "us_per_launch": ml * 1e3, "site_updates_per_s": su,
+                                 "exchange_copies_us": mx * 1e3})
+    if out["tiles"]:
+        out["best_site_updates_per_s"] = max(t["site_updates_per_s"] for t in out["tiles"])
     out["exchange_bytes_per_launch"] = 2 * 2 * info["fuse"] * CONFIG["nGrids"] ** 2 * 4
     return out
 
