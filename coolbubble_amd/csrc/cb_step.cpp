@@ -42,8 +42,10 @@ static hipError_t launch_one(cb_sim *s, bool energy, bool lite = false)
     p.pi_out = s->lagged ? s->pi[s->ipi ^ 1] : s->pi[s->ipi];
     int nblocks = s->cfg.grid;
     hipError_t e;
-    if (s->tiled >= 0) {
-        e = launch_step_tiled(k, s->fp32, s->lagged, energy, s->tiled, p, s->dsc, s->partials, s->st, &nblocks);
+    // an expanding run's all-sums step may take another tile (select_variant)
+    const int which = (energy && !lite && s->lagged && s->tiled_energy >= 0) ? s->tiled_energy : s->tiled;
+    if (which >= 0) {
+        e = launch_step_tiled(k, s->fp32, s->lagged, energy, which, p, s->dsc, s->partials, s->st, &nblocks);
     } else {
         e = launch_step_simple(k, s->fp32, s->lagged, energy, p, s->dsc, s->partials, s->cfg, s->st);
     }
@@ -299,6 +301,23 @@ cb_status select_variant(cb_sim *s, int variant)
                 }
             }
         s->tiled_energy = tiled_register(s->variants[(size_t)e]);
+    }
+    if (variant >= 0 && s->lagged && s->variants[(size_t)variant].fuse == 1 &&
+        s->variants[(size_t)variant].nthr == 384) {
+        // an expanding run's all-sums step (the last of a call) needs 98
+        // VGPRs: four waves per SIMD, two 384-thread blocks per CU, so the
+        // grid of a 384-thread tile runs in rounds its Friedmann-sums steps
+        // (six waves per SIMD) do not have; it takes the 192-thread tile of
+        // half the rows in 64-plane chunks instead (3D N = 384: 211 against
+        // 245 us), the same fields
+        const TiledCfg &f = s->variants[(size_t)variant];
+        for (size_t v = 0; v < s->variants.size(); ++v) {
+            const TiledCfg &c = s->variants[v];
+            if (c.fuse == 1 && c.nthr == 192 && 2 * c.TR == f.TR && c.NI == 64 && c.nt == f.nt && c.pf == f.pf) {
+                s->tiled_energy = tiled_register(c);
+                break;
+            }
+        }
     }
     s->tiled_pair = -1;
     s->pair_energy = false;
