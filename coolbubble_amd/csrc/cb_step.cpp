@@ -232,8 +232,8 @@ int companion_variant(const cb_sim *s, const TiledCfg &f)
 // N = 64-512, static and expanding, FP32 and FP64.  Small lattices (3D below
 // 2^19 sites, 2D below 512^2) fill the GPU only with the one-site-per-thread
 // kernel; static ones take three steps per pass (FP64 at 3D N >= 512 two:
-// no three-step tile of its 1024-wide rows fits a block), with the plane
-// prefetch; expanding ones one step.
+// no three-step tile of its 1024-wide rows fits a block; at N = 512 on the
+// direct-load tile), with the plane prefetch; expanding ones one step.
 int default_variant(const cb_sim *s)
 {
     const int D = s->p.D, N = s->p.nGrids;
@@ -248,7 +248,8 @@ int default_variant(const cb_sim *s)
     if (s->lagged && N <= 160) w = f32 ? Want{1, 256, 8, 8, 1, nt_big} : Want{1, 256, 4, 8, 1, nt_big};
     else if (s->lagged && N <= 288) w = f32 ? Want{1, 256, 4, 16, 1, nt_big} : Want{1, 256, 2, 32, 2, nt_big};
     else if (s->lagged) w = N % 48 == 0 ? Want{1, 384, 4, 48, 1, nt_big} : Want{1, 256, 2, 32, 1, nt_big};
-    else if (D == 3 && N >= 512) w = f32 ? Want{3, 1024, 4, 64, 2, 1} : Want{2, 1024, 4, 64, 2, 1};
+    else if (D == 3 && N >= 512)
+        w = f32 ? Want{3, 1024, 4, 64, 2, 1} : (N == 512 ? Want{2, 768, 4, 256, 4, 1} : Want{2, 1024, 4, 64, 2, 1});
     else if (D == 3 && N >= 320) w = f32 ? Want{3, 1024, 6, 32, 2, 1} : Want{3, 1024, 4, 32, 2, 1};
     else if (D == 3 && N >= 224) w = f32 ? Want{3, 1024, 8, 32, 2, 0} : Want{3, 1024, 4, 64, 2, 1};
     else if (D == 3 && N >= 160) w = f32 ? Want{3, 512, 6, 16, 2, 0} : Want{3, 512, 4, 16, 2, 1};

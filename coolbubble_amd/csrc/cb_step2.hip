@@ -38,7 +38,7 @@ static TiledGeom geom2(const SimConst &k, const TiledCfg &c)
 
 size_t step2_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c)
 {
-    if (c.vw == 8 && c.pf == 4) return step3_lds_bytes(k, fp32, c);   // the direct-load tile
+    if (c.pf == 4) return step3_lds_bytes(k, fp32, c);   // the direct-load tile
     const size_t esz = fp32 ? 4 : 8;
     const int rows = (k.D == 3) ? (c.TR + 4) + (c.TR + 2) : 2;
     return (size_t)rows * k.N * esz;
@@ -46,7 +46,7 @@ size_t step2_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c)
 
 bool step2_valid(const SimConst &k, int fp32, const TiledCfg &c)
 {
-    if (c.vw == 8 && c.pf == 4) return c.fuse == 2 && step3_valid(k, fp32, c);   // the direct-load tile, S = 2
+    if (c.pf == 4) return c.fuse == 2 && step3_valid(k, fp32, c);   // the direct-load tile, S = 2
     if (k.D < 2 || k.expanding) return false;
     const int want_vw = fp32 ? 4 : 2;
     if (c.vw != want_vw || k.N % c.vw != 0 || k.N < 4) return false;
@@ -116,7 +116,7 @@ hipError_t launch_step2(const SimConst &k, int fp32, int energy, const TiledCfg 
                         const DevScalars *sc, double *partials, hipStream_t st, int *nblocks, int nchunks)
 {
     if (!step2_valid(k, fp32, c)) return hipErrorInvalidValue;
-    if (c.vw == 8 && c.pf == 4) return launch_step3(k, fp32, energy, c, p, sc, partials, st, nblocks, nchunks);
+    if (c.pf == 4) return launch_step3(k, fp32, energy, c, p, sc, partials, st, nblocks, nchunks);
     const size_t shm = step2_lds_bytes(k, fp32, c);
     TiledGeom g = geom2(k, c);
     if (nchunks > 0 && nchunks < g.nic) g.nic = nchunks;   // the first nchunks chunks of planes only

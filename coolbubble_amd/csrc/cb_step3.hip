@@ -22,16 +22,16 @@ k_step3_tiled(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restri
                                           reinterpret_cast<T *>(smem_raw));
 }
 
-template <int S, bool ENERGY>
-__global__ void __launch_bounds__(kDmaThreads<S>)
-k_stepS_dma(SimConst k, TiledGeom g, const float *__restrict__ phi_in, float *__restrict__ phi_out,
-            const float *__restrict__ pi_in, float *__restrict__ pi_out, const DevScalars *__restrict__ sc,
+template <typename T, int S, bool ENERGY>
+__global__ void __launch_bounds__((kDmaThreads<T, S>))
+k_stepS_dma(SimConst k, TiledGeom g, const T *__restrict__ phi_in, T *__restrict__ phi_out,
+            const T *__restrict__ pi_in, T *__restrict__ pi_out, const DevScalars *__restrict__ sc,
             double *__restrict__ partials)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int G = gridDim.x, b = blockIdx.x;
     const int lt = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;   // XCD-contiguous tiles
-    tile_stepS_dma<S, ENERGY>(k, g, lt, phi_in, phi_out, pi_in, pi_out, sc, partials,
+    tile_stepS_dma<T, S, ENERGY>(k, g, lt, phi_in, phi_out, pi_in, pi_out, sc, partials,
                               reinterpret_cast<float *>(smem_raw));
 }
 
@@ -55,7 +55,7 @@ static TiledGeom geom3(const SimConst &k, const TiledCfg &c)
 size_t step3_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c)
 {
     const size_t esz = fp32 ? 4 : 8;
-    if (c.vw == 8 && c.pf == 4) {
+    if (c.pf == 4) {
         // the direct-load form: four planes of phi_n (TR + 2S rows), then
         // level l's plane (TR + 2S - 2l rows), l = 1 .. S-1
         const int S = c.fuse;
@@ -67,17 +67,20 @@ size_t step3_lds_bytes(const SimConst &k, int fp32, const TiledCfg &c)
     return (size_t)rows * k.N * esz;
 }
 
-// The wide tile (cb_tile3d.hpp): FP32 3D, eight sites per thread vector,
-// 768 threads, one vector per thread, phi_n loaded straight into four LDS
-// plane buffers (prefetch kind 4) -- a tile of TR = 8 rows fits one block
-// (12 rows of step A x 64 vectors at N = 512), so the rows recomputed for
-// steps A and B are 1.25x the owned ones instead of 1.5x at TR = 4, at up
-// to 160 KiB of LDS.  Rows of a multiple of 256 sites (whole waves of
+// The wide tile (cb_tile3d.hpp): 3D, 32-byte thread vectors (eight FP32
+// sites, four FP64), one vector per thread, phi_n loaded straight into four
+// LDS plane buffers (prefetch kind 4).  FP32, three steps: a tile of TR = 8
+// rows fits one block (12 rows of step A x 64 vectors at N = 512, 768
+// threads), so the rows recomputed for steps A and B are 1.25x the owned
+// ones instead of 1.5x at TR = 4; two steps: 10 rows, 640 threads.  FP64,
+// two steps: TR = 4 (6 rows x 128 vectors, 768 threads, 152 KiB of LDS).
+// At most 160 KiB of LDS; rows of a multiple of 256 sites (whole waves of
 // 16-byte pieces).
 static bool step3_wide(const SimConst &k, int fp32, const TiledCfg &c)
 {
-    return fp32 && k.D == 3 && c.vw == 8 && c.items == 1 &&
-           ((c.fuse == 3 && c.nthr == kDmaThreads<3>) || (c.fuse == 2 && c.nthr == kDmaThreads<2>));
+    if (k.D != 3 || c.items != 1 || c.vw != (fp32 ? 8 : 4)) return false;
+    if (fp32) return (c.fuse == 3 && c.nthr == kDmaThreads<float, 3>) || (c.fuse == 2 && c.nthr == kDmaThreads<float, 2>);
+    return c.fuse == 2 && c.nthr == kDmaThreads<double, 2>;
 }
 
 bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c)
@@ -89,7 +92,7 @@ bool step3_valid(const SimConst &k, int fp32, const TiledCfg &c)
     if (c.items != 1 && c.items != 2 && c.items != 4) return false;
     if (c.nthr != 256 && c.nthr != 512 && c.nthr != 1024 && !wide) return false;
     if (wide && !(c.pf == 4 && k.N % 256 == 0)) return false;
-    if (wide && c.fuse == 2 && (c.TR + 2) * (k.N / 8) != c.nthr) return false;   // every thread a vector
+    if (wide && c.fuse == 2 && (c.TR + 2) * (k.N / c.vw) != c.nthr) return false;   // every thread a vector
     if (c.NI < 1 || (c.nt != 0 && c.nt != 1)) return false;
     if (!wide && c.pf != 1 && !((c.pf == 2 || c.pf == 3) && c.items == 1)) return false;   // prefetch: one vector per thread
     if (k.D == 3 && (c.TR < 1 || k.N % c.TR != 0)) return false;
@@ -160,18 +163,28 @@ hipError_t launch_step3(const SimConst &k, int fp32, int energy, const TiledCfg 
         // phi_n staged by direct loads into LDS (cb_tile3d.hpp); the
         // energies on the same tile
         dim3 grid(g.ntr * g.nic), block(c.nthr);
+        if (!fp32) {
+            const double *pin = (const double *)p.phi_in, *qin = (const double *)p.pi_in;
+            double *pout = (double *)p.phi_out, *qout = (double *)p.pi_out;
+            const void *kern = energy ? (const void *)k_stepS_dma<double, 2, true> : (const void *)k_stepS_dma<double, 2, false>;
+            hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            if (e != hipSuccess) return e;
+            if (energy) k_stepS_dma<double, 2, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            else k_stepS_dma<double, 2, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            return hipGetLastError();
+        }
         const float *pin = (const float *)p.phi_in, *qin = (const float *)p.pi_in;
         float *pout = (float *)p.phi_out, *qout = (float *)p.pi_out;
-        const void *kern = c.fuse == 2 ? (energy ? (const void *)k_stepS_dma<2, true> : (const void *)k_stepS_dma<2, false>)
-                                       : (energy ? (const void *)k_stepS_dma<3, true> : (const void *)k_stepS_dma<3, false>);
+        const void *kern = c.fuse == 2 ? (energy ? (const void *)k_stepS_dma<float, 2, true> : (const void *)k_stepS_dma<float, 2, false>)
+                                       : (energy ? (const void *)k_stepS_dma<float, 3, true> : (const void *)k_stepS_dma<float, 3, false>);
         hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
         if (c.fuse == 2) {
-            if (energy) k_stepS_dma<2, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
-            else k_stepS_dma<2, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            if (energy) k_stepS_dma<float, 2, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            else k_stepS_dma<float, 2, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
         } else {
-            if (energy) k_stepS_dma<3, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
-            else k_stepS_dma<3, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            if (energy) k_stepS_dma<float, 3, true><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
+            else k_stepS_dma<float, 3, false><<<grid, block, shm, st>>>(k, g, pin, pout, qin, qout, sc, partials);
         }
         return hipGetLastError();
     }
@@ -228,7 +241,7 @@ int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
         if (fp32)
             for (int NI : {128, 64})
                 for (int nt : {1, 0}) {
-                    TiledCfg c{8, 1, kDmaThreads<3>, 8, NI, 4, nt, 3};
+                    TiledCfg c{8, 1, kDmaThreads<float, 3>, 8, NI, 4, nt, 3};
                     if (n < cap && step3_valid(k, fp32, c) && 12 * (k.N / 8) > 512) out[n++] = c;
                 }
         // its two-step form (ten rows of 64 vectors, 640 threads), listed
@@ -236,13 +249,21 @@ int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
         if (fp32)
             for (int NI : {128, 64})
                 for (int nt : {1, 0}) {
-                    TiledCfg c{8, 1, kDmaThreads<2>, 8, NI, 4, nt, 2};
+                    TiledCfg c{8, 1, kDmaThreads<float, 2>, 8, NI, 4, nt, 2};
                     if (n < cap && step3_valid(k, fp32, c)) out[n++] = c;
                 }
         // 256-thread tiles without prefetch in 64-plane chunks (an
         // ensemble's members), also listed last
         for (int TR : {4, 2})
             for (int nt : {1, 0}) push(256, TR, 64, nt, 1);
+        // the FP64 two-step direct-load tile (six rows of 128 vectors of
+        // four sites, 768 threads; at NI = 256 one block per CU), last
+        if (!fp32)
+            for (int NI : {64, 128, 256})
+                for (int nt : {1, 0}) {
+                    TiledCfg c{4, 1, kDmaThreads<double, 2>, 4, NI, 4, nt, 2};
+                    if (n < cap && step3_valid(k, fp32, c)) out[n++] = c;
+                }
 
     } else {
         for (int nthr : {256, 512})

@@ -2,8 +2,8 @@
 // LDS by direct global-to-LDS loads, S = 2, 3 or 4 steps per pass (internal;
 // included by cb_step3.hip).
 //
-// 3D FP32 only: eight sites per thread vector, one vector per thread, rows of
-// a multiple of 256 sites.  A block marching the slow axis i chains S steps
+// 3D only: 32-byte thread vectors (eight FP32 sites, four FP64), one vector
+// per thread, rows of a multiple of 256 sites.  A block marching the slow axis i chains S steps
 // plane by plane, as tile_step3 does for three: step 0 (n -> n+1) on plane p
 // for TR + 2(S-1) rows, step l on plane p-l for TR + 2(S-1-l) rows, the
 // last step (S-1) on plane p-S+1 for the TR owned rows, which go to HBM.
@@ -26,6 +26,8 @@
 //          tail of the three-step tile's calls;
 //   S = 3: TR = 8 at N = 512 (12 rows of step 0 x 64 vectors, 768 threads),
 //          1.25x the owned rows recomputed, 158 VGPRs;
+//   S = 2, FP64: TR = 4 at N = 512 (6 rows of step 0 x 128 vectors, 768
+//          threads), 1.25x the owned rows recomputed, 152 KiB of LDS;
 //   S = 4: TR = 4 at N = 512 (10 rows x 64 vectors, 640 threads) would move
 //          three quarters of the HBM bytes per step, but its three rings and
 //          six pi vectors need more than the 168 VGPRs of three waves per
@@ -44,29 +46,29 @@ namespace cb {
 
 // One wave-wide direct load: lane l's 16 bytes at src (its own address)
 // land at the wave-uniform LDS address dst + 16 l bytes.
-__device__ __forceinline__ void lds_dma16(const float *src, float *dst)
+__device__ __forceinline__ void lds_dma16(const void *src, void *dst)
 {
     __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
 }
 
-// threads per block of the S-step direct-load tile: TR + 2S - 2 rows of 64
-// vectors at N = 512 (S = 2 and 3: TR = 8; S = 4: TR = 4)
-template <int S>
-constexpr int kDmaThreads = S == 3 ? 768 : 640;
+// threads per block of the S-step direct-load tile: TR + 2S - 2 rows of the
+// N / VW vectors of a row at N = 512 (FP32, 64 vectors a row: S = 2 and 3
+// TR = 8, S = 4 TR = 4; FP64, 128 vectors a row: S = 2, TR = 4)
+template <typename T, int S>
+constexpr int kDmaThreads = sizeof(T) == 8 ? 768 : (S == 3 ? 768 : 640);
 
-template <int S, bool ENERGY>
+template <typename T, int S, bool ENERGY>
 __device__ __forceinline__ void tile_stepS_dma(const SimConst &k, const TiledGeom &g, int lt,
-                                               const float *__restrict__ phi_in, float *__restrict__ phi_out,
-                                               const float *__restrict__ pi_in, float *__restrict__ pi_out,
+                                               const T *__restrict__ phi_in, T *__restrict__ phi_out,
+                                               const T *__restrict__ pi_in, T *__restrict__ pi_out,
                                                const DevScalars *__restrict__ sc, double *__restrict__ partials,
                                                float *lds)
 {
     static_assert(S >= 2 && S <= 4, "two to four steps per pass");
-    constexpr int VW = 8;
+    constexpr int VW = 32 / sizeof(T);      // 32-byte thread vectors
     constexpr int D = 3;
-    constexpr int W = kDmaThreads<S> / 64;   // waves per block
+    constexpr int W = kDmaThreads<T, S> / 64;   // waves per block
     constexpr int NL = S - 1;                // register levels 1 .. S-1
-    using T = float;
     using V = Vec<T, VW>;
     typedef T nvec __attribute__((ext_vector_type(VW)));
 
@@ -81,11 +83,11 @@ __device__ __forceinline__ void tile_stepS_dma(const SimConst &k, const TiledGeo
     const int i1 = min(i0 + g.NI, N);
     const int rows0 = TR + 2 * S;                     // phi_n rows r0-S .. r0+TR+S-1
     const size_t pl0 = (size_t)rows0 * N;
-    float *L0b = lds;                                 // four phi_n plane buffers
+    T *L0b = reinterpret_cast<T *>(lds);              // four phi_n plane buffers
     // Ll[l-1]: phi_n+l at plane p-l, rows r0-S+l .. r0+TR+S-1-l (TR + 2S - 2l rows)
-    float *Ll[NL];
+    T *Ll[NL];
     {
-        float *q = L0b + 4 * pl0;
+        T *q = L0b + 4 * pl0;
 #pragma unroll
         for (int l = 1; l <= NL; ++l) {
             Ll[l - 1] = q;
@@ -115,16 +117,18 @@ __device__ __forceinline__ void tile_stepS_dma(const SimConst &k, const TiledGeo
     const int l0row = (rowA + 1) * N + col;           // the vector's place in an L0 buffer
 
     // direct loads of one plane of phi_n into buffer b: the plane's rows0
-    // rows in pieces of 256 floats, piece d by wave d mod W
+    // rows in pieces of 1 KiB (PE elements, 16 bytes a lane), piece d by
+    // wave d mod W
+    constexpr int LE = 16 / sizeof(T), PE = 64 * LE;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int segs = N / 256;
+    const int segs = N / PE;
     const int npieces = rows0 * segs;
     auto dma_plane = [&](int p, int b) __attribute__((always_inline)) {
         const long long base = (long long)wrapN(p) * NN;
-        float *dst = L0b + (size_t)b * pl0;
+        T *dst = L0b + (size_t)b * pl0;
         for (int d = wv; d < npieces; d += W) {
             const int r = d / segs, s = d - r * segs;
-            lds_dma16(phi_in + base + (long long)wrapN(r0 - S + r) * N + s * 256 + lane * 4, dst + r * N + s * 256);
+            lds_dma16(phi_in + base + (long long)wrapN(r0 - S + r) * N + s * PE + lane * LE, dst + r * N + s * PE);
         }
     };
 
@@ -142,15 +146,15 @@ __device__ __forceinline__ void tile_stepS_dma(const SimConst &k, const TiledGeo
     dma_plane(p_first - 1, 0);
     dma_plane(p_first, 1);
     dma_plane(p_first + 1, 2);
-    pr = (nvec)0.0f;
+    pr = (nvec)(T)0;
     if (ok) pr = *reinterpret_cast<const nvec *>(pi_in + (long long)wrapN(p_first) * NN + off);
 
     auto plane = [&](auto s3, int p, int t) __attribute__((always_inline)) {
         constexpr int S3 = decltype(s3)::value;
         constexpr int PV = (S3 + 2) % 3, NX = (S3 + 1) % 3;
-        const float *Lm = L0b + (size_t)(t & 3) * pl0;          // plane p-1
-        const float *Lc = L0b + (size_t)((t + 1) & 3) * pl0;    // plane p
-        const float *Lp = L0b + (size_t)((t + 2) & 3) * pl0;    // plane p+1
+        const T *Lm = L0b + (size_t)(t & 3) * pl0;              // plane p-1
+        const T *Lc = L0b + (size_t)((t + 1) & 3) * pl0;        // plane p
+        const T *Lp = L0b + (size_t)((t + 2) & 3) * pl0;    // plane p+1
         // this wave's direct loads of plane p+1 and its pi prefetch have landed
         vm_wait_all();
         V pi0;
