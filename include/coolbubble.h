@@ -152,8 +152,8 @@ cb_status cb_get_xns(cb_sim *s, double *xyz, int cap, int *n);
  * configuration valid for this (D, nGrids, precision).  cb_kernel_variants
  * reports how many tiled configurations exist and which one is selected.
  * cb_create selects the configuration nearest to what cb_autotune picks on
- * MI355X for lattices of this kind (static boxes: three steps per launch;
- * expanding: one), or the general kernel for 3D lattices below 2^19 sites
+ * MI355X for lattices of this kind (static boxes: three steps per launch,
+ * FP64 at 3D nGrids >= 512 two; expanding: one), or the general kernel for 3D lattices below 2^19 sites
  * and 2D lattices below 512^2. */
 cb_status cb_kernel_variants(cb_sim *s, int *count, int *selected);
 cb_status cb_select_kernel(cb_sim *s, int variant);
