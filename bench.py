@@ -181,7 +181,7 @@ def slab_share(sim, info, log, planes=64):
     out = {"planes": planes, "shape": "%d x %d x %d" % (CONFIG["nGrids"], CONFIG["nGrids"], planes), "tiles": []}
     bases = [info]
     tile1024 = {"vw": 4, "items": 1, "nthr": 1024, "TR": 4, "NI": 64, "pf": 2, "nt": 1, "fuse": 3}
-    if any(i == tile1024 for i in infos) and info != tile1024:
+    if any(i == tile1024 for i in infos) and dict(info, NI=64) != tile1024:
         bases.append(tile1024)
     for base in bases:
         for ni in (64, 32, 16):

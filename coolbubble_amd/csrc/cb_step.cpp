@@ -233,7 +233,8 @@ int companion_variant(const cb_sim *s, const TiledCfg &f)
 // 2^19 sites, 2D below 512^2) fill the GPU only with the one-site-per-thread
 // kernel; static ones take three steps per pass (FP64 at 3D N >= 512 two:
 // no three-step tile of its 1024-wide rows fits a block; at N = 512 on the
-// direct-load tile), with the plane prefetch; expanding ones one step.
+// direct-load tile), with the plane prefetch (at 3D N = 512 in chunks of
+// 256 planes: one block per CU in one round); expanding ones one step.
 int default_variant(const cb_sim *s)
 {
     const int D = s->p.D, N = s->p.nGrids;
@@ -249,7 +250,8 @@ int default_variant(const cb_sim *s)
     else if (s->lagged && N <= 288) w = f32 ? Want{1, 256, 4, 16, 1, nt_big} : Want{1, 256, 2, 32, 2, nt_big};
     else if (s->lagged) w = N % 48 == 0 ? Want{1, 384, 4, 48, 1, nt_big} : Want{1, 256, 2, 32, 1, nt_big};
     else if (D == 3 && N >= 512)
-        w = f32 ? Want{3, 1024, 4, 64, 2, 1} : (N == 512 ? Want{2, 768, 4, 256, 4, 1} : Want{2, 1024, 4, 64, 2, 1});
+        w = N == 512 ? (f32 ? Want{3, 1024, 4, 256, 2, 1} : Want{2, 768, 4, 256, 4, 1})
+                     : (f32 ? Want{3, 1024, 4, 64, 2, 1} : Want{2, 1024, 4, 64, 2, 1});
     else if (D == 3 && N >= 320) w = f32 ? Want{3, 1024, 6, 32, 2, 1} : Want{3, 1024, 4, 32, 2, 1};
     else if (D == 3 && N >= 224) w = f32 ? Want{3, 1024, 8, 32, 2, 0} : Want{3, 1024, 4, 64, 2, 1};
     else if (D == 3 && N >= 160) w = f32 ? Want{3, 512, 6, 16, 2, 0} : Want{3, 512, 4, 16, 2, 1};
@@ -324,17 +326,26 @@ cb_status select_variant(cb_sim *s, int variant)
     s->pair_energy = false;
     s->pair_dma = false;
     if (variant >= 0 && s->variants[(size_t)variant].fuse == 3 && s->variants[(size_t)variant].nthr == 1024) {
-        // the two-step tile of the same shape (threads, rows, chunk, stores;
-        // prefetch one plane ahead at most)
+        // the direct-load tile's two-step form where it exists (its energy
+        // launch costs less than this tile's two-step sibling's), else the
+        // two-step tile of the same shape (threads, rows, stores; prefetch
+        // one plane ahead at most), the chunk nearest its own
         const TiledCfg &f = s->variants[(size_t)variant];
+        int best = -1;
+        double best_d = 1e300;
         for (size_t v = 0; v < s->variants.size(); ++v) {
             const TiledCfg &c = s->variants[v];
-            if (c.fuse == 2 && c.nthr == f.nthr && c.TR == f.TR && c.NI == f.NI && c.pf == std::min(f.pf, 2) &&
-                c.nt == f.nt) {
-                s->tiled_pair = tiled_register(c);
-                s->pair_energy = c.items == 1;
-                break;
-            }
+            if (c.fuse != 2 || c.nt != f.nt) continue;
+            const bool dma = c.pf == 4;
+            if (!dma && !(c.nthr == f.nthr && c.TR == f.TR && c.pf == std::min(f.pf, 2))) continue;
+            const double d = std::fabs(std::log2((double)c.NI / f.NI)) + (dma ? 0.0 : 10.0);
+            if (d < best_d) { best_d = d; best = (int)v; }
+        }
+        if (best >= 0) {
+            const TiledCfg &c = s->variants[(size_t)best];
+            s->tiled_pair = tiled_register(c);
+            s->pair_energy = c.items == 1;
+            s->pair_dma = c.pf == 4;
         }
     } else if (variant >= 0 && s->variants[(size_t)variant].fuse == 3 && s->variants[(size_t)variant].pf == 4) {
         // the direct-load tile has no two-step sibling: its calls end on the

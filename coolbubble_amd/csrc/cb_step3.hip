@@ -264,6 +264,10 @@ int step3_candidates(const SimConst &k, int fp32, TiledCfg *out, int cap)
                     TiledCfg c{4, 1, kDmaThreads<double, 2>, 4, NI, 4, nt, 2};
                     if (n < cap && step3_valid(k, fp32, c)) out[n++] = c;
                 }
+        // 1024-thread tiles in chunks of 256 planes (FP32 3D N = 512: 256
+        // blocks, one round of one block per CU), last
+        for (int pf : {2, 3})
+            for (int nt : {1, 0}) push(1024, 4, 256, nt, pf);
 
     } else {
         for (int nthr : {256, 512})

@@ -382,11 +382,12 @@ def test_wide_row_variants_bit_identical(D, N, precision):
     steps of eight sites per thread, FP64 two steps of four): every such
     variant gives the general kernel's fields bit for bit over a call of 7
     steps (multi-step launches, then the leftover step on the one-step
-    companion), the direct-load tiles also over calls of 8 and then 6 more
-    (the three-step tile ends 3q + 1, 3q + 2 and 3q steps on one to three
-    launches of its two-step form, the last one measuring the energies:
-    rows within 1e-13 of the general kernel's).  At N = 512 only the wide tiles are run (the
-    others are the C2 tests')."""
+    companion), and at N = 512 also over calls of 8 and then 6 more (the
+    three-step tiles end 3q + 1, 3q + 2 and 3q steps on one to three
+    launches of the direct-load two-step form, the last one measuring the
+    energies: rows within 1e-13 of the general kernel's).  At N = 512 only
+    the direct-load tiles and the 1024-thread four-row three-step tiles are
+    run (the others are the C2 tests')."""
     kw = params(D=D, readxns=-2, nGrids=N, L=N / 8.0)
     ref = cb.Bubble(precision=precision, **kw)
     ref.select_kernel(-1)
@@ -405,7 +406,10 @@ def test_wide_row_variants_bit_identical(D, N, precision):
     g.close()
     multi = [v for v in range(n) if infos[v]["fuse"] >= 2 and (infos[v]["pf"] >= 2 or infos[v]["vw"] == 8)]
     if N == 512 and D == 3:
-        multi = [v for v in multi if infos[v]["pf"] == 4]
+        # the direct-load tiles, and the 1024-thread four-row three-step
+        # tiles, whose calls end on the direct-load tile's two-step form
+        multi = [v for v in multi if infos[v]["pf"] == 4 or
+                 (infos[v]["fuse"] == 3 and infos[v]["nthr"] == 1024 and infos[v]["TR"] == 4)]
         assert any(infos[v]["pf"] == 4 for v in multi)
     assert multi
 
@@ -416,7 +420,7 @@ def test_wide_row_variants_bit_identical(D, N, precision):
         phi, pi = g.fields()
         np.testing.assert_array_equal(phi, phi0, err_msg=str(infos[v]))
         np.testing.assert_array_equal(pi, pi0, err_msg=str(infos[v]))
-        if infos[v]["pf"] == 4:
+        if N == 512 and D == 3:
             g.step(8)
             phi, pi = g.fields()
             np.testing.assert_array_equal(phi, phi1, err_msg=str(infos[v]))
