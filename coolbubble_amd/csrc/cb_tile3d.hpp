@@ -3,8 +3,8 @@
 // included by cb_step3.hip).
 //
 // 3D only: 32-byte thread vectors (eight FP32 sites, four FP64), one vector
-// per thread, rows of a multiple of 256 sites.  A block marching the slow axis i chains S steps
-// plane by plane, as tile_step3 does for three: step 0 (n -> n+1) on plane p
+// per thread, rows of a multiple of 256 sites.  A block marching the slow
+// axis i chains S steps plane by plane, as tile_step3 does for three: step 0 (n -> n+1) on plane p
 // for TR + 2(S-1) rows, step l on plane p-l for TR + 2(S-1-l) rows, the
 // last step (S-1) on plane p-S+1 for the TR owned rows, which go to HBM.
 // Each site's arithmetic is that of S single steps, so fields are
@@ -23,7 +23,7 @@
 // next plane is still prefetched into registers.
 //   S = 2: TR = 8 at N = 512 (10 rows of step 0 x 64 vectors, 640 threads),
 //          1.125x the owned rows recomputed, 116 KiB of LDS; the two-step
-//          tail of the three-step tile's calls;
+//          tail of the FP32 three-step tiles' calls;
 //   S = 3: TR = 8 at N = 512 (12 rows of step 0 x 64 vectors, 768 threads),
 //          1.25x the owned rows recomputed, 158 VGPRs;
 //   S = 2, FP64: TR = 4 at N = 512 (6 rows of step 0 x 128 vectors, 768
@@ -32,7 +32,7 @@
 //          three quarters of the HBM bytes per step, but its three rings and
 //          six pi vectors need more than the 168 VGPRs of three waves per
 //          SIMD (200 bytes of scratch per lane: 0.575 ms/step at 3D N = 512
-//          against 0.135), so only S = 3 is launched.
+//          against 0.135), so S = 4 is not launched.
 #pragma once
 
 #include <type_traits>
