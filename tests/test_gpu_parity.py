@@ -486,6 +486,35 @@ def test_default_variant_follows_the_lattice():
     assert 0 <= sel < n and wide.variant_info(sel)["fuse"] == 3
 
 
+def test_default_variant_at_512_cubed():
+    """3D N = 512 static boxes start on the tiles the autotuner picks there:
+    FP32 the 1024-thread three-step tile in 256-plane chunks (one block per
+    CU), FP64 the direct-load two-step tile (prefetch kind 4) in 256-plane
+    chunks; a 20-step call on the default equals the general kernel's."""
+    kw = params(D=3, readxns=-2, nGrids=512, L=64.0)
+    want = {"fp32": dict(fuse=3, nthr=1024, TR=4, NI=256, pf=2, nt=1),
+            "fp64": dict(fuse=2, nthr=768, TR=4, NI=256, pf=4, nt=1)}
+    for precision, w in want.items():
+        ref = cb.Bubble(precision=precision, **kw)
+        ref.select_kernel(-1)
+        ref.step(20)
+        phi0, pi0 = ref.fields()
+        e0 = ref.energy_row()
+        ref.close()
+        g = cb.Bubble(precision=precision, **kw)
+        n, sel = g.kernel_variants()
+        assert 0 <= sel < n
+        info = g.variant_info(sel)
+        assert {k: info[k] for k in w} == w, (precision, info)
+        g.step(20)
+        phi, pi = g.fields()
+        np.testing.assert_array_equal(phi, phi0)
+        np.testing.assert_array_equal(pi, pi0)
+        assert np.max(row_errors(e0[None], g.energy_row()[None])) <= 1e-13
+        g.close()
+        print(precision, "default", info, "bit-identical", flush=True)
+
+
 def test_spectrum_with_more_bins_than_64k_of_lds_matches_oracle():
     """2D N = 4096 has 2896 |k| bins: the binning histogram (3 doubles per
     bin) needs more than 64 KiB of LDS per block, which gfx950 allows."""
